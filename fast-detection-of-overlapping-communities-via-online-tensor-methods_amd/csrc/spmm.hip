@@ -1,0 +1,163 @@
+// CSR x dense SpMM for gfx950: Y = alpha * G * B, FP32, row-major B and Y.
+//
+// Every sparse-times-dense product of the pipeline goes through this kernel: the randomized
+// range finder of P_BC = G_XB^T G_XC / n_X, the Z_B / Z_C factors, the whitened neighbour
+// vectors y_A, y_B, y_C and the four Pi products. The reference forms these as dense n x n
+// DGEMMs (culaBlockDgemm, gemm_pinv_nys.c:27-145; pre_proc.c:154-366; post_proc.c:87-106);
+// here the adjacency blocks stay in CSR and B is a tall n x r matrix.
+//
+// Mapping: one wave64 per output row. The row's column indices are read 64 at a time with one
+// coalesced load and broadcast with ds_bpermute (__shfl). Each lane owns float4 column chunks of
+// the dense rows it gathers, so one gather of a B row is a single contiguous 16*64-byte access
+// per wave instruction. When r/4 < 64 the wave is split into G = 64/LPG lane groups that gather
+// G different nonzeros at once, then reduced across groups with butterfly shuffles. The nonzero
+// loop is unrolled so that several independent B-row gathers are in flight per wave -- the
+// kernel is HBM-bound (each nonzero moves 4r bytes of B) and needs that memory-level
+// parallelism to approach the bandwidth roofline.
+#include "common.h"
+
+namespace mmsb {
+
+template <int LPG, int P, bool VALS>
+__global__ __launch_bounds__(256) void spmm_csr_rowwave(int32_t m, const int32_t* __restrict__ rowptr,
+                                                        const int32_t* __restrict__ colidx,
+                                                        const float* __restrict__ vals, float alpha,
+                                                        const float* __restrict__ B, int32_t ldb,
+                                                        int32_t r, float* __restrict__ Y, int32_t ldy) {
+  const int32_t r4 = (r + 3) >> 2;
+  constexpr int G = 64 / LPG;  // nonzeros gathered concurrently by one wave
+  const int lane = threadIdx.x & 63;
+  const int64_t row = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (row >= m) return;  // uniform per wave
+  const int grp = lane / LPG;
+  const int gl = lane % LPG;
+
+  floatx4 acc[P];
+#pragma unroll
+  for (int p = 0; p < P; ++p) acc[p] = floatx4{0.f, 0.f, 0.f, 0.f};
+
+  const int32_t beg = rowptr[row];
+  const int32_t end = rowptr[row + 1];
+  for (int32_t base = beg; base < end; base += 64) {
+    const int32_t cnt = min(64, end - base);
+    const int32_t my_c = (lane < cnt) ? colidx[base + lane] : 0;
+    float my_v = 1.f;
+    if (VALS) my_v = (lane < cnt) ? vals[base + lane] : 0.f;
+    // Unrolled by U groups of G nonzeros: U*P independent 16-byte gathers in flight per lane.
+    constexpr int U = (P >= 8) ? 1 : 8 / P;
+    int32_t j = 0;
+    for (; j + U * G <= cnt; j += U * G) {
+      int32_t c[U];
+      float v[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        c[u] = __shfl(my_c, j + u * G + grp);
+        v[u] = VALS ? __shfl(my_v, j + u * G + grp) : 1.f;
+      }
+      floatx4 x[U][P];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const floatx4* brow = reinterpret_cast<const floatx4*>(B + (int64_t)c[u] * ldb);
+#pragma unroll
+        for (int p = 0; p < P; ++p) {
+          const int q = gl + p * LPG;
+          x[u][p] = (q < r4) ? brow[q] : floatx4{0.f, 0.f, 0.f, 0.f};
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+#pragma unroll
+        for (int p = 0; p < P; ++p) acc[p] += VALS ? v[u] * x[u][p] : x[u][p];
+    }
+    for (; j < cnt; j += G) {
+      const int32_t jj = j + grp;
+      const int32_t c = __shfl(my_c, jj < 64 ? jj : 63);
+      const float v = VALS ? __shfl(my_v, jj < 64 ? jj : 63) : 1.f;
+      if (jj < cnt) {
+        const floatx4* brow = reinterpret_cast<const floatx4*>(B + (int64_t)c * ldb);
+#pragma unroll
+        for (int p = 0; p < P; ++p) {
+          const int q = gl + p * LPG;
+          if (q < r4) acc[p] += VALS ? v * brow[q] : brow[q];
+        }
+      }
+    }
+  }
+  // Reduce the G lane groups (lanes gl, gl+LPG, gl+2*LPG, ...) onto group 0.
+#pragma unroll
+  for (int off = LPG; off < 64; off <<= 1) {
+#pragma unroll
+    for (int p = 0; p < P; ++p) {
+      acc[p].x += __shfl_xor(acc[p].x, off);
+      acc[p].y += __shfl_xor(acc[p].y, off);
+      acc[p].z += __shfl_xor(acc[p].z, off);
+      acc[p].w += __shfl_xor(acc[p].w, off);
+    }
+  }
+  if (grp == 0) {
+    float* yrow = Y + row * (int64_t)ldy;
+#pragma unroll
+    for (int p = 0; p < P; ++p) {
+      const int q = gl + p * LPG;
+      const floatx4 o = alpha * acc[p];
+      if (4 * q + 4 <= r) {
+        __builtin_nontemporal_store(o, reinterpret_cast<floatx4*>(yrow) + q);
+      } else if (q < r4) {  // last partial chunk: columns >= r are left untouched
+        yrow[4 * q] = o.x;
+        if (4 * q + 1 < r) yrow[4 * q + 1] = o.y;
+        if (4 * q + 2 < r) yrow[4 * q + 2] = o.z;
+      }
+    }
+  }
+}
+
+template <int LPG, int P>
+static int launch_spmm(hipStream_t s, int32_t m, const int32_t* rowptr, const int32_t* colidx,
+                       const float* vals, float alpha, const float* B, int32_t ldb, int32_t r,
+                       float* Y, int32_t ldy) {
+  const int waves_per_block = 4;
+  const dim3 grid((unsigned)ceil_div(m, waves_per_block));
+  if (vals)
+    hipLaunchKernelGGL((spmm_csr_rowwave<LPG, P, true>), grid, dim3(64 * waves_per_block), 0, s, m,
+                       rowptr, colidx, vals, alpha, B, ldb, r, Y, ldy);
+  else
+    hipLaunchKernelGGL((spmm_csr_rowwave<LPG, P, false>), grid, dim3(64 * waves_per_block), 0, s, m,
+                       rowptr, colidx, vals, alpha, B, ldb, r, Y, ldy);
+  MMSB_LAUNCH_CHECK();
+  return MMSB_OK;
+}
+
+int spmm_f32(hipStream_t s, int32_t m, int32_t n, int64_t nnz, const int32_t* rowptr,
+             const int32_t* colidx, const float* vals, float alpha, const float* B, int32_t ldb,
+             int32_t r, float* Y, int32_t ldy) {
+  MMSB_CHECK_ARG(m >= 0 && n >= 0 && nnz >= 0 && nnz < (int64_t)INT32_MAX, "spmm: bad sizes m=%d n=%d nnz=%lld", m, n, (long long)nnz);
+  MMSB_CHECK_ARG(r >= 1 && r <= ldb && r <= ldy, "spmm: need 1 <= r <= ldb, ldy (r=%d ldb=%d ldy=%d)", r, ldb, ldy);
+  MMSB_CHECK_ARG(ldb % 4 == 0 && ldy % 4 == 0, "spmm: ldb and ldy must be multiples of 4");
+  MMSB_CHECK_ARG(r <= 4096, "spmm: r=%d above the supported 4096", r);
+  if (m == 0) return MMSB_OK;
+  MMSB_CHECK_ARG(rowptr && Y && (nnz == 0 || (colidx && B)), "spmm: null pointer");
+  MMSB_CHECK_ARG(aligned16(B) && aligned16(Y), "spmm: B and Y must be 16-byte aligned");
+  const int32_t r4 = (r + 3) / 4;
+  if (r4 <= 1) return launch_spmm<1, 1>(s, m, rowptr, colidx, vals, alpha, B, ldb, r, Y, ldy);
+  if (r4 <= 2) return launch_spmm<2, 1>(s, m, rowptr, colidx, vals, alpha, B, ldb, r, Y, ldy);
+  if (r4 <= 4) return launch_spmm<4, 1>(s, m, rowptr, colidx, vals, alpha, B, ldb, r, Y, ldy);
+  if (r4 <= 8) return launch_spmm<8, 1>(s, m, rowptr, colidx, vals, alpha, B, ldb, r, Y, ldy);
+  if (r4 <= 16) return launch_spmm<16, 1>(s, m, rowptr, colidx, vals, alpha, B, ldb, r, Y, ldy);
+  if (r4 <= 32) return launch_spmm<32, 1>(s, m, rowptr, colidx, vals, alpha, B, ldb, r, Y, ldy);
+  if (r4 <= 64) return launch_spmm<64, 1>(s, m, rowptr, colidx, vals, alpha, B, ldb, r, Y, ldy);
+  if (r4 <= 128) return launch_spmm<64, 2>(s, m, rowptr, colidx, vals, alpha, B, ldb, r, Y, ldy);
+  if (r4 <= 192) return launch_spmm<64, 3>(s, m, rowptr, colidx, vals, alpha, B, ldb, r, Y, ldy);
+  if (r4 <= 256) return launch_spmm<64, 4>(s, m, rowptr, colidx, vals, alpha, B, ldb, r, Y, ldy);
+  if (r4 <= 512) return launch_spmm<64, 8>(s, m, rowptr, colidx, vals, alpha, B, ldb, r, Y, ldy);
+  return launch_spmm<64, 16>(s, m, rowptr, colidx, vals, alpha, B, ldb, r, Y, ldy);
+}
+
+}  // namespace mmsb
+
+extern "C" int mmsb_spmm_f32(void* stream, int32_t m, int32_t n, int64_t nnz, const int32_t* rowptr,
+                             const int32_t* colidx, const float* vals, float alpha, const float* B,
+                             int32_t ldb, int32_t r, float* Y, int32_t ldy) {
+  mmsb::clear_error();
+  return mmsb::spmm_f32(static_cast<hipStream_t>(stream), m, n, nnz, rowptr, colidx, vals, alpha, B,
+                        ldb, r, Y, ldy);
+}

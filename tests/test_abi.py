@@ -1,0 +1,47 @@
+"""CPU checks of the C-ABI boundary: the in-tree library loads and exports every function
+declared in include/*.h, and the Python binding declares a signature for each of them."""
+import ctypes
+import glob
+import os
+import re
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    names = set()
+    for h in glob.glob(os.path.join(ROOT, "include", "*.h")):
+        text = open(h).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        text = re.sub(r"//[^\n]*", "", text)
+        text = re.sub(r"typedef[^;]*;", "", text, flags=re.S)
+        for m in re.finditer(r"^\s*(?:const\s+)?[A-Za-z_]\w*[\s\*]+\**\s*([A-Za-z_]\w*)\s*\(", text, flags=re.M):
+            names.add(m.group(1))
+    return names
+
+
+def test_header_parses():
+    names = declared_functions()
+    assert "mmsb_spmm_f32" in names and "mmsb_last_error" in names
+
+
+def test_library_exports_every_declared_symbol(mmsb):
+    lib = ctypes.CDLL(mmsb.LIB_PATH)
+    missing = [n for n in sorted(declared_functions()) if not hasattr(lib, n)]
+    assert not missing, f"declared but not exported: {missing}"
+
+
+def test_binding_covers_header(mmsb):
+    assert declared_functions() <= set(mmsb.SIGNATURES), declared_functions() - set(mmsb.SIGNATURES)
+
+
+def test_version_and_error_strings(mmsb):
+    assert mmsb.lib.mmsb_version().decode().startswith("mmsb-amd")
+    assert isinstance(mmsb.last_error(), str)
+
+
+def test_argument_errors_are_status_codes_not_exits(mmsb):
+    # bad leading dimension: rejected on the host before any launch, no GPU needed
+    st = mmsb.lib.mmsb_spmm_f32(None, 4, 4, 0, None, None, None, 1.0, None, 3, 3, None, 3)
+    assert st == mmsb.MMSB_ERR_ARG
+    assert "multiples of 4" in mmsb.last_error()

@@ -1,0 +1,75 @@
+"""GPU parity of the CSR SpMM kernel (mmsb_spmm_f32) against the oracle's FP64 row loop."""
+import numpy as np
+import pytest
+
+from oracle import csr_spmm
+
+pytestmark = pytest.mark.gpu
+
+
+def random_csr(rng, m, n, density, empty_rows=()):
+    dense = rng.random((m, n)) < density
+    for i in empty_rows:
+        dense[i] = False
+    rowptr = np.zeros(m + 1, np.int32)
+    rowptr[1:] = np.cumsum(dense.sum(1))
+    colidx = np.nonzero(dense)[1].astype(np.int32)
+    return rowptr, colidx
+
+
+@pytest.mark.parametrize("r", [1, 3, 4, 8, 20, 64, 110, 256, 300, 510, 1010])
+@pytest.mark.parametrize("weighted", [False, True])
+def test_spmm_matches_oracle(mmsb, dev, r, weighted):
+    import torch
+    rng = np.random.default_rng(r * 7 + weighted)
+    m, n = 257, 190
+    rowptr, colidx = random_csr(rng, m, n, 0.08, empty_rows=(0, 5, 256))
+    # a long row exercising several 64-index chunks
+    vals = rng.standard_normal(colidx.size).astype(np.float32) if weighted else None
+    ld = (r + 3) // 4 * 4 + (4 if r % 2 else 0)
+    B = np.zeros((n, ld), np.float32)
+    B[:, :r] = rng.standard_normal((n, r))
+    Y = torch.full((m, ld), 7.0, dtype=torch.float32, device=dev)
+    t = lambda a: torch.from_numpy(a).to(dev)
+    bufs = [t(rowptr), t(colidx), t(vals) if weighted else None, t(B)]
+    mmsb.spmm_f32(*bufs, Y, r=r, n=n, alpha=0.5)
+    torch.cuda.synchronize()
+    ref = csr_spmm(rowptr, colidx, vals, B[:, :r].astype(np.float64), m, alpha=0.5)
+    got = Y.cpu().numpy()
+    np.testing.assert_allclose(got[:, :r], ref, rtol=1e-5, atol=1e-5)
+    assert np.all(got[:, r:] == 7.0), "columns beyond r must not be written"
+
+
+def test_spmm_long_rows(mmsb, dev):
+    import torch
+    rng = np.random.default_rng(11)
+    m, n, r = 9, 5000, 100
+    rowptr, colidx = random_csr(rng, m, n, 0.6)  # ~3000 nnz per row
+    B = rng.standard_normal((n, r)).astype(np.float32)
+    Y = torch.empty((m, r), dtype=torch.float32, device=dev)
+    t = lambda a: torch.from_numpy(a).to(dev)
+    bufs = [t(rowptr), t(colidx), None, t(B)]
+    mmsb.spmm_f32(*bufs, Y, r=r, n=n)
+    torch.cuda.synchronize()
+    np.testing.assert_allclose(Y.cpu().numpy(), csr_spmm(rowptr, colidx, None, B.astype(np.float64), m), rtol=1e-4, atol=1e-3)
+
+
+def test_spmm_row_window(mmsb, dev):
+    """A sub-range of rows via an offset rowptr (used by checkpointed STGD)."""
+    import torch
+    rng = np.random.default_rng(5)
+    m, n, r = 100, 80, 16
+    rowptr, colidx = random_csr(rng, m, n, 0.1)
+    B = rng.standard_normal((n, r)).astype(np.float32)
+    # every device buffer stays referenced until after the synchronize below
+    d_rowptr = torch.from_numpy(rowptr).to(dev)
+    d_colidx = torch.from_numpy(colidx).to(dev)
+    d_B = torch.from_numpy(B).to(dev)
+    Y = torch.empty((10, r), dtype=torch.float32, device=dev)
+    assert int(rowptr[50]) <= colidx.size and colidx.max() < n
+    st = mmsb.lib.mmsb_spmm_f32(None, 10, n, int(colidx.size), d_rowptr.data_ptr() + 4 * 40,
+                                d_colidx.data_ptr(), None, 1.0, d_B.data_ptr(), r, r, Y.data_ptr(), r)
+    mmsb.check(st)
+    torch.cuda.synchronize()
+    full = csr_spmm(rowptr, colidx, None, B.astype(np.float64), m)
+    np.testing.assert_allclose(Y.cpu().numpy(), full[40:50], rtol=1e-5, atol=1e-5)
