@@ -39,12 +39,88 @@ class MMSBError(RuntimeError):
 _c_int32_p = ctypes.POINTER(ctypes.c_int32)
 _vp = ctypes.c_void_p
 
+
+class CsrC(ctypes.Structure):
+    _fields_ = [("n_rows", ctypes.c_int32), ("n_cols", ctypes.c_int32), ("nnz", ctypes.c_int64),
+                ("rowptr", ctypes.POINTER(ctypes.c_int32)), ("colidx", ctypes.POINTER(ctypes.c_int32)),
+                ("vals", ctypes.POINTER(ctypes.c_float))]
+
+
+class GraphC(ctypes.Structure):
+    _fields_ = [("xa", CsrC), ("xb", CsrC), ("xc", CsrC), ("ba", CsrC), ("ca", CsrC),
+                ("n_nodes", ctypes.c_int64), ("n_edges", ctypes.c_int64), ("k", ctypes.c_int32),
+                ("theta", ctypes.POINTER(ctypes.c_double)), ("part", ctypes.POINTER(ctypes.c_int8)),
+                ("local", ctypes.POINTER(ctypes.c_int32)), ("pi", ctypes.POINTER(ctypes.c_double))]
+
+
+class GenParamsC(ctypes.Structure):
+    _fields_ = [("n_x", ctypes.c_int32), ("n_a", ctypes.c_int32), ("n_b", ctypes.c_int32), ("n_c", ctypes.c_int32),
+                ("k", ctypes.c_int32), ("alpha0", ctypes.c_double), ("p_in", ctypes.c_double),
+                ("p_out", ctypes.c_double), ("seed", ctypes.c_uint64), ("degree_corrected", ctypes.c_int32),
+                ("pareto_shape", ctypes.c_double), ("mode", ctypes.c_int32), ("threads", ctypes.c_int32)]
+
+
+GEN_MODES = {"auto": 0, "exact": 1, "balls": 2}
+
+
+class StpmConfigC(ctypes.Structure):
+    _fields_ = [("nx", ctypes.c_int32), ("na", ctypes.c_int32), ("nb", ctypes.c_int32), ("nc", ctypes.c_int32),
+                ("ncom", ctypes.c_int32), ("alpha_0", ctypes.c_double), ("learn_rate", ctypes.c_double),
+                ("thresh", ctypes.c_double), ("pinv_tol", ctypes.c_double), ("sgd_iter_min", ctypes.c_int64),
+                ("sgd_iter_max", ctypes.c_int64), ("binary", ctypes.c_int32), ("synthetic", ctypes.c_int32),
+                ("folder_name", ctypes.c_char * 512), ("file_prefix", ctypes.c_char * 64),
+                ("alpha_0_str", ctypes.c_char * 32), ("learn_rate_str", ctypes.c_char * 32),
+                ("minibatch", ctypes.c_int32), ("epochs", ctypes.c_int32), ("theta", ctypes.c_double),
+                ("renorm_every", ctypes.c_int32), ("ns_iters", ctypes.c_int32), ("precision", ctypes.c_int32),
+                ("polish_epochs", ctypes.c_int32), ("oversample", ctypes.c_int32), ("power_iters", ctypes.c_int32),
+                ("m2_scaling", ctypes.c_int32), ("member_tau", ctypes.c_double), ("seed", ctypes.c_uint64),
+                ("ba_first_is_row", ctypes.c_int32), ("feof_repeat", ctypes.c_int32), ("device", ctypes.c_int32),
+                ("verbose", ctypes.c_int32)]
+
+
+PREC = {"fp32": 0, "bf16": 1}
+M2_SCALING = {"paper": 0, "gpu": 1}
+BLOCKS = {"XA": 0, "XB": 1, "XC": 2, "BA": 3, "CA": 4}
+QUANTITIES = ["mu_a", "mu_b", "mu_c", "W", "U", "V", "sigma", "LB", "LC", "lam_m2", "YA", "YB", "YC",
+              "muw_a", "muw_b", "muw_c", "phi", "frob", "lambda", "lambda_p", "alpha_hat",
+              "pi_x", "pi_a", "pi_b", "pi_c", "pin_x", "pin_a", "pin_b", "pin_c",
+              "count_x", "count_a", "count_b", "count_c", "times"]
+Q = {name: i for i, name in enumerate(QUANTITIES)}
+
 # name -> (restype, argtypes); every symbol declared in include/mmsb_stpm.h appears here.
 SIGNATURES: dict = {
     "mmsb_last_error": (ctypes.c_char_p, []),
     "mmsb_version": (ctypes.c_char_p, []),
     "mmsb_spmm_f32": (ctypes.c_int, [_vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int64, _vp, _vp, _vp,
                                      ctypes.c_float, _vp, ctypes.c_int32, ctypes.c_int32, _vp, ctypes.c_int32]),
+    "mmsb_gen_params_default": (None, [ctypes.POINTER(GenParamsC)]),
+    "mmsb_generate": (ctypes.c_int, [ctypes.POINTER(GenParamsC), ctypes.c_int32, ctypes.POINTER(ctypes.POINTER(GraphC))]),
+    "mmsb_graph_free": (None, [ctypes.POINTER(GraphC)]),
+    "mmsb_read_coo": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                                     ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(CsrC)]),
+    "mmsb_csr_free": (None, [ctypes.POINTER(CsrC)]),
+    "mmsb_write_coo": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(CsrC)]),
+    "mmsb_write_values": (ctypes.c_int, [ctypes.c_char_p, _vp, ctypes.c_int64]),
+    "mmsb_write_pi_triplets": (ctypes.c_int, [ctypes.c_char_p, _vp, ctypes.c_int64, ctypes.c_int32]),
+    "stpm_config_default": (None, [ctypes.POINTER(StpmConfigC)]),
+    "mmsb_create": (ctypes.c_int, [ctypes.POINTER(StpmConfigC), ctypes.POINTER(_vp)]),
+    "mmsb_destroy": (None, [_vp]),
+    "mmsb_set_block": (ctypes.c_int, [_vp, ctypes.c_int32, ctypes.POINTER(CsrC)]),
+    "mmsb_read_inputs": (ctypes.c_int, [_vp]),
+    "mmsb_whiten": (ctypes.c_int, [_vp]),
+    "mmsb_set_phi": (ctypes.c_int, [_vp, _vp]),
+    "mmsb_set_yvectors": (ctypes.c_int, [_vp, ctypes.c_int64, ctypes.c_int32, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "mmsb_stgd": (ctypes.c_int, [_vp, ctypes.c_int64, ctypes.c_int32]),
+    "mmsb_eigen": (ctypes.c_int, [_vp]),
+    "mmsb_post": (ctypes.c_int, [_vp]),
+    "mmsb_write_outputs": (ctypes.c_int, [_vp]),
+    "mmsb_get": (ctypes.c_int, [_vp, ctypes.c_int32, _vp, ctypes.c_int64]),
+    "mmsb_get_size": (ctypes.c_int64, [_vp, ctypes.c_int32]),
+    "mmsb_sync": (ctypes.c_int, [_vp]),
+    "stpm_set_config": (ctypes.c_int, [ctypes.POINTER(StpmConfigC)]),
+    "pre_proc": (ctypes.c_int, []),
+    "GPU_Task": (_vp, [_vp]),
+    "post_proc": (ctypes.c_int, []),
 }
 
 
@@ -95,3 +171,145 @@ def spmm_f32(rowptr, colidx, vals, B, Y, r: int, n: int, alpha: float = 1.0, str
     check(lib.mmsb_spmm_f32(stream_handle(stream), m, n, colidx.numel(), ptr(rowptr), ptr(colidx), ptr(vals),
                             float(alpha), ptr(B), B.stride(0), r, ptr(Y), Y.stride(0)))
     return Y
+
+
+def _csr_to_numpy(c: CsrC):
+    import numpy as np
+    rowptr = np.ctypeslib.as_array(c.rowptr, shape=(c.n_rows + 1,)).copy()
+    colidx = np.ctypeslib.as_array(c.colidx, shape=(max(c.nnz, 1),))[: c.nnz].copy()
+    vals = None if not c.vals else np.ctypeslib.as_array(c.vals, shape=(max(c.nnz, 1),))[: c.nnz].copy()
+    return rowptr, colidx, vals, c.n_rows, c.n_cols
+
+
+def generate(n_x, n_a, n_b, n_c, k, alpha0, p_in, p_out, seed=0, degree_corrected=False, pareto_shape=2.5,
+             mode="auto", threads=0, keep_truth=False) -> dict:
+    """Seeded MMSB graph from the library generator; returns numpy copies of the five CSR blocks
+    (rowptr, colidx, vals, n_rows, n_cols) and, optionally, the ground truth."""
+    import numpy as np
+    p = GenParamsC(n_x, n_a, n_b, n_c, k, alpha0, p_in, p_out, seed, int(degree_corrected), pareto_shape,
+                   GEN_MODES[mode], threads)
+    g = ctypes.POINTER(GraphC)()
+    check(lib.mmsb_generate(ctypes.byref(p), int(keep_truth), ctypes.byref(g)))
+    try:
+        G = g.contents
+        out = {name: _csr_to_numpy(getattr(G, name.lower())) for name in ("XA", "XB", "XC", "BA", "CA")}
+        out["n_edges"] = G.n_edges
+        if keep_truth:
+            n = G.n_nodes
+            out["theta"] = np.ctypeslib.as_array(G.theta, shape=(n,)).copy()
+            out["part"] = np.ctypeslib.as_array(G.part, shape=(n,)).copy()
+            out["local"] = np.ctypeslib.as_array(G.local, shape=(n,)).copy()
+            out["pi"] = np.ctypeslib.as_array(G.pi, shape=(n * k,)).copy().reshape(n, k)
+        return out
+    finally:
+        lib.mmsb_graph_free(g)
+
+
+def config(**kw) -> StpmConfigC:
+    """stpm_config with the library defaults, overridden by keyword (strings for the char fields)."""
+    c = StpmConfigC()
+    lib.stpm_config_default(ctypes.byref(c))
+    for key, val in kw.items():
+        if key == "precision" and isinstance(val, str):
+            val = PREC[val]
+        if key == "m2_scaling" and isinstance(val, str):
+            val = M2_SCALING[val]
+        if isinstance(val, str):
+            val = val.encode()
+        if not hasattr(c, key):
+            raise AttributeError(key)
+        setattr(c, key, val)
+    return c
+
+
+def _csr_c(rowptr, colidx, vals, n_rows, n_cols):
+    import numpy as np
+    rp = np.ascontiguousarray(rowptr, np.int32)
+    ci = np.ascontiguousarray(colidx, np.int32)
+    v = None if vals is None else np.ascontiguousarray(vals, np.float32)
+    c = CsrC(n_rows, n_cols, ci.size, rp.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+             ci.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+             None if v is None else v.ctypes.data_as(ctypes.POINTER(ctypes.c_float)))
+    return c, (rp, ci, v)
+
+
+def read_coo(path, n_rows, n_cols, first_is_row=True, weighted=False, duplicates="one", feof_repeat=False):
+    dup = {"one": 0, "last": 1, "sum": 2}[duplicates]
+    m = CsrC()
+    check(lib.mmsb_read_coo(str(path).encode(), n_rows, n_cols, int(first_is_row), int(weighted), dup,
+                            int(feof_repeat), ctypes.byref(m)))
+    try:
+        return _csr_to_numpy(m)
+    finally:
+        lib.mmsb_csr_free(ctypes.byref(m))
+
+
+class Pipeline:
+    """Owns an mmsb_ctx: the whitening / STGD / post-processing state on one GPU."""
+
+    def __init__(self, cfg: StpmConfigC):
+        self.cfg = cfg
+        self.k = cfg.ncom
+        self.ctx = _vp()
+        check(lib.mmsb_create(ctypes.byref(cfg), ctypes.byref(self.ctx)))
+
+    def close(self):
+        if self.ctx:
+            lib.mmsb_destroy(self.ctx)
+            self.ctx = _vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_block(self, name, rowptr, colidx, vals, n_rows, n_cols):
+        c, keep = _csr_c(rowptr, colidx, vals, n_rows, n_cols)
+        check(lib.mmsb_set_block(self.ctx, BLOCKS[name], ctypes.byref(c)))
+
+    def set_graph(self, g: dict):
+        for name in BLOCKS:
+            if name in g and g[name] is not None:
+                self.set_block(name, *g[name])
+
+    def read_inputs(self):
+        check(lib.mmsb_read_inputs(self.ctx))
+
+    def whiten(self):
+        check(lib.mmsb_whiten(self.ctx))
+
+    def set_phi(self, phi):
+        import numpy as np
+        a = np.ascontiguousarray(phi, np.float64)
+        check(lib.mmsb_set_phi(self.ctx, a.ctypes.data))
+
+    def set_yvectors(self, ya, yb, yc, mua, mub, muc):
+        import numpy as np
+        arrs = [np.ascontiguousarray(x, np.float64) for x in (ya, yb, yc, mua, mub, muc)]
+        n, k = arrs[0].shape
+        check(lib.mmsb_set_yvectors(self.ctx, n, k, *[a.ctypes.data for a in arrs]))
+
+    def stgd(self, steps=-1, precision="fp32"):
+        check(lib.mmsb_stgd(self.ctx, int(steps), PREC[precision] if isinstance(precision, str) else precision))
+
+    def eigen(self):
+        check(lib.mmsb_eigen(self.ctx))
+
+    def post(self):
+        check(lib.mmsb_post(self.ctx))
+
+    def write_outputs(self):
+        check(lib.mmsb_write_outputs(self.ctx))
+
+    def get(self, name):
+        import numpy as np
+        q = Q[name]
+        n = lib.mmsb_get_size(self.ctx, q)
+        if n <= 0:
+            raise MMSBError(MMSB_ERR_STATE, f"{name} not available")
+        out = np.empty(n, np.float64)
+        check(lib.mmsb_get(self.ctx, q, out.ctypes.data, n))
+        if name in ("W", "U", "V", "LB", "LC", "YA", "YB", "YC", "phi") or name.startswith("pi"):
+            out = out.reshape(-1, self.k)
+        return out

@@ -1,0 +1,47 @@
+// Internal declarations of the dense FP64 kernels (dense.hip, jacobi.hip).
+#pragma once
+
+#include <hip/hip_runtime_api.h>
+#include <cstddef>
+#include <cstdint>
+
+namespace mmsb {
+
+int gaussian_fill(hipStream_t s, uint64_t seed, uint32_t index, int64_t n, int32_t r, float* out, int32_t ld);
+int csr_rowsum(hipStream_t s, int32_t m, const int32_t* rowptr, const float* vals, double scale, double* out);
+
+size_t tn_workspace_bytes(int64_t n, int32_t p, int32_t q);
+// C[p x q] = alpha * A^T B + beta * C (FP64 accumulate; `work` holds tn_workspace_bytes).
+template <typename TA, typename TB>
+int gemm_tn(hipStream_t s, int64_t n, int32_t p, int32_t q, double alpha, const TA* A, int32_t lda, const TB* B,
+            int32_t ldb, double beta, double* C, double* work);
+// Y[n x q] = alpha * X C + beta * Y, C FP64 [p x q] (row-major, or column-major when c_trans).
+template <typename TX, typename TY>
+int gemm_nn(hipStream_t s, int64_t n, int32_t p, int32_t q, double alpha, const TX* X, int32_t ldx, const double* C,
+            int32_t ldc, double beta, TY* Y, int32_t ldy, bool c_trans = false);
+
+// Upper Cholesky factor R of G + shift*trace(G)*I (G overwritten) and R^{-1}; status[0] != 0 on failure.
+int chol_inv(hipStream_t s, int32_t r, double* G, double shift, double* R, double* Rinv, int32_t* status);
+
+template <typename T>
+int col_sign(hipStream_t s, int64_t n, int32_t c, const T* M, int32_t ld, unsigned long long* best, double* sign);
+template <typename T>
+int scale_cols(hipStream_t s, int64_t n, int32_t c, T* M, int32_t ld, const double* sc);
+
+// One-sided Jacobi SVD of a square r x r FP64 matrix given column-major in A (overwritten):
+// A = U diag(sigma) V^T with sigma sorted descending; U, V column-major r x r.
+// work: jacobi_workspace_bytes(r). sweeps_out[0] = sweeps used (device int).
+size_t jacobi_workspace_bytes(int32_t r);
+int jacobi_svd(hipStream_t s, int32_t r, double* A, double* U, double* sigma, double* V, void* work, int32_t* sweeps_out);
+
+int transpose_f32(hipStream_t s, int64_t R, int32_t C, const float* in, int64_t ldi, float* out, int64_t ldo);
+int pinv_scale(hipStream_t s, int32_t k, const double* v, double shift, double tol, int mode, double* out, double* vout);
+int sym_shift(hipStream_t s, int32_t r, double* S, double* shift_out);
+int scale_rows(hipStream_t s, int32_t k, int32_t c, double* P, int32_t ld, const double* d, double alpha);
+int diag_axpy(hipStream_t s, int64_t n, int32_t c, const double* mu, double alpha, const float* X, int32_t ldx, float* O,
+              int32_t ldo);
+int copy2d_f32(hipStream_t s, int32_t rows, int32_t cols, const float* in, int32_t ldi, float* out, int32_t ldo);
+int spmm_f32(hipStream_t s, int32_t m, int32_t n, int64_t nnz, const int32_t* rowptr, const int32_t* colidx, const float* vals,
+             float alpha, const float* B, int32_t ldb, int32_t r, float* Y, int32_t ldy);
+
+}  // namespace mmsb

@@ -1,0 +1,513 @@
+// Dense building blocks of the whitening stage, FP64 arithmetic on gfx950.
+//
+// The reference runs every dense product through culaBlockDgemm (gemm_pinv_nys.c:27-145) on
+// n x n buffers and factors with culaDgeqrfp/culaDorgqr/culaDgesvd (gemm_pinv_nys.c:284-299,
+// 385-394). Here the dense operands are tall n x r (r = k + oversampling) or small r x r, so the
+// work is: tall-skinny A^T B (Gram-type, reduce over n), tall X * small C, a Cholesky factor with
+// its triangular inverse (CholeskyQR), the Gaussian test matrices and column-sign canonicalisation.
+// Tall operands are FP32 in HBM (the SpMM outputs) and are widened to FP64 on load; the
+// accumulation and every small matrix are FP64 -- MI355X runs vector FP64 FMA at the same
+// 78.6 TFLOP/s as its FP64 matrix cores, so these kernels use plain FP64 FMAs.
+#include <hip/hip_cooperative_groups.h>
+
+#include "common.h"
+#include "dense.h"
+#include "philox.h"
+
+namespace mmsb {
+
+// ---- Gaussian test matrix (replaces gen_rnd_mat, curand_rand_mat.cu:30-53) ------------------
+// z(i, j) = sqrt(-2 ln u1) cos(2 pi u2), (u1, u2) from Philox counter (i, j, kStreamOmega+index, 0).
+__global__ void gaussian_fill_kernel(uint64_t seed, uint32_t stream, int64_t n, int32_t r, float* out, int32_t ld) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n * r) return;
+  const int64_t i = t / r;
+  const int32_t j = (int32_t)(t - i * r);
+  const u32x4 w = draw(seed, stream, (uint32_t)i, (uint32_t)j, 0);
+  const double u1 = u53_open0(w.x, w.y);
+  const double u2 = u53(w.z, w.w);
+  const double z = sqrt(-2.0 * log(u1)) * cos(6.283185307179586476925 * u2);
+  out[i * ld + j] = (float)z;
+}
+
+int gaussian_fill(hipStream_t s, uint64_t seed, uint32_t index, int64_t n, int32_t r, float* out, int32_t ld) {
+  MMSB_CHECK_ARG(n >= 0 && r >= 1 && ld >= r && n < ((int64_t)1 << 32), "gaussian: bad shape");
+  if (n == 0) return MMSB_OK;
+  const int64_t total = n * r;
+  hipLaunchKernelGGL(gaussian_fill_kernel, dim3((unsigned)ceil_div(total, 256)), dim3(256), 0, s, seed,
+                     (uint32_t)kStreamOmega + index, n, r, out, ld);
+  MMSB_LAUNCH_CHECK();
+  return MMSB_OK;
+}
+
+// ---- column means from the transposed CSR: out[j] = scale * sum_{e in row j of G^T} val ------
+__global__ void csr_rowsum_kernel(int32_t m, const int32_t* __restrict__ rowptr, const float* __restrict__ vals,
+                                  double scale, double* __restrict__ out) {
+  const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  double acc;
+  if (vals) {
+    acc = 0.0;
+    for (int32_t e = rowptr[i]; e < rowptr[i + 1]; ++e) acc += (double)vals[e];
+  } else {
+    acc = (double)(rowptr[i + 1] - rowptr[i]);
+  }
+  out[i] = scale * acc;
+}
+
+int csr_rowsum(hipStream_t s, int32_t m, const int32_t* rowptr, const float* vals, double scale, double* out) {
+  if (m <= 0) return MMSB_OK;
+  hipLaunchKernelGGL(csr_rowsum_kernel, dim3((unsigned)ceil_div(m, 256)), dim3(256), 0, s, m, rowptr, vals, scale, out);
+  MMSB_LAUNCH_CHECK();
+  return MMSB_OK;
+}
+
+// ---- C[p x q] (FP64) = alpha * A^T B, A [n x p], B [n x q] row-major ------------------------
+// Grid: (tiles of C 64 x 64) x S row slices. Each workgroup stages 16-row slabs of A and B through
+// LDS as FP64 and accumulates a 4 x 4 register tile per thread; the S partial tiles are written to
+// a workspace and reduced in a fixed order by a second kernel (deterministic result).
+template <typename TA, typename TB>
+__global__ __launch_bounds__(256) void tn_partial_kernel(int64_t n, int32_t p, int32_t q, const TA* __restrict__ A,
+                                                         int32_t lda, const TB* __restrict__ B, int32_t ldb,
+                                                         int32_t slices, double* __restrict__ work) {
+  __shared__ double As[16][64 + 1];
+  __shared__ double Bs[16][64 + 1];
+  const int tp = blockIdx.x * 64, tq = blockIdx.y * 64;
+  const int slice = blockIdx.z;
+  const int64_t r0 = n * slice / slices, r1 = n * (slice + 1) / slices;
+  const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;  // output rows tp+4tx.., cols tq+4ty..
+  double acc[4][4] = {};
+  for (int64_t base = r0; base < r1; base += 16) {
+    // 16 x 64 slab of A and of B: 1024 elements each, 4 per thread
+    for (int e = threadIdx.x; e < 16 * 64; e += 256) {
+      const int rr = e >> 6, cc = e & 63;
+      const int64_t row = base + rr;
+      As[rr][cc] = (row < r1 && tp + cc < p) ? (double)A[row * lda + tp + cc] : 0.0;
+      Bs[rr][cc] = (row < r1 && tq + cc < q) ? (double)B[row * ldb + tq + cc] : 0.0;
+    }
+    __syncthreads();
+#pragma unroll 4
+    for (int rr = 0; rr < 16; ++rr) {
+      double a[4], b[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        a[u] = As[rr][4 * tx + u];
+        b[u] = Bs[rr][4 * ty + u];
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) acc[u][v] = fma(a[u], b[v], acc[u][v]);
+    }
+    __syncthreads();
+  }
+  double* w = work + (int64_t)slice * p * q;
+#pragma unroll
+  for (int u = 0; u < 4; ++u)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int i = tp + 4 * tx + u, j = tq + 4 * ty + v;
+      if (i < p && j < q) w[(int64_t)i * q + j] = acc[u][v];
+    }
+}
+
+__global__ void tn_reduce_kernel(int64_t pq, int32_t slices, const double* __restrict__ work, double alpha,
+                                 double beta, double* __restrict__ C) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= pq) return;
+  double acc = 0.0;
+  for (int s = 0; s < slices; ++s) acc += work[(int64_t)s * pq + t];
+  C[t] = (beta == 0.0) ? alpha * acc : alpha * acc + beta * C[t];
+}
+
+static int tn_slices(int64_t n, int32_t p, int32_t q) {
+  const int64_t tiles = ceil_div(p, 64) * ceil_div(q, 64);
+  int64_t s = ceil_div(2048, tiles);                 // aim for ~2048 workgroups
+  s = std::min<int64_t>(s, std::max<int64_t>(1, n / 256));
+  return (int)std::max<int64_t>(1, std::min<int64_t>(s, 512));
+}
+
+size_t tn_workspace_bytes(int64_t n, int32_t p, int32_t q) {
+  return sizeof(double) * (size_t)tn_slices(n, p, q) * (size_t)p * (size_t)q;
+}
+
+template <typename TA, typename TB>
+int gemm_tn(hipStream_t s, int64_t n, int32_t p, int32_t q, double alpha, const TA* A, int32_t lda, const TB* B,
+            int32_t ldb, double beta, double* C, double* work) {
+  MMSB_CHECK_ARG(n >= 0 && p >= 1 && q >= 1 && lda >= p && ldb >= q, "gemm_tn: bad shape");
+  const int slices = tn_slices(n, p, q);
+  dim3 grid((unsigned)ceil_div(p, 64), (unsigned)ceil_div(q, 64), (unsigned)slices);
+  hipLaunchKernelGGL((tn_partial_kernel<TA, TB>), grid, dim3(256), 0, s, n, p, q, A, lda, B, ldb, slices, work);
+  MMSB_LAUNCH_CHECK();
+  const int64_t pq = (int64_t)p * q;
+  hipLaunchKernelGGL(tn_reduce_kernel, dim3((unsigned)ceil_div(pq, 256)), dim3(256), 0, s, pq, slices, work, alpha,
+                     beta, C);
+  MMSB_LAUNCH_CHECK();
+  return MMSB_OK;
+}
+
+template int gemm_tn<float, float>(hipStream_t, int64_t, int32_t, int32_t, double, const float*, int32_t, const float*,
+                                   int32_t, double, double*, double*);
+template int gemm_tn<float, double>(hipStream_t, int64_t, int32_t, int32_t, double, const float*, int32_t,
+                                    const double*, int32_t, double, double*, double*);
+template int gemm_tn<double, double>(hipStream_t, int64_t, int32_t, int32_t, double, const double*, int32_t,
+                                     const double*, int32_t, double, double*, double*);
+template int gemm_tn<double, float>(hipStream_t, int64_t, int32_t, int32_t, double, const double*, int32_t,
+                                    const float*, int32_t, double, double*, double*);
+
+// ---- Y [n x q] = alpha * X [n x p] * C [p x q] (+ beta * Y), C FP64 ---------------------------
+// Workgroup: 64 rows x 64 columns of Y; the p-loop stages 16-wide slabs of X and C through LDS.
+template <typename TX, typename TY, bool CT>
+__global__ __launch_bounds__(256) void nn_kernel(int64_t n, int32_t p, int32_t q, double alpha, const TX* __restrict__ X,
+                                                 int32_t ldx, const double* __restrict__ C, int32_t ldc, double beta,
+                                                 TY* __restrict__ Y, int32_t ldy) {
+  __shared__ double Xs[64][16 + 1];
+  __shared__ double Cs[16][64 + 1];
+  const int64_t r0 = (int64_t)blockIdx.x * 64;
+  const int c0 = blockIdx.y * 64;
+  const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;  // rows r0 + ty*4.., cols c0 + tx*4..
+  double acc[4][4] = {};
+  for (int kb = 0; kb < p; kb += 16) {
+    for (int e = threadIdx.x; e < 64 * 16; e += 256) {
+      const int rr = e >> 4, kk = e & 15;
+      const int64_t row = r0 + rr;
+      Xs[rr][kk] = (row < n && kb + kk < p) ? (double)X[row * ldx + kb + kk] : 0.0;
+      const int kk2 = e >> 6, cc = e & 63;
+      const int64_t cidx = CT ? (int64_t)(c0 + cc) * ldc + kb + kk2 : (int64_t)(kb + kk2) * ldc + c0 + cc;
+      Cs[kk2][cc] = (kb + kk2 < p && c0 + cc < q) ? C[cidx] : 0.0;
+    }
+    __syncthreads();
+#pragma unroll 4
+    for (int kk = 0; kk < 16; ++kk) {
+      double a[4], b[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        a[u] = Xs[4 * ty + u][kk];
+        b[u] = Cs[kk][4 * tx + u];
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) acc[u][v] = fma(a[u], b[v], acc[u][v]);
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    const int64_t row = r0 + 4 * ty + u;
+    if (row >= n) continue;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int col = c0 + 4 * tx + v;
+      if (col < q) {
+        TY* y = Y + row * ldy + col;
+        const double o = alpha * acc[u][v];
+        *y = (TY)((beta == 0.0) ? o : o + beta * (double)*y);
+      }
+    }
+  }
+}
+
+template <typename TX, typename TY>
+int gemm_nn(hipStream_t s, int64_t n, int32_t p, int32_t q, double alpha, const TX* X, int32_t ldx, const double* C,
+            int32_t ldc, double beta, TY* Y, int32_t ldy, bool c_trans) {
+  MMSB_CHECK_ARG(n >= 0 && p >= 1 && q >= 1 && ldx >= p && ldy >= q && ldc >= (c_trans ? p : q), "gemm_nn: bad shape");
+  if (n == 0) return MMSB_OK;
+  dim3 grid((unsigned)ceil_div(n, 64), (unsigned)ceil_div(q, 64));
+  if (c_trans)
+    hipLaunchKernelGGL((nn_kernel<TX, TY, true>), grid, dim3(256), 0, s, n, p, q, alpha, X, ldx, C, ldc, beta, Y, ldy);
+  else
+    hipLaunchKernelGGL((nn_kernel<TX, TY, false>), grid, dim3(256), 0, s, n, p, q, alpha, X, ldx, C, ldc, beta, Y, ldy);
+  MMSB_LAUNCH_CHECK();
+  return MMSB_OK;
+}
+
+template int gemm_nn<float, float>(hipStream_t, int64_t, int32_t, int32_t, double, const float*, int32_t,
+                                   const double*, int32_t, double, float*, int32_t, bool);
+template int gemm_nn<double, double>(hipStream_t, int64_t, int32_t, int32_t, double, const double*, int32_t,
+                                     const double*, int32_t, double, double*, int32_t, bool);
+template int gemm_nn<float, double>(hipStream_t, int64_t, int32_t, int32_t, double, const float*, int32_t,
+                                    const double*, int32_t, double, double*, int32_t, bool);
+template int gemm_nn<double, float>(hipStream_t, int64_t, int32_t, int32_t, double, const double*, int32_t,
+                                    const double*, int32_t, double, float*, int32_t, bool);
+
+// ---- Cholesky of G + shift*trace(G)*I and the inverse of its upper factor ------------------
+// One workgroup of 1024 threads; G is r x r row-major FP64 in global memory (L2-resident).
+// Right-looking Crout on the lower triangle, then R = L^T and R^{-1} by column-parallel back
+// substitution. status[0] = 0 on success, j+1 if the pivot of column j is not positive.
+__global__ __launch_bounds__(1024) void chol_inv_kernel(int32_t r, double* __restrict__ G, double shift,
+                                                        double* __restrict__ R, double* __restrict__ Rinv,
+                                                        int32_t* __restrict__ status) {
+  __shared__ double red[1024];
+  __shared__ double piv;
+  __shared__ int bad;
+  const int t = threadIdx.x;
+  double tr = 0.0;
+  for (int i = t; i < r; i += blockDim.x) tr += G[(int64_t)i * r + i];
+  red[t] = tr;
+  if (t == 0) bad = 0;
+  __syncthreads();
+  for (int off = blockDim.x / 2; off > 0; off >>= 1) {
+    if (t < off) red[t] += red[t + off];
+    __syncthreads();
+  }
+  double s = shift * red[0];
+  if (!(s > 0.0)) s = 1e-300;
+  for (int i = t; i < r; i += blockDim.x) G[(int64_t)i * r + i] += s;
+  __syncthreads();
+  for (int j = 0; j < r; ++j) {
+    if (t == 0) {
+      const double d = G[(int64_t)j * r + j];
+      if (!(d > 0.0)) {
+        bad = j + 1;
+        piv = 1.0;
+      } else {
+        piv = sqrt(d);
+      }
+      G[(int64_t)j * r + j] = piv;
+    }
+    __syncthreads();
+    const double pj = piv;
+    for (int i = j + 1 + t; i < r; i += blockDim.x) G[(int64_t)i * r + j] /= pj;  // column j of L
+    __syncthreads();
+    // trailing update of the lower triangle: G[i][l] -= L[i][j] L[l][j], j < l <= i
+    const int m = r - j - 1;
+    const int64_t cnt = (int64_t)m * (m + 1) / 2;
+    for (int64_t e = t; e < cnt; e += blockDim.x) {
+      // e -> (ii, ll) with 0 <= ll <= ii < m, row-wise enumeration of the lower triangle
+      int ii = (int)((sqrt(8.0 * (double)e + 1.0) - 1.0) * 0.5);
+      while ((int64_t)ii * (ii + 1) / 2 > e) --ii;
+      while ((int64_t)(ii + 1) * (ii + 2) / 2 <= e) ++ii;
+      const int ll = (int)(e - (int64_t)ii * (ii + 1) / 2);
+      const int i = j + 1 + ii, l = j + 1 + ll;
+      G[(int64_t)i * r + l] -= G[(int64_t)i * r + j] * G[(int64_t)l * r + j];
+    }
+    __syncthreads();
+  }
+  // R = L^T (upper), zero below the diagonal
+  for (int64_t e = t; e < (int64_t)r * r; e += blockDim.x) {
+    const int i = (int)(e / r), l = (int)(e % r);
+    R[e] = (l >= i) ? G[(int64_t)l * r + i] : 0.0;
+  }
+  __syncthreads();
+  // Rinv: solve R X = I column by column (one thread per column), upper triangular
+  for (int c = t; c < r; c += blockDim.x) {
+    for (int i = r - 1; i >= 0; --i) {
+      double acc = (i == c) ? 1.0 : 0.0;
+      if (i <= c) {
+        for (int l = i + 1; l <= c; ++l) acc -= R[(int64_t)i * r + l] * Rinv[(int64_t)l * r + c];
+        Rinv[(int64_t)i * r + c] = acc / R[(int64_t)i * r + i];
+      } else {
+        Rinv[(int64_t)i * r + c] = 0.0;
+      }
+    }
+  }
+  if (t == 0) status[0] = bad;
+}
+
+int chol_inv(hipStream_t s, int32_t r, double* G, double shift, double* R, double* Rinv, int32_t* status) {
+  MMSB_CHECK_ARG(r >= 1 && r <= 4096, "chol_inv: r=%d out of range", r);
+  hipLaunchKernelGGL(chol_inv_kernel, dim3(1), dim3(1024), 0, s, r, G, shift, R, Rinv, status);
+  MMSB_LAUNCH_CHECK();
+  return MMSB_OK;
+}
+
+__global__ void fill_u64_kernel(int64_t n, unsigned long long* p, unsigned long long v) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) p[t] = v;
+}
+
+// ---- column sign canonicalisation: largest-|.| entry of each column made positive ----------
+// key = (|v| bits << 32) | ~row: atomicMax picks the largest magnitude, then the smallest row.
+template <typename T>
+__global__ void colmax_kernel(int64_t n, int32_t c, const T* __restrict__ M, int32_t ld,
+                              unsigned long long* __restrict__ best) {
+  const int col = blockIdx.y * blockDim.x + threadIdx.x;
+  if (col >= c) return;
+  const int64_t r0 = n * blockIdx.x / gridDim.x, r1 = n * (blockIdx.x + 1) / gridDim.x;
+  unsigned long long b = 0;
+  for (int64_t i = r0; i < r1; ++i) {
+    const float a = fabsf((float)M[i * ld + col]);
+    const unsigned long long key = ((unsigned long long)__float_as_uint(a) << 32) | (0xFFFFFFFFull - (unsigned long long)i);
+    b = key > b ? key : b;
+  }
+  atomicMax(best + col, b);
+}
+
+template <typename T>
+__global__ void colsign_kernel(int32_t c, const T* __restrict__ M, int32_t ld, const unsigned long long* __restrict__ best,
+                               double* __restrict__ sign) {
+  const int col = blockIdx.x * blockDim.x + threadIdx.x;
+  if (col >= c) return;
+  const int64_t row = (int64_t)(0xFFFFFFFFull - (best[col] & 0xFFFFFFFFull));
+  sign[col] = ((double)M[row * ld + col] < 0.0) ? -1.0 : 1.0;
+}
+
+template <typename T>
+__global__ void scale_cols_kernel(int64_t n, int32_t c, T* __restrict__ M, int32_t ld, const double* __restrict__ sc) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n * c) return;
+  const int64_t i = t / c;
+  const int j = (int)(t - i * c);
+  M[i * ld + j] = (T)((double)M[i * ld + j] * sc[j]);
+}
+
+template <typename T>
+int col_sign(hipStream_t s, int64_t n, int32_t c, const T* M, int32_t ld, unsigned long long* best, double* sign) {
+  MMSB_CHECK_ARG(n >= 1 && n < 0xFFFFFFFFll && c >= 1, "col_sign: bad shape");
+  hipLaunchKernelGGL(fill_u64_kernel, dim3((unsigned)ceil_div(c, 256)), dim3(256), 0, s, (int64_t)c, best, 0ull);
+  MMSB_LAUNCH_CHECK();
+  const int slabs = (int)std::min<int64_t>(256, ceil_div(n, 1024));
+  dim3 grid((unsigned)slabs, (unsigned)ceil_div(c, 64));
+  hipLaunchKernelGGL((colmax_kernel<T>), grid, dim3(64), 0, s, n, c, M, ld, best);
+  MMSB_LAUNCH_CHECK();
+  hipLaunchKernelGGL((colsign_kernel<T>), dim3((unsigned)ceil_div(c, 64)), dim3(64), 0, s, c, M, ld, best, sign);
+  MMSB_LAUNCH_CHECK();
+  return MMSB_OK;
+}
+
+template <typename T>
+int scale_cols(hipStream_t s, int64_t n, int32_t c, T* M, int32_t ld, const double* sc) {
+  if (n <= 0 || c <= 0) return MMSB_OK;
+  hipLaunchKernelGGL((scale_cols_kernel<T>), dim3((unsigned)ceil_div(n * c, 256)), dim3(256), 0, s, n, c, M, ld, sc);
+  MMSB_LAUNCH_CHECK();
+  return MMSB_OK;
+}
+
+template int col_sign<float>(hipStream_t, int64_t, int32_t, const float*, int32_t, unsigned long long*, double*);
+template int col_sign<double>(hipStream_t, int64_t, int32_t, const double*, int32_t, unsigned long long*, double*);
+template int scale_cols<float>(hipStream_t, int64_t, int32_t, float*, int32_t, const double*);
+template int scale_cols<double>(hipStream_t, int64_t, int32_t, double*, int32_t, const double*);
+
+// ---- small helpers ---------------------------------------------------------------------------
+// out[c][r] = in[r][c] for an R x C FP32 matrix (32 x 32 LDS tiles)
+__global__ void transpose_kernel(int64_t R, int32_t C, const float* __restrict__ in, int64_t ldi, float* __restrict__ out,
+                                 int64_t ldo) {
+  __shared__ float t[32][33];
+  const int64_t r0 = (int64_t)blockIdx.x * 32;
+  const int c0 = blockIdx.y * 32;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 256 threads: 32 x 8
+  for (int j = ty; j < 32; j += 8) {
+    const int64_t r = r0 + j;
+    const int c = c0 + tx;
+    t[j][tx] = (r < R && c < C) ? in[r * ldi + c] : 0.f;
+  }
+  __syncthreads();
+  for (int j = ty; j < 32; j += 8) {
+    const int c = c0 + j;
+    const int64_t r = r0 + tx;
+    if (c < C && r < R) out[(int64_t)c * ldo + r] = t[tx][j];
+  }
+}
+
+int transpose_f32(hipStream_t s, int64_t R, int32_t C, const float* in, int64_t ldi, float* out, int64_t ldo) {
+  if (R <= 0 || C <= 0) return MMSB_OK;
+  hipLaunchKernelGGL(transpose_kernel, dim3((unsigned)ceil_div(R, 32), (unsigned)ceil_div(C, 32)), dim3(256), 0, s, R, C, in,
+                     ldi, out, ldo);
+  MMSB_LAUNCH_CHECK();
+  return MMSB_OK;
+}
+
+// sinv[j] = sig[j] > tol * sig[0] ? 1 / sig[j] : 0      (pinv_vector, CPU Spectral.cpp:50-57)
+// mode 1: out[j] = lam[j] > tol * lam[0] ? 1 / sqrt(lam[j]) : 0 (W = U Lambda^-1/2, CPU Util.cpp:119)
+__global__ void pinv_scale_kernel(int32_t k, const double* __restrict__ v, double shift, double tol, int mode,
+                                  double* __restrict__ out, double* __restrict__ vout) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= k) return;
+  const double x = v[j] - shift, x0 = v[0] - shift;
+  if (vout) vout[j] = x;
+  const bool keep = x > tol * (x0 > 0.0 ? x0 : 0.0) && x > 0.0;
+  out[j] = keep ? (mode == 0 ? 1.0 / x : 1.0 / sqrt(x)) : 0.0;
+}
+
+int pinv_scale(hipStream_t s, int32_t k, const double* v, double shift, double tol, int mode, double* out, double* vout) {
+  hipLaunchKernelGGL(pinv_scale_kernel, dim3((unsigned)ceil_div(k, 256)), dim3(256), 0, s, k, v, shift, tol, mode, out, vout);
+  MMSB_LAUNCH_CHECK();
+  return MMSB_OK;
+}
+
+// S <- (S + S^T)/2 + c I with c = ||S||_F (makes the Rayleigh-Ritz matrix positive semidefinite
+// so that its SVD is its eigendecomposition); c written to shift_out. One workgroup.
+__global__ __launch_bounds__(1024) void sym_shift_kernel(int32_t r, double* __restrict__ S, double* __restrict__ shift_out) {
+  __shared__ double red[1024];
+  const int t = threadIdx.x;
+  const int64_t rr = (int64_t)r * r;
+  double acc = 0.0;
+  for (int64_t e = t; e < rr; e += blockDim.x) acc += S[e] * S[e];
+  red[t] = acc;
+  __syncthreads();
+  for (int off = blockDim.x / 2; off > 0; off >>= 1) {
+    if (t < off) red[t] += red[t + off];
+    __syncthreads();
+  }
+  const double c = sqrt(red[0]);
+  for (int64_t e = t; e < rr; e += blockDim.x) {
+    const int i = (int)(e / r), j = (int)(e % r);
+    if (i < j) {
+      const double v = 0.5 * (S[e] + S[(int64_t)j * r + i]);
+      S[e] = v;
+      S[(int64_t)j * r + i] = v;
+    }
+  }
+  __syncthreads();
+  for (int i = t; i < r; i += blockDim.x) S[(int64_t)i * r + i] += c;
+  if (t == 0) shift_out[0] = c;
+}
+
+int sym_shift(hipStream_t s, int32_t r, double* S, double* shift_out) {
+  hipLaunchKernelGGL(sym_shift_kernel, dim3(1), dim3(1024), 0, s, r, S, shift_out);
+  MMSB_LAUNCH_CHECK();
+  return MMSB_OK;
+}
+
+// P[i][j] *= alpha * d[i]  (k x c FP64 row-major)
+__global__ void scale_rows_kernel(int32_t k, int32_t c, double* __restrict__ P, int32_t ld, const double* __restrict__ d,
+                                  double alpha) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (int64_t)k * c) return;
+  const int i = (int)(t / c), j = (int)(t % c);
+  P[(int64_t)i * ld + j] *= alpha * d[i];
+}
+
+int scale_rows(hipStream_t s, int32_t k, int32_t c, double* P, int32_t ld, const double* d, double alpha) {
+  hipLaunchKernelGGL(scale_rows_kernel, dim3((unsigned)ceil_div((int64_t)k * c, 256)), dim3(256), 0, s, k, c, P, ld, d, alpha);
+  MMSB_LAUNCH_CHECK();
+  return MMSB_OK;
+}
+
+// OUT[i][j] += alpha * mu[i]^2 * X[i][j]   (the diagonal of mu mu^T added back, GPU M2 scaling)
+__global__ void diag_axpy_kernel(int64_t n, int32_t c, const double* __restrict__ mu, double alpha, const float* __restrict__ X,
+                                 int32_t ldx, float* __restrict__ O, int32_t ldo) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n * c) return;
+  const int64_t i = t / c;
+  const int j = (int)(t - i * c);
+  O[i * ldo + j] = (float)((double)O[i * ldo + j] + alpha * mu[i] * mu[i] * (double)X[i * ldx + j]);
+}
+
+int diag_axpy(hipStream_t s, int64_t n, int32_t c, const double* mu, double alpha, const float* X, int32_t ldx, float* O,
+              int32_t ldo) {
+  if (n <= 0) return MMSB_OK;
+  hipLaunchKernelGGL(diag_axpy_kernel, dim3((unsigned)ceil_div(n * c, 256)), dim3(256), 0, s, n, c, mu, alpha, X, ldx, O, ldo);
+  MMSB_LAUNCH_CHECK();
+  return MMSB_OK;
+}
+
+// copy a k x k FP32 block into a zero-padded kp x kp buffer (and back)
+__global__ void pad_copy_kernel(int32_t rows, int32_t cols, const float* __restrict__ in, int32_t ldi, float* __restrict__ out,
+                                int32_t ldo) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (int64_t)rows * cols) return;
+  const int i = (int)(t / cols), j = (int)(t % cols);
+  out[(int64_t)i * ldo + j] = in[(int64_t)i * ldi + j];
+}
+
+int copy2d_f32(hipStream_t s, int32_t rows, int32_t cols, const float* in, int32_t ldi, float* out, int32_t ldo) {
+  if (rows <= 0 || cols <= 0) return MMSB_OK;
+  hipLaunchKernelGGL(pad_copy_kernel, dim3((unsigned)ceil_div((int64_t)rows * cols, 256)), dim3(256), 0, s, rows, cols, in, ldi,
+                     out, ldo);
+  MMSB_LAUNCH_CHECK();
+  return MMSB_OK;
+}
+
+}  // namespace mmsb

@@ -1,0 +1,979 @@
+// Host orchestration of the MMSB tensor-method pipeline on one GPU: whitening, whitened neighbour
+// vectors, STGD, eigenvalues and Pi, behind the stpm.h-shaped configuration.
+//
+// The reference keeps every operand in process-global dense FP64 buffers sized at compile time
+// (pre_proc.c:29-42, stpm.c:40-47, post_proc.c:34-42) and moves tiles over PCIe for each product
+// (culaBlockDgemm, gemm_pinv_nys.c:27-145). Here a context owns device buffers: the adjacency
+// blocks in CSR (and the CSR of their transposes), tall n x r FP32 sketches, small r x r FP64
+// factors, and the STGD state; all products run in HBM on one stream.
+//
+// Whitening (pre_proc.c:151-366, with the intended mathematics of the Nystrom helpers):
+//   mu_* = column means                                             (pre_proc.c:154-162)
+//   P_BC = G_XB^T G_XC / n_X ~ U diag(sigma) V^T  (randomized SVD)  (pinv_nys_asym, gemm_pinv_nys.c:325-421)
+//   Z_B = P_AC P_BC^+ = L_B U^T,  L_B = G_XA^T G_XC V diag(1/sigma) / n_X   (pre_proc.c:215-218)
+//   Z_C = P_AB P_CB^+ = L_C V^T,  L_C = G_XA^T G_XB U diag(1/sigma) / n_X   (pre_proc.c:269)
+//   M2  = s_main (L_C Sigma L_B^T + L_B Sigma L_C^T)/2 - s_shift mu_A mu_A^T  (pre_proc.c:292-302,
+//         CPU Util.cpp:81-113), applied as an operator; top-k eigenpairs by a randomized range
+//         finder and Rayleigh-Ritz; W = U_k Lambda_k^-1/2           (nystrom_whitening, gemm_pinv_nys.c:207-322)
+//   y_A = W^T g_A, y_B = W^T Z_B g_B, y_C = W^T Z_C g_C             (pre_proc.c:339-366)
+// Orthonormal bases come from CholeskyQR2 (FP64 Gram, Cholesky and triangular inverse on the GPU),
+// small SVD / eigenproblems from the FP64 one-sided Jacobi kernel.
+#include <hip/hip_runtime_api.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "dense.h"
+#include "host_util.h"
+#include "mmsb_stpm.h"
+#include "post.h"
+#include "stgd.h"
+
+namespace mmsb {
+
+#define HCHK(expr)                                                                                     \
+  do {                                                                                                 \
+    hipError_t e_ = (expr);                                                                            \
+    if (e_ != hipSuccess)                                                                              \
+      return ::mmsb::set_error(MMSB_ERR_HIP, "%s:%d %s -> %s", __FILE__, __LINE__, #expr, hipGetErrorString(e_)); \
+  } while (0)
+#define TRY(expr)                                                                                      \
+  do {                                                                                                 \
+    int s_ = (expr);                                                                                   \
+    if (s_ != MMSB_OK) return s_;                                                                      \
+  } while (0)
+
+static constexpr double kCholShift = 1e-14;
+
+struct DevCsr {
+  int32_t n_rows = 0, n_cols = 0;
+  int64_t nnz = 0;
+  int32_t* rowptr = nullptr;
+  int32_t* colidx = nullptr;
+  float* vals = nullptr;
+};
+
+static int64_t r64(int64_t a, int64_t b) { return (a + b - 1) / b * b; }
+
+}  // namespace mmsb
+
+using namespace mmsb;
+
+struct mmsb_ctx {
+  stpm_config cfg{};
+  hipStream_t s = nullptr;
+  int32_t k = 0, r = 0, kp = 0, rld = 0, rmax = 0, b = 0, bl = 0;
+  int64_t n[4] = {0, 0, 0, 0};  // X, A, B, C
+  int64_t nmax = 0, n_pad = 0;
+  DevCsr blk[5], blkT[5];
+  bool have[5] = {false, false, false, false, false};
+  std::vector<void*> allocs;
+  // whitening scratch
+  float* T[4] = {nullptr, nullptr, nullptr, nullptr};
+  double *gram = nullptr, *R1 = nullptr, *R1i = nullptr, *R2 = nullptr, *R2i = nullptr, *Rq = nullptr;
+  double* work = nullptr;
+  size_t work_bytes = 0;
+  unsigned char* jwork = nullptr;
+  int32_t* dstat = nullptr;  // [8] chol status, jacobi sweeps
+  double *Ut = nullptr, *Vt = nullptr, *sig = nullptr, *sinv = nullptr, *sgn = nullptr, *shift = nullptr;
+  unsigned long long* best = nullptr;
+  double *P1 = nullptr, *P2 = nullptr, *mrow = nullptr, *S = nullptr, *E = nullptr, *E2 = nullptr, *lamv = nullptr,
+         *lamis = nullptr;
+  double* mu[3] = {nullptr, nullptr, nullptr};
+  float *W = nullptr, *U = nullptr, *V = nullptr, *LB = nullptr, *LC = nullptr, *MB = nullptr, *MC = nullptr;
+  double *LBW = nullptr, *LCW = nullptr;
+  bool whitened = false;
+  // neighbour vectors
+  float *YA = nullptr, *YB = nullptr, *YCt = nullptr;
+  double* muw[3] = {nullptr, nullptr, nullptr};
+  bool have_y = false;
+  // STGD
+  float* phi[2] = {nullptr, nullptr};
+  int cur = 0;
+  float *phit = nullptr, *gt = nullptr, *phia = nullptr, *h = nullptr, *cgt = nullptr, *cs_part = nullptr, *mab = nullptr;
+  double* frob = nullptr;
+  int64_t frob_cap = 0, steps_done = 0;
+  bf16_bits *YA16 = nullptr, *YB16 = nullptr, *YCt16 = nullptr, *cgt16 = nullptr;
+  float *phiT = nullptr, *phitT = nullptr, *phiaT = nullptr;
+  bool y16_valid = false;
+  bool have_phi = false;
+  // eigen / post
+  double *lam = nullptr, *lamp = nullptr, *ahat = nullptr, *phil = nullptr, *LBU = nullptr;
+  float *Upost = nullptr, *ZBtU = nullptr;
+  float* Pi[4] = {nullptr, nullptr, nullptr, nullptr};
+  float* PiN[4] = {nullptr, nullptr, nullptr, nullptr};
+  int32_t* cnt[4] = {nullptr, nullptr, nullptr, nullptr};
+  bool eig_done = false, post_done = false;
+  double times[4] = {0, 0, 0, 0};
+
+  void log(const char* fmt, ...) const {
+    if (!cfg.verbose) return;
+    va_list ap;
+    va_start(ap, fmt);
+    std::vprintf(fmt, ap);
+    va_end(ap);
+    std::fflush(stdout);
+  }
+};
+
+namespace mmsb {
+
+template <typename T>
+static int dalloc(mmsb_ctx* c, T*& p, size_t count) {
+  if (p) return MMSB_OK;
+  void* q = nullptr;
+  const size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
+  hipError_t e = hipMalloc(&q, bytes);
+  if (e != hipSuccess)
+    return set_error(MMSB_ERR_NOMEM, "hipMalloc(%zu bytes) failed: %s", bytes, hipGetErrorString(e));
+  HCHK(hipMemsetAsync(q, 0, bytes, c->s));
+  c->allocs.push_back(q);
+  p = static_cast<T*>(q);
+  return MMSB_OK;
+}
+
+static double now_s() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+// ---- CSR upload (with the transpose built on the host by a counting sort) -------------------
+static void host_transpose(const mmsb_csr& a, std::vector<int32_t>& rp, std::vector<int32_t>& ci, std::vector<float>& v) {
+  rp.assign((size_t)a.n_cols + 1, 0);
+  for (int64_t e = 0; e < a.nnz; ++e) rp[a.colidx[e] + 1]++;
+  for (int32_t j = 0; j < a.n_cols; ++j) rp[j + 1] += rp[j];
+  ci.resize((size_t)a.nnz);
+  if (a.vals) v.resize((size_t)a.nnz);
+  std::vector<int32_t> fill(rp.begin(), rp.end() - 1);
+  for (int32_t i = 0; i < a.n_rows; ++i)
+    for (int32_t e = a.rowptr[i]; e < a.rowptr[i + 1]; ++e) {
+      const int32_t j = a.colidx[e];
+      ci[fill[j]] = i;
+      if (a.vals) v[fill[j]] = a.vals[e];
+      ++fill[j];
+    }
+}
+
+static int upload_csr(mmsb_ctx* c, DevCsr& d, int32_t nr, int32_t nc, int64_t nnz, const int32_t* rp, const int32_t* ci,
+                      const float* v) {
+  d.n_rows = nr;
+  d.n_cols = nc;
+  d.nnz = nnz;
+  TRY(dalloc(c, d.rowptr, (size_t)nr + 1));
+  TRY(dalloc(c, d.colidx, (size_t)std::max<int64_t>(nnz, 1)));
+  HCHK(hipMemcpyAsync(d.rowptr, rp, sizeof(int32_t) * ((size_t)nr + 1), hipMemcpyHostToDevice, c->s));
+  if (nnz) HCHK(hipMemcpyAsync(d.colidx, ci, sizeof(int32_t) * nnz, hipMemcpyHostToDevice, c->s));
+  if (v) {
+    TRY(dalloc(c, d.vals, (size_t)std::max<int64_t>(nnz, 1)));
+    if (nnz) HCHK(hipMemcpyAsync(d.vals, v, sizeof(float) * nnz, hipMemcpyHostToDevice, c->s));
+  }
+  HCHK(hipStreamSynchronize(c->s));  // host arrays may be temporaries
+  return MMSB_OK;
+}
+
+static int spmm(mmsb_ctx* c, const DevCsr& g, float alpha, const float* B, int32_t ldb, int32_t r, float* Y, int32_t ldy) {
+  return spmm_f32(c->s, g.n_rows, g.n_cols, g.nnz, g.rowptr, g.colidx, g.vals, alpha, B, ldb, r, Y, ldy);
+}
+
+// ---- CholeskyQR2: Q = X R^{-1} (Q may alias X); optionally R = R2 R1 -------------------------
+static int orth(mmsb_ctx* c, float* X, int64_t n, int32_t cols, int32_t ld, float* Q, float* tmp, double* R_out,
+                int status_slot) {
+  hipStream_t s = c->s;
+  TRY((gemm_tn<float, float>(s, n, cols, cols, 1.0, X, ld, X, ld, 0.0, c->gram, c->work)));
+  TRY(chol_inv(s, cols, c->gram, kCholShift, c->R1, c->R1i, c->dstat + status_slot));
+  TRY((gemm_nn<float, float>(s, n, cols, cols, 1.0, X, ld, c->R1i, cols, 0.0, tmp, ld)));
+  TRY((gemm_tn<float, float>(s, n, cols, cols, 1.0, tmp, ld, tmp, ld, 0.0, c->gram, c->work)));
+  TRY(chol_inv(s, cols, c->gram, kCholShift, c->R2, c->R2i, c->dstat + status_slot + 1));
+  TRY((gemm_nn<float, float>(s, n, cols, cols, 1.0, tmp, ld, c->R2i, cols, 0.0, Q, ld)));
+  if (R_out) TRY((gemm_nn<double, double>(s, cols, cols, cols, 1.0, c->R2, cols, c->R1, cols, 0.0, R_out, cols)));
+  return MMSB_OK;
+}
+
+static int check_status(mmsb_ctx* c, const char* where) {
+  int32_t st[8];
+  HCHK(hipMemcpyAsync(st, c->dstat, sizeof(st), hipMemcpyDeviceToHost, c->s));
+  HCHK(hipStreamSynchronize(c->s));
+  for (int i = 0; i < 6; ++i)
+    if (st[i] != 0) return set_error(MMSB_ERR_NUMERIC, "%s: Cholesky pivot %d not positive (slot %d)", where, st[i], i);
+  return MMSB_OK;
+}
+
+// M2 applied to X [n_A x r]:  OUT = s_main/2 (L_C Sigma L_B^T X + L_B Sigma L_C^T X) - s_shift mu (mu^T X)
+//                                  [+ s_shift diag(mu^2) X in the GPU scaling]
+static int m2_apply(mmsb_ctx* c, const float* X, float* OUT) {
+  hipStream_t s = c->s;
+  const int64_t na = c->n[1];
+  const int32_t k = c->k, r = c->r, kp = c->kp, ld = c->rld;
+  const bool paper = c->cfg.m2_scaling == MMSB_M2_PAPER;
+  const double a0 = c->cfg.alpha_0;
+  const double s_main = paper ? a0 + 1.0 : 1.0;
+  const double s_shift = paper ? a0 : a0 / (a0 + 1.0);
+  TRY((gemm_tn<float, float>(s, na, k, r, 1.0, c->LB, kp, X, ld, 0.0, c->P1, c->work)));
+  TRY((gemm_tn<float, float>(s, na, k, r, 1.0, c->LC, kp, X, ld, 0.0, c->P2, c->work)));
+  TRY((gemm_tn<double, float>(s, na, 1, r, 1.0, c->mu[0], 1, X, ld, 0.0, c->mrow, c->work)));
+  TRY(scale_rows(s, k, r, c->P1, r, c->sig, 0.5 * s_main));
+  TRY(scale_rows(s, k, r, c->P2, r, c->sig, 0.5 * s_main));
+  TRY((gemm_nn<float, float>(s, na, k, r, 1.0, c->LC, kp, c->P1, r, 0.0, OUT, ld)));
+  TRY((gemm_nn<float, float>(s, na, k, r, 1.0, c->LB, kp, c->P2, r, 1.0, OUT, ld)));
+  TRY((gemm_nn<double, float>(s, na, 1, r, -s_shift, c->mu[0], 1, c->mrow, r, 1.0, OUT, ld)));
+  if (!paper) TRY(diag_axpy(s, na, r, c->mu[0], s_shift, X, ld, OUT, ld));
+  return MMSB_OK;
+}
+
+static int alloc_small(mmsb_ctx* c) {
+  const int32_t rm = c->rmax;
+  TRY(dalloc(c, c->gram, (size_t)rm * rm));
+  TRY(dalloc(c, c->R1, (size_t)rm * rm));
+  TRY(dalloc(c, c->R1i, (size_t)rm * rm));
+  TRY(dalloc(c, c->R2, (size_t)rm * rm));
+  TRY(dalloc(c, c->R2i, (size_t)rm * rm));
+  TRY(dalloc(c, c->Rq, (size_t)rm * rm));
+  size_t wb = 0;
+  for (int64_t nn : {c->nmax, (int64_t)rm})
+    for (int32_t p : {rm, c->k, 1})
+      for (int32_t q : {rm, c->k, 1}) wb = std::max(wb, tn_workspace_bytes(nn, p, q));
+  TRY(dalloc(c, c->work, wb / sizeof(double) + 1));
+  c->work_bytes = wb;
+  TRY(dalloc(c, c->jwork, jacobi_workspace_bytes(rm)));
+  TRY(dalloc(c, c->dstat, 8));
+  TRY(dalloc(c, c->Ut, (size_t)rm * rm));
+  TRY(dalloc(c, c->Vt, (size_t)rm * rm));
+  TRY(dalloc(c, c->sig, rm));
+  TRY(dalloc(c, c->sinv, rm));
+  TRY(dalloc(c, c->sgn, rm));
+  TRY(dalloc(c, c->shift, 1));
+  TRY(dalloc(c, c->best, rm));
+  TRY(dalloc(c, c->P1, (size_t)c->k * rm));
+  TRY(dalloc(c, c->P2, (size_t)c->k * rm));
+  TRY(dalloc(c, c->mrow, rm));
+  TRY(dalloc(c, c->S, (size_t)rm * rm));
+  TRY(dalloc(c, c->E, (size_t)rm * rm));
+  TRY(dalloc(c, c->E2, (size_t)rm * rm));
+  TRY(dalloc(c, c->lamv, rm));
+  TRY(dalloc(c, c->lamis, rm));
+  return MMSB_OK;
+}
+
+static int alloc_whitening(mmsb_ctx* c) {
+  TRY(alloc_small(c));
+  for (int i = 0; i < 4; ++i) TRY(dalloc(c, c->T[i], (size_t)c->nmax * c->rld));
+  for (int i = 0; i < 3; ++i) TRY(dalloc(c, c->mu[i], (size_t)c->n[i + 1]));
+  TRY(dalloc(c, c->W, (size_t)c->n[1] * c->kp));
+  TRY(dalloc(c, c->U, (size_t)c->n[2] * c->kp));
+  TRY(dalloc(c, c->V, (size_t)c->n[3] * c->kp));
+  TRY(dalloc(c, c->LB, (size_t)c->n[1] * c->kp));
+  TRY(dalloc(c, c->LC, (size_t)c->n[1] * c->kp));
+  TRY(dalloc(c, c->MB, (size_t)c->n[2] * c->kp));
+  TRY(dalloc(c, c->MC, (size_t)c->n[3] * c->kp));
+  TRY(dalloc(c, c->LBW, (size_t)c->k * c->k));
+  TRY(dalloc(c, c->LCW, (size_t)c->k * c->k));
+  return MMSB_OK;
+}
+
+static int alloc_y(mmsb_ctx* c) {
+  TRY(dalloc(c, c->YA, (size_t)c->n_pad * c->kp));
+  TRY(dalloc(c, c->YB, (size_t)c->n_pad * c->kp));
+  TRY(dalloc(c, c->YCt, (size_t)c->kp * c->n_pad));
+  for (int i = 0; i < 3; ++i) TRY(dalloc(c, c->muw[i], (size_t)c->kp));
+  return MMSB_OK;
+}
+
+static int whiten_impl(mmsb_ctx* c) {
+  hipStream_t s = c->s;
+  const stpm_config& cf = c->cfg;
+  const int32_t k = c->k, r = c->r, kp = c->kp, ld = c->rld;
+  const int64_t nx = c->n[0], na = c->n[1], nb = c->n[2], ncn = c->n[3];
+  const float inv = (float)(1.0 / (double)nx);
+  const DevCsr &XA = c->blk[0], &XB = c->blk[1], &XC = c->blk[2];
+  const DevCsr &XAt = c->blkT[0], &XBt = c->blkT[1], &XCt = c->blkT[2];
+  float **T = c->T;
+  TRY(alloc_whitening(c));
+  TRY(alloc_y(c));
+  // means (pre_proc.c:154-162)
+  TRY(csr_rowsum(s, XAt.n_rows, XAt.rowptr, XAt.vals, 1.0 / (double)nx, c->mu[0]));
+  TRY(csr_rowsum(s, XBt.n_rows, XBt.rowptr, XBt.vals, 1.0 / (double)nx, c->mu[1]));
+  TRY(csr_rowsum(s, XCt.n_rows, XCt.rowptr, XCt.vals, 1.0 / (double)nx, c->mu[2]));
+  c->log("whiten: randomized SVD of P_BC (n_B=%lld n_C=%lld r=%d q=%d)\n", (long long)nb, (long long)ncn, r, cf.power_iters);
+  // range of P_BC
+  TRY(gaussian_fill(s, cf.seed, 0, ncn, r, T[0], ld));
+  TRY(spmm(c, XC, 1.f, T[0], ld, r, T[1], ld));
+  TRY(spmm(c, XBt, inv, T[1], ld, r, T[2], ld));
+  TRY(orth(c, T[2], nb, r, ld, T[2], T[3], nullptr, 0));
+  for (int q = 0; q < cf.power_iters; ++q) {
+    TRY(spmm(c, XB, 1.f, T[2], ld, r, T[1], ld));
+    TRY(spmm(c, XCt, inv, T[1], ld, r, T[0], ld));
+    TRY(orth(c, T[0], ncn, r, ld, T[0], T[3], nullptr, 2));
+    TRY(spmm(c, XC, 1.f, T[0], ld, r, T[1], ld));
+    TRY(spmm(c, XBt, inv, T[1], ld, r, T[2], ld));
+    TRY(orth(c, T[2], nb, r, ld, T[2], T[3], nullptr, 0));
+  }
+  // P_CB Q = Q2 R2  =>  P_BC ~ Q R2^T Q2^T
+  TRY(spmm(c, XB, 1.f, T[2], ld, r, T[1], ld));
+  TRY(spmm(c, XCt, inv, T[1], ld, r, T[0], ld));
+  TRY(orth(c, T[0], ncn, r, ld, T[0], T[3], c->Rq, 2));
+  // SVD of R2^T: the row-major R2 buffer is R2^T in column-major order
+  TRY(jacobi_svd(s, r, c->Rq, c->Ut, c->sig, c->Vt, c->jwork, c->dstat + 6));
+  TRY((gemm_nn<float, float>(s, nb, r, k, 1.0, T[2], ld, c->Ut, r, 0.0, c->U, kp, true)));   // U = Q Ut[:, :k]
+  TRY((gemm_nn<float, float>(s, ncn, r, k, 1.0, T[0], ld, c->Vt, r, 0.0, c->V, kp, true)));  // V = Q2 Vt[:, :k]
+  TRY(col_sign<float>(s, nb, k, c->U, kp, c->best, c->sgn));
+  TRY(scale_cols<float>(s, nb, k, c->U, kp, c->sgn));
+  TRY(scale_cols<float>(s, ncn, k, c->V, kp, c->sgn));
+  TRY(pinv_scale(s, k, c->sig, 0.0, cf.pinv_tol, 0, c->sinv, nullptr));
+  // L_B, L_C
+  c->log("whiten: Z_B / Z_C factors\n");
+  TRY(spmm(c, XC, 1.f, c->V, kp, k, T[1], ld));
+  TRY(spmm(c, XAt, inv, T[1], ld, k, c->LB, kp));
+  TRY(scale_cols<float>(s, na, k, c->LB, kp, c->sinv));
+  TRY(spmm(c, XB, 1.f, c->U, kp, k, T[1], ld));
+  TRY(spmm(c, XAt, inv, T[1], ld, k, c->LC, kp));
+  TRY(scale_cols<float>(s, na, k, c->LC, kp, c->sinv));
+  // M2: range finder, Rayleigh-Ritz, W
+  c->log("whiten: M2 eigenpairs (n_A=%lld)\n", (long long)na);
+  TRY(gaussian_fill(s, cf.seed, 1, na, r, T[0], ld));
+  TRY(m2_apply(c, T[0], T[2]));
+  TRY(orth(c, T[2], na, r, ld, T[2], T[3], nullptr, 4));
+  int qbuf = 2;
+  for (int q = 0; q < cf.power_iters; ++q) {
+    const int other = qbuf == 2 ? 0 : 2;
+    TRY(m2_apply(c, T[qbuf], T[other]));
+    TRY(orth(c, T[other], na, r, ld, T[other], T[3], nullptr, 4));
+    qbuf = other;
+  }
+  TRY(m2_apply(c, T[qbuf], T[1]));
+  TRY((gemm_tn<float, float>(s, na, r, r, 1.0, T[qbuf], ld, T[1], ld, 0.0, c->S, c->work)));
+  TRY(sym_shift(s, r, c->S, c->shift));
+  TRY(jacobi_svd(s, r, c->S, c->E, c->lamv, c->E2, c->jwork, c->dstat + 7));
+  double hshift = 0.0;
+  HCHK(hipMemcpyAsync(&hshift, c->shift, sizeof(double), hipMemcpyDeviceToHost, s));
+  HCHK(hipStreamSynchronize(s));
+  TRY(pinv_scale(s, r, c->lamv, hshift, cf.pinv_tol, 1, c->lamis, c->lamv));  // lamv <- sigma - shift
+  TRY((gemm_nn<float, float>(s, na, r, k, 1.0, T[qbuf], ld, c->E, r, 0.0, c->W, kp, true)));
+  TRY(scale_cols<float>(s, na, k, c->W, kp, c->lamis));
+  TRY(col_sign<float>(s, na, k, c->W, kp, c->best, c->sgn));
+  TRY(scale_cols<float>(s, na, k, c->W, kp, c->sgn));
+  // neighbour vectors (pre_proc.c:339-366)
+  c->log("whiten: neighbour vectors\n");
+  TRY(spmm(c, XA, 1.f, c->W, kp, k, c->YA, kp));
+  TRY((gemm_tn<float, float>(s, na, k, k, 1.0, c->LB, kp, c->W, kp, 0.0, c->LBW, c->work)));
+  TRY((gemm_nn<float, float>(s, nb, k, k, 1.0, c->U, kp, c->LBW, k, 0.0, c->MB, kp)));
+  TRY(spmm(c, XB, 1.f, c->MB, kp, k, c->YB, kp));
+  TRY((gemm_tn<float, float>(s, na, k, k, 1.0, c->LC, kp, c->W, kp, 0.0, c->LCW, c->work)));
+  TRY((gemm_nn<float, float>(s, ncn, k, k, 1.0, c->V, kp, c->LCW, k, 0.0, c->MC, kp)));
+  TRY(spmm(c, XC, 1.f, c->MC, kp, k, T[1], ld));
+  TRY(transpose_f32(s, nx, k, T[1], ld, c->YCt, c->n_pad));
+  TRY((gemm_tn<float, double>(s, na, k, 1, 1.0, c->W, kp, c->mu[0], 1, 0.0, c->muw[0], c->work)));
+  TRY((gemm_tn<float, double>(s, nb, k, 1, 1.0, c->MB, kp, c->mu[1], 1, 0.0, c->muw[1], c->work)));
+  TRY((gemm_tn<float, double>(s, ncn, k, 1, 1.0, c->MC, kp, c->mu[2], 1, 0.0, c->muw[2], c->work)));
+  TRY(check_status(c, "whitening"));
+  c->whitened = true;
+  c->have_y = true;
+  c->y16_valid = false;
+  return MMSB_OK;
+}
+
+// ---- STGD --------------------------------------------------------------------------------------
+static int alloc_stgd(mmsb_ctx* c) {
+  const size_t kk = (size_t)c->kp * c->kp;
+  TRY(dalloc(c, c->phi[0], kk));
+  TRY(dalloc(c, c->phi[1], kk));
+  TRY(dalloc(c, c->phit, kk));
+  TRY(dalloc(c, c->gt, kk));
+  TRY(dalloc(c, c->phia, kk));
+  TRY(dalloc(c, c->h, kk));
+  TRY(dalloc(c, c->cgt, (size_t)c->kp * c->b));
+  TRY(dalloc(c, c->cs_part, (size_t)(c->b / 64) * c->kp));
+  TRY(dalloc(c, c->mab, (size_t)2 * c->kp));
+  return MMSB_OK;
+}
+
+static int init_phi(mmsb_ctx* c) {
+  // Phi0 = CholeskyQR2 of a seeded k x k Gaussian (replaces the time-seeded QR of rand(),
+  // stpm.c:1041-1064); phit and phia serve as scratch.
+  TRY(alloc_small(c));
+  TRY(gaussian_fill(c->s, c->cfg.seed, 2, c->k, c->k, c->phit, c->kp));
+  TRY(orth(c, c->phit, c->k, c->k, c->kp, c->phi[c->cur], c->phia, nullptr, 0));
+  TRY(check_status(c, "Phi0"));
+  c->have_phi = true;
+  return MMSB_OK;
+}
+
+static int prepare_bf16(mmsb_ctx* c) {
+  const size_t kk = (size_t)c->kp * c->kp;
+  const size_t ny = (size_t)c->n_pad * c->kp;
+  TRY(dalloc(c, c->YA16, ny));
+  TRY(dalloc(c, c->YB16, ny));
+  TRY(dalloc(c, c->YCt16, ny));
+  TRY(dalloc(c, c->cgt16, (size_t)c->kp * c->b));
+  TRY(dalloc(c, c->phiT, kk));
+  TRY(dalloc(c, c->phitT, kk));
+  TRY(dalloc(c, c->phiaT, kk));
+  if (!c->y16_valid) {
+    TRY(f32_to_bf16(c->s, (int64_t)ny, c->YA, c->YA16));
+    TRY(f32_to_bf16(c->s, (int64_t)ny, c->YB, c->YB16));
+    TRY(f32_to_bf16(c->s, (int64_t)ny, c->YCt, c->YCt16));
+    c->y16_valid = true;
+  }
+  return MMSB_OK;
+}
+
+static int run_stgd_steps(mmsb_ctx* c, int64_t nsteps, int32_t precision) {
+  if (nsteps <= 0) return MMSB_OK;
+  if (precision != MMSB_PREC_FP32 && precision != MMSB_PREC_BF16)
+    return set_error(MMSB_ERR_ARG, "stgd: unknown precision %d", precision);
+  if (precision == MMSB_PREC_BF16) TRY(prepare_bf16(c));
+  if (c->steps_done + nsteps > c->frob_cap) {
+    int64_t cap = std::max<int64_t>(1024, 2 * (c->steps_done + nsteps));
+    double* nf = nullptr;
+    HCHK(hipMalloc(&nf, sizeof(double) * cap));
+    HCHK(hipMemsetAsync(nf, 0, sizeof(double) * cap, c->s));
+    if (c->frob && c->steps_done)
+      HCHK(hipMemcpyAsync(nf, c->frob, sizeof(double) * c->steps_done, hipMemcpyDeviceToDevice, c->s));
+    if (c->frob) {
+      HCHK(hipStreamSynchronize(c->s));
+      auto it = std::find(c->allocs.begin(), c->allocs.end(), (void*)c->frob);
+      if (it != c->allocs.end()) c->allocs.erase(it);
+      HCHK(hipFree(c->frob));
+    }
+    c->allocs.push_back(nf);
+    c->frob = nf;
+    c->frob_cap = cap;
+  }
+  StgdArgs a{};
+  a.kp = c->kp;
+  a.b = c->b;
+  a.bl = c->bl;
+  a.n_x = c->n[0];
+  a.n_pad = c->n_pad;
+  const double a0 = c->cfg.alpha_0;
+  a.c3 = 1.0f;
+  a.c1 = (float)(-a0 / (a0 + 2.0));
+  a.c0 = (float)(2.0 * a0 * a0 / ((a0 + 1.0) * (a0 + 2.0)));
+  a.eta = (float)c->cfg.learn_rate;
+  a.theta = (float)c->cfg.theta;
+  a.mu_a = c->muw[0];
+  a.mu_b = c->muw[1];
+  a.mu_c = c->muw[2];
+  a.h = c->h;
+  a.cgt = c->cgt;
+  a.cs_part = c->cs_part;
+  a.mab = c->mab;
+  if (precision == MMSB_PREC_BF16)
+    TRY(stgd_run_bf16(c->s, a, c->YA16, c->YB16, c->YCt16, c->steps_done, nsteps, c->cfg.renorm_every, c->cfg.ns_iters,
+                      c->phi, c->cur, c->phit, c->gt, c->phia, c->phiT, c->phitT, c->phiaT, c->cgt16, c->frob));
+  else
+    TRY(stgd_run_f32(c->s, a, c->YA, c->YB, c->YCt, c->steps_done, nsteps, c->cfg.renorm_every, c->cfg.ns_iters, c->phi,
+                     c->cur, c->phit, c->gt, c->phia, c->frob));
+  c->steps_done += nsteps;
+  return MMSB_OK;
+}
+
+// ---- eigenvalues and Pi ------------------------------------------------------------------------
+static int eigen_impl(mmsb_ctx* c) {
+  const int32_t k = c->k;
+  TRY(dalloc(c, c->lam, k));
+  TRY(dalloc(c, c->lamp, k));
+  TRY(dalloc(c, c->ahat, k));
+  TRY(dalloc(c, c->phil, (size_t)k * k));
+  TRY(eig_post(c->s, k, c->kp, c->phi[c->cur], c->cfg.pinv_tol, c->lam, c->lamp, c->ahat, c->phil));
+  c->eig_done = true;
+  return MMSB_OK;
+}
+
+static int post_impl(mmsb_ctx* c) {
+  hipStream_t s = c->s;
+  const int32_t k = c->k, kp = c->kp;
+  if (!c->whitened) return set_error(MMSB_ERR_STATE, "post_proc needs the whitening of this context");
+  if (!c->eig_done) TRY(eigen_impl(c));
+  TRY(dalloc(c, c->Upost, (size_t)c->n[1] * kp));
+  TRY((gemm_nn<float, float>(s, c->n[1], k, k, 1.0, c->W, kp, c->phil, k, 0.0, c->Upost, kp)));  // W Phi Lambda'
+  TRY(dalloc(c, c->Pi[0], (size_t)c->n[0] * kp));
+  TRY(spmm(c, c->blk[MMSB_BLOCK_XA], 1.f, c->Upost, kp, k, c->Pi[0], kp));                           // post_proc.c:93
+  if (c->have[MMSB_BLOCK_BA]) {
+    TRY(dalloc(c, c->LBU, (size_t)k * k));
+    TRY(dalloc(c, c->ZBtU, (size_t)c->n[2] * kp));
+    TRY((gemm_tn<float, float>(s, c->n[1], k, k, 1.0, c->LB, kp, c->Upost, kp, 0.0, c->LBU, c->work)));
+    TRY((gemm_nn<float, float>(s, c->n[2], k, k, 1.0, c->U, kp, c->LBU, k, 0.0, c->ZBtU, kp)));      // Z_B^T Upost
+    TRY(dalloc(c, c->Pi[1], (size_t)c->n[1] * kp));
+    TRY(spmm(c, c->blkT[MMSB_BLOCK_BA], 1.f, c->ZBtU, kp, k, c->Pi[1], kp));                         // post_proc.c:97-100
+    TRY(dalloc(c, c->Pi[2], (size_t)c->n[2] * kp));
+    TRY(spmm(c, c->blk[MMSB_BLOCK_BA], 1.f, c->Upost, kp, k, c->Pi[2], kp));                          // post_proc.c:103
+  }
+  if (c->have[MMSB_BLOCK_CA]) {
+    TRY(dalloc(c, c->Pi[3], (size_t)c->n[3] * kp));
+    TRY(spmm(c, c->blk[MMSB_BLOCK_CA], 1.f, c->Upost, kp, k, c->Pi[3], kp));                          // post_proc.c:106
+  }
+  const int64_t rows[4] = {c->n[0], c->n[1], c->n[2], c->n[3]};
+  for (int p = 0; p < 4; ++p) {
+    if (!c->Pi[p]) continue;
+    TRY(dalloc(c, c->PiN[p], (size_t)rows[p] * kp));
+    TRY(dalloc(c, c->cnt[p], (size_t)rows[p]));
+    TRY(norm_prob(s, rows[p], k, c->Pi[p], kp, c->PiN[p], kp, (float)c->cfg.member_tau, c->cnt[p]));
+  }
+  c->post_done = true;
+  return MMSB_OK;
+}
+
+// ---- getters -------------------------------------------------------------------------------------
+struct View {
+  const void* p = nullptr;
+  int dtype = 0;  // 0 float, 1 double, 2 int32
+  int64_t rows = 0;
+  int32_t cols = 0;
+  int64_t ld = 0;
+  bool trans = false;  // stored [cols x rows] with leading dimension ld
+};
+
+static View view_of(mmsb_ctx* c, int what) {
+  View v;
+  const int32_t k = c->k, kp = c->kp;
+  auto tall = [&](const void* p, int64_t rows) { View w; w.p = p; w.rows = rows; w.cols = k; w.ld = kp; return w; };
+  auto vec = [&](const void* p, int64_t n) { View w; w.p = p; w.dtype = 1; w.rows = n; w.cols = 1; w.ld = 1; return w; };
+  switch (what) {
+    case MMSB_Q_MU_A: case MMSB_Q_MU_B: case MMSB_Q_MU_C:
+      return c->mu[what - MMSB_Q_MU_A] ? vec(c->mu[what - MMSB_Q_MU_A], c->n[1 + what - MMSB_Q_MU_A]) : v;
+    case MMSB_Q_W: return c->whitened ? tall(c->W, c->n[1]) : v;
+    case MMSB_Q_U: return c->whitened ? tall(c->U, c->n[2]) : v;
+    case MMSB_Q_V: return c->whitened ? tall(c->V, c->n[3]) : v;
+    case MMSB_Q_SIGMA: return c->whitened ? vec(c->sig, k) : v;
+    case MMSB_Q_LB: return c->whitened ? tall(c->LB, c->n[1]) : v;
+    case MMSB_Q_LC: return c->whitened ? tall(c->LC, c->n[1]) : v;
+    case MMSB_Q_LAM_M2: return c->whitened ? vec(c->lamv, k) : v;
+    case MMSB_Q_YA: return c->have_y ? tall(c->YA, c->n[0]) : v;
+    case MMSB_Q_YB: return c->have_y ? tall(c->YB, c->n[0]) : v;
+    case MMSB_Q_YC:
+      if (!c->have_y) return v;
+      v.p = c->YCt; v.rows = c->n[0]; v.cols = k; v.ld = c->n_pad; v.trans = true;
+      return v;
+    case MMSB_Q_MUW_A: case MMSB_Q_MUW_B: case MMSB_Q_MUW_C:
+      return c->have_y ? vec(c->muw[what - MMSB_Q_MUW_A], k) : v;
+    case MMSB_Q_PHI:
+      if (!c->have_phi) return v;
+      v.p = c->phi[c->cur]; v.rows = k; v.cols = k; v.ld = kp;
+      return v;
+    case MMSB_Q_FROB: return c->steps_done ? vec(c->frob, c->steps_done) : v;
+    case MMSB_Q_LAMBDA: return c->eig_done ? vec(c->lam, k) : v;
+    case MMSB_Q_LAMBDA_P: return c->eig_done ? vec(c->lamp, k) : v;
+    case MMSB_Q_ALPHA_HAT: return c->eig_done ? vec(c->ahat, k) : v;
+    case MMSB_Q_PI_X: case MMSB_Q_PI_A: case MMSB_Q_PI_B: case MMSB_Q_PI_C: {
+      const int p = what - MMSB_Q_PI_X;
+      return c->Pi[p] ? tall(c->Pi[p], c->n[p]) : v;
+    }
+    case MMSB_Q_PIN_X: case MMSB_Q_PIN_A: case MMSB_Q_PIN_B: case MMSB_Q_PIN_C: {
+      const int p = what - MMSB_Q_PIN_X;
+      return c->PiN[p] ? tall(c->PiN[p], c->n[p]) : v;
+    }
+    case MMSB_Q_COUNT_X: case MMSB_Q_COUNT_A: case MMSB_Q_COUNT_B: case MMSB_Q_COUNT_C: {
+      const int p = what - MMSB_Q_COUNT_X;
+      if (!c->cnt[p]) return v;
+      v.p = c->cnt[p]; v.dtype = 2; v.rows = c->n[p]; v.cols = 1; v.ld = 1;
+      return v;
+    }
+    default: return v;
+  }
+}
+
+static int copy_out(mmsb_ctx* c, const View& v, double* host) {
+  const size_t esz = v.dtype == 1 ? 8 : 4;
+  const int64_t outer = v.trans ? v.cols : v.rows;
+  const int64_t inner = v.trans ? v.rows : v.cols;
+  std::vector<unsigned char> buf((size_t)outer * inner * esz);
+  HCHK(hipMemcpy2DAsync(buf.data(), inner * esz, v.p, v.ld * esz, inner * esz, outer, hipMemcpyDeviceToHost, c->s));
+  HCHK(hipStreamSynchronize(c->s));
+  for (int64_t i = 0; i < v.rows; ++i)
+    for (int32_t j = 0; j < v.cols; ++j) {
+      const int64_t src = v.trans ? (int64_t)j * v.rows + i : i * v.cols + j;
+      double x;
+      if (v.dtype == 0) x = reinterpret_cast<const float*>(buf.data())[src];
+      else if (v.dtype == 1) x = reinterpret_cast<const double*>(buf.data())[src];
+      else x = reinterpret_cast<const int32_t*>(buf.data())[src];
+      host[i * v.cols + j] = x;
+    }
+  return MMSB_OK;
+}
+
+static std::string out_name(const stpm_config& cf, const char* stem) {
+  return std::string(cf.folder_name) + stem + "_" + std::to_string(cf.ncom) + "_" + cf.alpha_0_str + "_" + cf.learn_rate_str + ".txt";
+}
+
+static int set_sizes(mmsb_ctx* c) {
+  const stpm_config& cf = c->cfg;
+  c->k = cf.ncom;
+  c->r = cf.ncom + std::max(0, cf.oversample);
+  c->kp = (int32_t)r64(c->k, 64);
+  c->rld = (int32_t)r64(c->r, 4);
+  c->rmax = std::max(c->r, c->kp);
+  c->bl = std::max(1, cf.minibatch);
+  c->b = (int32_t)r64(c->bl, 64);
+  c->n[0] = cf.nx;
+  c->n[1] = cf.na;
+  c->n[2] = cf.nb;
+  c->n[3] = cf.nc;
+  c->nmax = std::max(std::max(c->n[0], c->n[1]), std::max(c->n[2], c->n[3]));
+  c->n_pad = r64((r64(c->n[0], c->bl) - c->bl + c->b), 64);
+  return MMSB_OK;
+}
+
+}  // namespace mmsb
+
+// ===================================== C-ABI =========================================================
+extern "C" void stpm_config_default(stpm_config* c) {
+  std::memset(c, 0, sizeof(*c));
+  c->nx = 29079;  // stpm.h:72-76
+  c->na = 29080;
+  c->nb = 29079;
+  c->nc = 29079;
+  c->ncom = 100;
+  c->alpha_0 = 1.0;       // stpm.h:78
+  c->learn_rate = 1e-3;   // stpm.h:79
+  c->thresh = 1.0;        // stpm.h:94
+  c->pinv_tol = 1e-6;     // stpm.h:95
+  c->sgd_iter_min = 0;    // 0: run every configured epoch
+  c->sgd_iter_max = 0;
+  c->binary = 1;
+  c->synthetic = 1;
+  std::snprintf(c->folder_name, sizeof(c->folder_name), "DBLP_116317/");
+  std::snprintf(c->file_prefix, sizeof(c->file_prefix), "DBLP_");
+  std::snprintf(c->alpha_0_str, sizeof(c->alpha_0_str), "1.0");
+  std::snprintf(c->learn_rate_str, sizeof(c->learn_rate_str), "1e-3");
+  c->minibatch = 1024;
+  c->epochs = 1;
+  c->theta = 1.0;
+  c->renorm_every = 1;
+  c->ns_iters = 2;
+  c->precision = MMSB_PREC_FP32;
+  c->polish_epochs = 1;
+  c->oversample = 10;
+  c->power_iters = 1;
+  c->m2_scaling = MMSB_M2_PAPER;
+  c->member_tau = 0.0;
+  c->seed = 0;
+  c->ba_first_is_row = 1;
+  c->feof_repeat = 0;
+  c->device = 0;
+  c->verbose = 0;
+}
+
+extern "C" int mmsb_create(const stpm_config* cfg, mmsb_ctx** out) {
+  clear_error();
+  MMSB_CHECK_ARG(cfg && out, "create: null argument");
+  MMSB_CHECK_ARG(cfg->nx > 0 && cfg->na > 0 && cfg->nb > 0 && cfg->nc > 0, "create: partition sizes must be positive");
+  MMSB_CHECK_ARG(cfg->ncom >= 1 && cfg->ncom <= 2048, "create: NCOM=%d outside [1, 2048]", cfg->ncom);
+  MMSB_CHECK_ARG(cfg->ncom + cfg->oversample <= 4096 && cfg->oversample >= 0, "create: bad oversampling");
+  MMSB_CHECK_ARG(cfg->ncom + cfg->oversample <= std::min(std::min(cfg->na, cfg->nb), cfg->nc),
+                 "create: k + oversample must not exceed the partition sizes");
+  MMSB_CHECK_ARG(cfg->minibatch >= 1 && cfg->power_iters >= 0 && cfg->ns_iters >= 0, "create: bad STGD settings");
+  mmsb_ctx* c = new (std::nothrow) mmsb_ctx();
+  if (!c) return set_error(MMSB_ERR_NOMEM, "create: out of host memory");
+  c->cfg = *cfg;
+  set_sizes(c);
+  hipError_t e = hipSetDevice(cfg->device);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    delete c;
+    return set_error(MMSB_ERR_HIP, "create: %s", hipGetErrorString(e));
+  }
+  *out = c;
+  return MMSB_OK;
+}
+
+extern "C" void mmsb_destroy(mmsb_ctx* c) {
+  if (!c) return;
+  (void)hipSetDevice(c->cfg.device);
+  if (c->s) (void)hipStreamSynchronize(c->s);
+  for (void* p : c->allocs) (void)hipFree(p);
+  if (c->s) (void)hipStreamDestroy(c->s);
+  delete c;
+}
+
+extern "C" int mmsb_set_block(mmsb_ctx* c, int32_t which, const mmsb_csr* b) {
+  clear_error();
+  MMSB_CHECK_ARG(c && b && which >= 0 && which < 5, "set_block: bad arguments");
+  const int32_t rows_of[5] = {c->cfg.nx, c->cfg.nx, c->cfg.nx, c->cfg.nb, c->cfg.nc};
+  const int32_t cols_of[5] = {c->cfg.na, c->cfg.nb, c->cfg.nc, c->cfg.na, c->cfg.na};
+  MMSB_CHECK_ARG(b->n_rows == rows_of[which] && b->n_cols == cols_of[which], "set_block %d: shape %d x %d, expected %d x %d",
+                 which, b->n_rows, b->n_cols, rows_of[which], cols_of[which]);
+  MMSB_CHECK_ARG(b->nnz >= 0 && b->nnz < INT32_MAX && b->rowptr && (b->nnz == 0 || b->colidx), "set_block: bad CSR");
+  MMSB_CHECK_ARG(b->rowptr[0] == 0 && b->rowptr[b->n_rows] == b->nnz, "set_block: rowptr does not match nnz");
+  for (int32_t i = 0; i < b->n_rows; ++i)
+    MMSB_CHECK_ARG(b->rowptr[i + 1] >= b->rowptr[i], "set_block: rowptr not monotone at row %d", i);
+  for (int64_t e = 0; e < b->nnz; ++e)
+    MMSB_CHECK_ARG(b->colidx[e] >= 0 && b->colidx[e] < b->n_cols, "set_block: column index %d out of range", b->colidx[e]);
+  HCHK(hipSetDevice(c->cfg.device));
+  if (c->blk[which].rowptr) return set_error(MMSB_ERR_STATE, "set_block: block %d already set", which);
+  TRY(upload_csr(c, c->blk[which], b->n_rows, b->n_cols, b->nnz, b->rowptr, b->colidx, b->vals));
+  std::vector<int32_t> rp, ci;
+  std::vector<float> v;
+  host_transpose(*b, rp, ci, v);
+  TRY(upload_csr(c, c->blkT[which], b->n_cols, b->n_rows, b->nnz, rp.data(), ci.data(), b->vals ? v.data() : nullptr));
+  c->have[which] = true;
+  return MMSB_OK;
+}
+
+extern "C" int mmsb_read_inputs(mmsb_ctx* c) {
+  clear_error();
+  MMSB_CHECK_ARG(c, "read_inputs: null context");
+  const stpm_config& cf = c->cfg;
+  const char* stems[5] = {"Gx_a", "Gx_b", "Gx_c", "Gb_a", "Gc_a"};
+  const int32_t rows_of[5] = {cf.nx, cf.nx, cf.nx, cf.nb, cf.nc};
+  const int32_t cols_of[5] = {cf.na, cf.nb, cf.nc, cf.na, cf.na};
+  const int nfiles = cf.synthetic ? 5 : 3;
+  for (int b = 0; b < nfiles; ++b) {
+    const std::string path = std::string(cf.folder_name) + cf.file_prefix + stems[b] + ".txt";
+    c->log("reading %s\n", path.c_str());
+    mmsb_csr m{};
+    const int32_t first_is_row = b < 3 ? 1 : cf.ba_first_is_row;
+    const int st = mmsb_read_coo(path.c_str(), rows_of[b], cols_of[b], first_is_row, cf.binary ? 0 : 1,
+                                 cf.binary ? MMSB_DUP_ONE : MMSB_DUP_LAST, cf.feof_repeat, &m);
+    if (st != MMSB_OK) return st;
+    const int st2 = mmsb_set_block(c, b, &m);
+    mmsb_csr_free(&m);
+    if (st2 != MMSB_OK) return st2;
+  }
+  return MMSB_OK;
+}
+
+extern "C" int mmsb_whiten(mmsb_ctx* c) {
+  clear_error();
+  MMSB_CHECK_ARG(c, "whiten: null context");
+  for (int b = 0; b < 3; ++b)
+    if (!c->have[b]) return set_error(MMSB_ERR_STATE, "whiten: block %d (G_X%c) not set", b, "ABC"[b]);
+  HCHK(hipSetDevice(c->cfg.device));
+  const double t0 = now_s();
+  TRY(whiten_impl(c));
+  HCHK(hipStreamSynchronize(c->s));
+  c->times[0] = now_s() - t0;
+  return MMSB_OK;
+}
+
+extern "C" int mmsb_set_phi(mmsb_ctx* c, const double* phi) {
+  clear_error();
+  MMSB_CHECK_ARG(c && phi, "set_phi: null argument");
+  HCHK(hipSetDevice(c->cfg.device));
+  TRY(alloc_stgd(c));
+  std::vector<float> buf((size_t)c->kp * c->kp, 0.f);
+  for (int32_t m = 0; m < c->k; ++m)
+    for (int32_t i = 0; i < c->k; ++i) buf[(size_t)m * c->kp + i] = (float)phi[(size_t)m * c->k + i];
+  HCHK(hipMemcpyAsync(c->phi[c->cur], buf.data(), sizeof(float) * buf.size(), hipMemcpyHostToDevice, c->s));
+  HCHK(hipStreamSynchronize(c->s));
+  c->have_phi = true;
+  c->eig_done = c->post_done = false;
+  return MMSB_OK;
+}
+
+extern "C" int mmsb_set_yvectors(mmsb_ctx* c, int64_t n_x, int32_t k, const double* ya, const double* yb, const double* yc,
+                                 const double* mua, const double* mub, const double* muc) {
+  clear_error();
+  MMSB_CHECK_ARG(c && ya && yb && yc && mua && mub && muc, "set_yvectors: null argument");
+  MMSB_CHECK_ARG(n_x == c->n[0] && k == c->k, "set_yvectors: shape %lld x %d, context has %lld x %d", (long long)n_x, k,
+                 (long long)c->n[0], c->k);
+  HCHK(hipSetDevice(c->cfg.device));
+  TRY(alloc_y(c));
+  const int32_t kp = c->kp;
+  std::vector<float> a((size_t)c->n_pad * kp, 0.f), b((size_t)c->n_pad * kp, 0.f), ct((size_t)kp * c->n_pad, 0.f);
+  for (int64_t x = 0; x < n_x; ++x)
+    for (int32_t i = 0; i < k; ++i) {
+      a[(size_t)x * kp + i] = (float)ya[x * k + i];
+      b[(size_t)x * kp + i] = (float)yb[x * k + i];
+      ct[(size_t)i * c->n_pad + x] = (float)yc[x * k + i];
+    }
+  HCHK(hipMemcpyAsync(c->YA, a.data(), sizeof(float) * a.size(), hipMemcpyHostToDevice, c->s));
+  HCHK(hipMemcpyAsync(c->YB, b.data(), sizeof(float) * b.size(), hipMemcpyHostToDevice, c->s));
+  HCHK(hipMemcpyAsync(c->YCt, ct.data(), sizeof(float) * ct.size(), hipMemcpyHostToDevice, c->s));
+  const double* mus[3] = {mua, mub, muc};
+  for (int i = 0; i < 3; ++i) HCHK(hipMemcpyAsync(c->muw[i], mus[i], sizeof(double) * k, hipMemcpyHostToDevice, c->s));
+  HCHK(hipStreamSynchronize(c->s));
+  c->have_y = true;
+  c->y16_valid = false;
+  return MMSB_OK;
+}
+
+extern "C" int mmsb_stgd(mmsb_ctx* c, int64_t steps, int32_t precision) {
+  clear_error();
+  MMSB_CHECK_ARG(c, "stgd: null context");
+  if (!c->have_y) return set_error(MMSB_ERR_STATE, "stgd: neighbour vectors not computed (call mmsb_whiten)");
+  HCHK(hipSetDevice(c->cfg.device));
+  TRY(alloc_stgd(c));
+  if (!c->have_phi) TRY(init_phi(c));
+  c->eig_done = c->post_done = false;
+  const double t0 = now_s();
+  if (steps >= 0) {
+    TRY(run_stgd_steps(c, steps, precision));
+  } else {
+    const stpm_config& cf = c->cfg;
+    const int64_t spe = (c->n[0] + c->bl - 1) / c->bl;
+    const int64_t min_steps = cf.sgd_iter_min > 0 ? cf.sgd_iter_min : (int64_t)cf.epochs * spe;
+    for (int32_t e = 0; e < cf.epochs; ++e) {
+      const bool low = precision == MMSB_PREC_BF16 && e < cf.epochs - cf.polish_epochs;
+      TRY(run_stgd_steps(c, spe, low ? MMSB_PREC_BF16 : MMSB_PREC_FP32));
+      if (cf.sgd_iter_max > 0 && c->steps_done >= cf.sgd_iter_max) break;          // stpm.c:761
+      if (c->steps_done >= min_steps) {
+        double last = 0.0;
+        HCHK(hipMemcpyAsync(&last, c->frob + c->steps_done - 1, sizeof(double), hipMemcpyDeviceToHost, c->s));
+        HCHK(hipStreamSynchronize(c->s));
+        if (last < cf.thresh) break;                                                     // stpm.c:747
+      }
+    }
+  }
+  HCHK(hipStreamSynchronize(c->s));
+  c->times[1] += now_s() - t0;
+  return MMSB_OK;
+}
+
+extern "C" int mmsb_eigen(mmsb_ctx* c) {
+  clear_error();
+  MMSB_CHECK_ARG(c, "eigen: null context");
+  if (!c->have_phi) return set_error(MMSB_ERR_STATE, "eigen: no Phi yet");
+  HCHK(hipSetDevice(c->cfg.device));
+  TRY(eigen_impl(c));
+  HCHK(hipStreamSynchronize(c->s));
+  return MMSB_OK;
+}
+
+extern "C" int mmsb_post(mmsb_ctx* c) {
+  clear_error();
+  MMSB_CHECK_ARG(c, "post: null context");
+  if (!c->have_phi) return set_error(MMSB_ERR_STATE, "post: no Phi yet");
+  HCHK(hipSetDevice(c->cfg.device));
+  const double t0 = now_s();
+  TRY(post_impl(c));
+  HCHK(hipStreamSynchronize(c->s));
+  c->times[2] = now_s() - t0;
+  c->times[3] = c->times[0] + c->times[1] + c->times[2];
+  return MMSB_OK;
+}
+
+extern "C" int64_t mmsb_get_size(mmsb_ctx* c, int32_t what) {
+  if (!c) return 0;
+  if (what == MMSB_Q_TIMES) return 4;
+  const View v = view_of(c, what);
+  return v.p ? v.rows * v.cols : 0;
+}
+
+extern "C" int mmsb_get(mmsb_ctx* c, int32_t what, double* host, int64_t count) {
+  clear_error();
+  MMSB_CHECK_ARG(c && host, "get: null argument");
+  if (what == MMSB_Q_TIMES) {
+    MMSB_CHECK_ARG(count >= 4, "get: need 4 doubles for the timings");
+    std::memcpy(host, c->times, sizeof(c->times));
+    return MMSB_OK;
+  }
+  const View v = view_of(c, what);
+  if (!v.p) return set_error(MMSB_ERR_STATE, "get: quantity %d not computed", what);
+  MMSB_CHECK_ARG(count >= v.rows * v.cols, "get: buffer of %lld doubles, need %lld", (long long)count, (long long)(v.rows * v.cols));
+  HCHK(hipSetDevice(c->cfg.device));
+  return copy_out(c, v, host);
+}
+
+extern "C" int mmsb_sync(mmsb_ctx* c) {
+  clear_error();
+  MMSB_CHECK_ARG(c, "sync: null context");
+  HCHK(hipStreamSynchronize(c->s));
+  return MMSB_OK;
+}
+
+extern "C" int mmsb_write_outputs(mmsb_ctx* c) {
+  clear_error();
+  MMSB_CHECK_ARG(c, "write_outputs: null context");
+  const stpm_config& cf = c->cfg;
+  const char* pis[4] = {"Pi_X", "Pi_A", "Pi_B", "Pi_C"};
+  for (int p = 0; p < 4; ++p) {
+    const View v = view_of(c, MMSB_Q_PI_X + p);
+    if (!v.p) continue;
+    std::vector<double> h((size_t)v.rows * v.cols);
+    TRY(copy_out(c, v, h.data()));
+    TRY(mmsb_write_values(out_name(cf, pis[p]).c_str(), h.data(), (int64_t)h.size()));  // post_proc.c:110-114
+  }
+  if (c->have_phi) {  // evecs: k x k column-major (stpm.c:935-937)
+    std::vector<double> h((size_t)c->k * c->k), cm((size_t)c->k * c->k);
+    TRY(copy_out(c, view_of(c, MMSB_Q_PHI), h.data()));
+    for (int32_t i = 0; i < c->k; ++i)
+      for (int32_t m = 0; m < c->k; ++m) cm[(size_t)i * c->k + m] = h[(size_t)m * c->k + i];
+    TRY(mmsb_write_values(out_name(cf, "evecs").c_str(), cm.data(), (int64_t)cm.size()));
+  }
+  if (c->eig_done) {  // evals: alpha_hat (stpm.c:926-929)
+    std::vector<double> h(c->k);
+    TRY(copy_out(c, view_of(c, MMSB_Q_ALPHA_HAT), h.data()));
+    TRY(mmsb_write_values(out_name(cf, "evals").c_str(), h.data(), c->k));
+  }
+  if (c->steps_done) {  // frob_norm (stpm.c:948-951)
+    std::vector<double> h((size_t)c->steps_done);
+    TRY(copy_out(c, view_of(c, MMSB_Q_FROB), h.data()));
+    TRY(mmsb_write_values(out_name(cf, "frob_norm").c_str(), h.data(), c->steps_done));
+  }
+  const std::string path = out_name(cf, "sim_info");  // stpm.c:957-975
+  FILE* f = std::fopen(path.c_str(), "w");
+  if (!f) return set_error(MMSB_ERR_IO, "writing %s failed", path.c_str());
+  std::fprintf(f, "Total number of nodes          = %d\n", cf.nx + cf.na + cf.nb + cf.nc);
+  std::fprintf(f, "Number of nodes in partition X = %d\n", cf.nx);
+  std::fprintf(f, "Number of nodes in partition A = %d\n", cf.na);
+  std::fprintf(f, "Number of nodes in partition B = %d\n", cf.nb);
+  std::fprintf(f, "Number of nodes in partition C = %d\n", cf.nc);
+  std::fprintf(f, "Number of nodes communities    = %d\n", cf.ncom);
+  std::fprintf(f, "Alpha_0 (Dirichlet parameter)  = %5.25e\n", cf.alpha_0);
+  std::fprintf(f, "Learning rate = %5.25e\n", cf.learn_rate);
+  std::fprintf(f, "Threshold for STPM convergence    = %5.25e\n", cf.thresh);
+  std::fprintf(f, "Pseudoinverse numerical tolerance = %5.25e\n", cf.pinv_tol);
+  std::fprintf(f, "Total number of Frob Norm values written to file = %d\n", (int)c->steps_done);
+  std::fprintf(f, "Number of iterations between convergence test    = %d\n", 1);
+  std::fprintf(f, "Maximum number of stochastic iterations allowed  = %d\n", (int)cf.sgd_iter_max);
+  std::fprintf(f, "Minibatch size = %d\n", c->bl);
+  std::fprintf(f, "Exec Time Pre Proc  = %5.25e (Seconds)\n", c->times[0]);
+  std::fprintf(f, "Exec Time STPM      = %5.25e (Seconds)\n", c->times[1]);
+  std::fprintf(f, "Exec Time Post Proc = %5.25e (Seconds)\n", c->times[2]);
+  std::fprintf(f, "Total Exec Time     = %5.25e (Seconds)\n", c->times[0] + c->times[1] + c->times[2]);
+  if (std::fclose(f) != 0) return set_error(MMSB_ERR_IO, "closing %s failed", path.c_str());
+  return MMSB_OK;
+}
+
+// ---- program-level drop-in: one process-wide context, as the reference's globals ------------------
+static stpm_config g_cfg;
+static bool g_cfg_set = false;
+static mmsb_ctx* g_ctx = nullptr;
+static int g_task_status = 0;
+
+extern "C" int stpm_set_config(const stpm_config* cfg) {
+  clear_error();
+  MMSB_CHECK_ARG(cfg, "stpm_set_config: null");
+  g_cfg = *cfg;
+  g_cfg_set = true;
+  if (g_ctx) {
+    mmsb_destroy(g_ctx);
+    g_ctx = nullptr;
+  }
+  return MMSB_OK;
+}
+
+extern "C" int pre_proc(void) {
+  clear_error();
+  if (!g_cfg_set) stpm_config_default(&g_cfg), g_cfg_set = true;
+  if (g_ctx) mmsb_destroy(g_ctx), g_ctx = nullptr;
+  TRY(mmsb_create(&g_cfg, &g_ctx));
+  TRY(mmsb_read_inputs(g_ctx));
+  return mmsb_whiten(g_ctx);
+}
+
+extern "C" int post_proc(void) {
+  clear_error();
+  if (!g_ctx) return set_error(MMSB_ERR_STATE, "post_proc before pre_proc");
+  TRY(mmsb_post(g_ctx));
+  return mmsb_write_outputs(g_ctx);
+}
+
+extern "C" void* GPU_Task(void*) {
+  g_task_status = 0;
+  if (!g_ctx) {
+    g_task_status = set_error(MMSB_ERR_STATE, "GPU_Task before pre_proc");
+    return &g_task_status;
+  }
+  int st = mmsb_stgd(g_ctx, -1, g_cfg.precision);
+  if (st == MMSB_OK) st = mmsb_eigen(g_ctx);
+  if (st == MMSB_OK) st = post_proc();
+  g_task_status = st;
+  return st == MMSB_OK ? nullptr : &g_task_status;
+}

@@ -60,6 +60,190 @@ int mmsb_spmm_f32(void* stream, int32_t m, int32_t n, int64_t nnz, const int32_t
                   const int32_t* colidx, const float* vals, float alpha, const float* B,
                   int32_t ldb, int32_t r, float* Y, int32_t ldy);
 
+
+/* ------------------------------------------------------------------------------------------
+ * Host graph containers and the seeded MMSB generator
+ * ------------------------------------------------------------------------------------------ */
+
+/* CSR adjacency block in host memory, owned by the library (freed by mmsb_graph_free).
+ * Replaces the dense NA x NX double buffers that read_G fills (pre_proc.c:29,49-80). */
+typedef struct mmsb_csr {
+  int32_t n_rows, n_cols;
+  int64_t nnz;
+  int32_t* rowptr;  /* [n_rows + 1] */
+  int32_t* colidx;  /* [nnz], ascending within a row */
+  float* vals;      /* [nnz] or NULL for a binary graph (all ones) */
+} mmsb_csr;
+
+/* The five partition blocks the reference reads (stpm.h:63-69): G_XA, G_XB, G_XC (rows = X
+ * nodes) and G_BA (rows = B), G_CA (rows = C), all with columns in the partner partition. */
+typedef struct mmsb_graph {
+  mmsb_csr xa, xb, xc, ba, ca;
+  int64_t n_nodes, n_edges;
+  int32_t k;
+  /* ground truth, filled only when requested (global node order): */
+  double* theta;    /* [n_nodes] degree propensities                      */
+  int8_t* part;     /* [n_nodes] 0..3 = X, A, B, C                          */
+  int32_t* local;   /* [n_nodes] index inside the partition                 */
+  double* pi;       /* [n_nodes * k] memberships, row-major                 */
+} mmsb_graph;
+
+enum { MMSB_GEN_AUTO = 0, MMSB_GEN_EXACT = 1, MMSB_GEN_BALLS = 2 };
+
+typedef struct mmsb_gen_params {
+  int32_t n_x, n_a, n_b, n_c, k;
+  double alpha0;             /* Dirichlet concentration, pi ~ Dir(alpha0/k * 1)        */
+  double p_in, p_out;        /* block matrix: p_in on the diagonal, p_out off it       */
+  uint64_t seed;
+  int32_t degree_corrected;  /* theta ~ Pareto(pareto_shape), normalised to mean 1     */
+  double pareto_shape;
+  int32_t mode;              /* MMSB_GEN_*: exact pairwise Bernoulli or ball dropping  */
+  int32_t threads;           /* 0 = hardware concurrency (capped at 16)               */
+} mmsb_gen_params;
+
+/* ---- partition files (read_G, pre_proc.c:49-80; read_G_sparse, CPU IOfn.cpp:21-54) ---------- */
+
+/* Repeated entries: MMSB_DUP_ONE = binary (entry = 1, BINARY pre_proc.c:67-68); MMSB_DUP_LAST =
+ * last value wins (EXPECTED, pre_proc.c:64-65); MMSB_DUP_SUM = summed triplets (IOfn.cpp:51). */
+enum { MMSB_DUP_ONE = 0, MMSB_DUP_LAST = 1, MMSB_DUP_SUM = 2 };
+
+/* Reads a 1-based COO text file into a host CSR block (columns ascending). first_is_row selects
+ * which column indexes rows; weighted reads a third value per entry; feof_repeat re-applies the
+ * last entry as the reference's while(!feof) loops do. Status MMSB_ERR_IO if the file is missing
+ * (reference: exit(1), pre_proc.c:54-58) or an index is out of range. */
+int mmsb_read_coo(const char* path, int32_t n_rows, int32_t n_cols, int32_t first_is_row, int32_t weighted,
+                  int32_t dup_policy, int32_t feof_repeat, mmsb_csr* out);
+void mmsb_csr_free(mmsb_csr* m);
+/* 1-based COO text writer (row<TAB>col[<TAB>value] per nonzero). */
+int mmsb_write_coo(const char* path, const mmsb_csr* m);
+/* One "%5.25e" value per line (write_mat, post_proc.c:45-59; stpm.c:926-951). */
+int mmsb_write_values(const char* path, const double* v, int64_t n);
+/* Nonzeros of a k x n matrix given node-major ([n][k]) as "row\tcol\tvalue" triplets in
+ * column-major order, 6 significant digits (write_pi, CPU IOfn.cpp:57-67). */
+int mmsb_write_pi_triplets(const char* path, const double* pi_nk, int64_t n, int32_t k);
+
+/* Defaults: 1000 nodes per partition, k = 10, alpha0 = 1, p_in 0.6, p_out 0.02, seed 0. */
+void mmsb_gen_params_default(mmsb_gen_params* p);
+/* Seeded synthetic MMSB graph (no reference counterpart: the reference reads files only).
+ * Bit-identical for a given parameter set whatever the thread count. */
+int mmsb_generate(const mmsb_gen_params* p, int32_t keep_truth, mmsb_graph** out);
+void mmsb_graph_free(mmsb_graph* g);
+
+
+/* ------------------------------------------------------------------------------------------
+ * Pipeline configuration (the stpm.h macros as a runtime struct) and the context handle
+ * ------------------------------------------------------------------------------------------ */
+
+enum { MMSB_PREC_FP32 = 0, MMSB_PREC_BF16 = 1 };
+enum { MMSB_M2_PAPER = 0, MMSB_M2_GPU = 1 };
+
+typedef struct stpm_config {
+  int32_t nx, na, nb, nc;     /* NX, NA, NB, NC (stpm.h:72-75)                                   */
+  int32_t ncom;               /* NCOM = k (stpm.h:76)                                            */
+  double alpha_0;             /* ALPHA_0 (stpm.h:78)                                             */
+  double learn_rate;          /* LEARN_RATE = eta (stpm.h:79)                                    */
+  double thresh;              /* THRESH: stop when ||dPhi||_F^2 < thresh ... (stpm.h:94)          */
+  double pinv_tol;            /* PINV_TOL (stpm.h:95), relative to the largest value here        */
+  int64_t sgd_iter_min;       /* ... after at least this many steps (SGD_ITER_MIN, stpm.h:96)    */
+  int64_t sgd_iter_max;       /* hard cap on steps (SGD_ITER_MAX, stpm.h:97); <= 0: epochs only  */
+  int32_t binary;             /* 1 = BINARY two-column files, 0 = EXPECTED three-column (stpm.h:56-57) */
+  int32_t synthetic;          /* SYNTHETIC: G_BA/G_CA present, Pi_A/B/C produced (stpm.h:55)    */
+  char folder_name[512];      /* FOLDER_NAME (stpm.h:47), with trailing '/'                       */
+  char file_prefix[64];       /* "DBLP_" of FILE_A..FILE_CA (stpm.h:63-69)                        */
+  char alpha_0_str[32];       /* ALPHA_0 as spelled in output file names (stpm.h:82-91)          */
+  char learn_rate_str[32];    /* LEARN_RATE as spelled in output file names                      */
+  /* extensions of the reference configuration */
+  int32_t minibatch;          /* nodes per STGD step (reference: 1); rounded up to 64             */
+  int32_t epochs;             /* passes over X                                                   */
+  double theta;               /* weight of the orthogonality term (reference GPU: 1)            */
+  int32_t renorm_every;       /* orthonormal renormalisation every n steps (reference: 1), 0 = never */
+  int32_t ns_iters;           /* Newton-Schulz iterations per renormalisation                    */
+  int32_t precision;          /* MMSB_PREC_*: STGD contraction precision                         */
+  int32_t polish_epochs;      /* FP32 epochs that follow BF16 epochs                             */
+  int32_t oversample;         /* r - k of the randomized range finders                           */
+  int32_t power_iters;        /* power iterations of the range finders                           */
+  int32_t m2_scaling;         /* MMSB_M2_PAPER (CPU Util.cpp:81,109-113) / MMSB_M2_GPU (pre_proc.c:292-302) */
+  double member_tau;          /* membership threshold on the normalised Pi (reference: 0)        */
+  uint64_t seed;              /* replaces time(NULL) / clock() seeding                           */
+  int32_t ba_first_is_row;    /* Gb_a/Gc_a orientation: 1 = first column is the B/C node (CPU order) */
+  int32_t feof_repeat;        /* re-apply the last entry of each input file (while(!feof) quirk) */
+  int32_t device;             /* HIP device ordinal                                              */
+  int32_t verbose;            /* progress lines on stdout                                        */
+} stpm_config;
+
+/* Fills the reference defaults of stpm.h:47-106 (DBLP sizes, NCOM 100, ALPHA_0 1, LEARN_RATE 1e-3,
+ * THRESH 1, PINV_TOL 1e-6, BINARY, SYNTHETIC) plus: minibatch 1024, 1 epoch, theta 1, renorm every
+ * step with 2 Newton-Schulz iterations, FP32, oversampling 10, 1 power iteration, paper M2
+ * scaling, tau 0, seed 0, CPU file orientation. */
+void stpm_config_default(stpm_config* cfg);
+
+typedef struct mmsb_ctx mmsb_ctx;
+
+enum { MMSB_BLOCK_XA = 0, MMSB_BLOCK_XB = 1, MMSB_BLOCK_XC = 2, MMSB_BLOCK_BA = 3, MMSB_BLOCK_CA = 4 };
+
+/* Quantities readable with mmsb_get (copied to host as double, row-major):                   */
+enum {
+  MMSB_Q_MU_A = 0, MMSB_Q_MU_B, MMSB_Q_MU_C,       /* [n_A], [n_B], [n_C] column means         */
+  MMSB_Q_W,                                         /* [n_A x k] whitening matrix               */
+  MMSB_Q_U, MMSB_Q_V, MMSB_Q_SIGMA,                 /* rank-k SVD of P_BC: [n_B x k], [n_C x k], [k] */
+  MMSB_Q_LB, MMSB_Q_LC,                             /* Z_B = L_B U^T, Z_C = L_C V^T: [n_A x k]  */
+  MMSB_Q_LAM_M2,                                    /* [k] top eigenvalues of M2                */
+  MMSB_Q_YA, MMSB_Q_YB, MMSB_Q_YC,                  /* [n_X x k] whitened neighbour vectors     */
+  MMSB_Q_MUW_A, MMSB_Q_MUW_B, MMSB_Q_MUW_C,         /* [k] whitened means                       */
+  MMSB_Q_PHI,                                       /* [k x k] Phi, column i = eigenvector i    */
+  MMSB_Q_FROB,                                      /* [steps] ||dPhi||_F^2 per step            */
+  MMSB_Q_LAMBDA, MMSB_Q_LAMBDA_P, MMSB_Q_ALPHA_HAT, /* [k] eigenvalues, Lambda', alpha_hat      */
+  MMSB_Q_PI_X, MMSB_Q_PI_A, MMSB_Q_PI_B, MMSB_Q_PI_C,     /* [n_p x k] unnormalised Pi          */
+  MMSB_Q_PIN_X, MMSB_Q_PIN_A, MMSB_Q_PIN_B, MMSB_Q_PIN_C, /* [n_p x k] normalised Pi            */
+  MMSB_Q_COUNT_X, MMSB_Q_COUNT_A, MMSB_Q_COUNT_B, MMSB_Q_COUNT_C, /* [n_p] memberships per node */
+  MMSB_Q_TIMES,                                     /* [4] seconds: whiten, stgd, post, total   */
+  MMSB_Q__COUNT
+};
+
+/* Creates a context on cfg->device. Device memory is allocated lazily by each stage. */
+int mmsb_create(const stpm_config* cfg, mmsb_ctx** out);
+void mmsb_destroy(mmsb_ctx* ctx);
+/* Uploads one host CSR block (copied; the caller keeps ownership). Transposes are built here. */
+int mmsb_set_block(mmsb_ctx* ctx, int32_t which, const mmsb_csr* blk);
+/* Reads FILE_A..FILE_CA of the configuration (read_G calls of pre_proc.c:146-148 and
+ * post_proc.c:79-82; G_CA is read from FILE_CA, not FILE_BA as post_proc.c:82 does). */
+int mmsb_read_inputs(mmsb_ctx* ctx);
+/* Whitening and whitened neighbour vectors: the computation of pre_proc (pre_proc.c:151-366). */
+int mmsb_whiten(mmsb_ctx* ctx);
+/* Sets Phi0 (k x k host, column i = eigenvector i); otherwise the seeded orthonormal start. */
+int mmsb_set_phi(mmsb_ctx* ctx, const double* phi);
+/* Supplies y_A, y_B, y_C ([n_X x k] host) and whitened means ([k]) directly, bypassing whitening. */
+int mmsb_set_yvectors(mmsb_ctx* ctx, int64_t n_x, int32_t k, const double* ya, const double* yb, const double* yc,
+                      const double* mua, const double* mub, const double* muc);
+/* STGD (the loop of GPU_Task, stpm.c:497-790): `steps` minibatch steps in `precision`; steps < 0
+ * runs the configured epochs (BF16 epochs followed by polish_epochs in FP32 when precision is
+ * BF16) with the THRESH / SGD_ITER_MIN / SGD_ITER_MAX stopping rules checked once per epoch. */
+int mmsb_stgd(mmsb_ctx* ctx, int64_t steps, int32_t precision);
+/* Eigenvalues, Lambda' and alpha_hat (stpm.c:796-870). */
+int mmsb_eigen(mmsb_ctx* ctx);
+/* Pi_X, Pi_A, Pi_B, Pi_C, normalisation and membership counts (post_proc.c:62-119). */
+int mmsb_post(mmsb_ctx* ctx);
+/* Writes Pi_*, evecs_, evals_, frob_norm_ and sim_info_ files named as stpm.h:82-91. */
+int mmsb_write_outputs(mmsb_ctx* ctx);
+/* Copies a quantity to host memory as double; count is the number of doubles available. */
+int mmsb_get(mmsb_ctx* ctx, int32_t what, double* host, int64_t count);
+/* Number of doubles mmsb_get(what) writes (0 if not computed yet). */
+int64_t mmsb_get_size(mmsb_ctx* ctx, int32_t what);
+/* Blocks until all work of the context has finished. */
+int mmsb_sync(mmsb_ctx* ctx);
+
+/* ---- program-level drop-in (stpm.c / pre_proc.c / post_proc.c) -------------------------- */
+/* Sets the configuration that pre_proc / GPU_Task / post_proc use (the stpm.h macros). */
+int stpm_set_config(const stpm_config* cfg);
+/* pre_proc.c:112: reads the three X blocks and whitens. Returns 0, or a negative status
+ * (the reference exits with 1 on I/O and 2 on CULA errors). */
+int pre_proc(void);
+/* stpm.c:132: STGD, eigenvalues, post_proc and the output files; returns NULL on success or a
+ * pointer to the (negative) status value on failure. */
+void* GPU_Task(void* arg);
+/* post_proc.c:62: Pi_X..Pi_C and their files (reference return type void, declared int at stpm.c:57). */
+int post_proc(void);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,165 @@
+"""GPU parity of the pipeline stages (through the C-ABI) against the FP64 oracle.
+
+Tolerances: tall whitening outputs are FP32 in HBM (FP64 Grams and factorisations), so they agree
+with the FP64 oracle to ~1e-4 relative to the column scale; STGD runs FP32 MFMA contractions and is
+compared step-for-step from identical inputs at 1e-4; integer outputs (means x n_X, membership
+counts away from the threshold) are exact.
+"""
+import numpy as np
+import pytest
+
+from oracle import core, gen
+
+pytestmark = pytest.mark.gpu
+
+
+def rel_err(a, b):
+    return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
+
+
+def make_case(mmsb, n=300, k=4, alpha0=1.0, p_in=0.5, p_out=0.02, seed=0, power_iters=2, oversample=10,
+              minibatch=100, **cfg_extra):
+    g = mmsb.generate(n, n, n, n, k, alpha0, p_in, p_out, seed=seed)
+    cfg = mmsb.config(nx=n, na=n, nb=n, nc=n, ncom=k, alpha_0=alpha0, oversample=oversample,
+                      power_iters=power_iters, seed=seed + 1, minibatch=minibatch, **cfg_extra)
+    bl = {nm: core.Csr(g[nm][0], g[nm][1], g[nm][3], g[nm][4]) for nm in ("XA", "XB", "XC", "BA", "CA")}
+    wcfg = core.WhitenConfig(k=k, alpha0=alpha0, oversample=oversample, power_iters=power_iters, seed=seed + 1,
+                             m2_scaling="paper" if cfg.m2_scaling == 0 else "gpu")
+    return g, cfg, bl, wcfg
+
+
+@pytest.mark.parametrize("k,n,scaling", [(4, 300, "paper"), (10, 1000, "paper"), (6, 400, "gpu")])
+def test_whitening_matches_oracle(mmsb, k, n, scaling):
+    g, cfg, bl, wcfg = make_case(mmsb, n=n, k=k, m2_scaling=scaling, p_in=0.6 if k > 4 else 0.5)
+    print(f"case k={k} n={n} scaling={scaling}", flush=True)
+    p = mmsb.Pipeline(cfg)
+    p.set_graph(g)
+    p.whiten()
+    ref = core.whiten(bl["XA"], bl["XB"], bl["XC"], wcfg)
+    ys = core.yvectors(bl["XA"], bl["XB"], bl["XC"], ref)
+    # column means: exact ratios of integers
+    for nm in ("mu_a", "mu_b", "mu_c"):
+        np.testing.assert_allclose(p.get(nm), ref["mu_" + nm[-1].upper()], rtol=1e-15)
+    np.testing.assert_allclose(p.get("sigma"), ref["sigma"], rtol=1e-5)
+    assert rel_err(p.get("U"), ref["U"]) < 1e-4
+    assert rel_err(p.get("V"), ref["V"]) < 1e-4
+    np.testing.assert_allclose(p.get("lam_m2"), ref["lam_M2"], rtol=1e-4)
+    assert rel_err(p.get("W"), ref["W"]) < 1e-4
+    for nm, key in (("YA", "Y_A"), ("YB", "Y_B"), ("YC", "Y_C")):
+        assert rel_err(p.get(nm), ys[key]) < 1e-4, nm
+    for nm, key in (("muw_a", "mu_A"), ("muw_b", "mu_B"), ("muw_c", "mu_C")):
+        assert rel_err(p.get(nm), ys[key]) < 1e-4, nm
+    # the defining invariant on the oracle's exact dense M2: W^T M2 W = I
+    we = core.whiten_exact(bl["XA"], bl["XB"], bl["XC"], wcfg)
+    W = p.get("W")
+    assert np.abs(W.T @ we["M2"] @ W - np.eye(k)).max() < 1e-3
+    p.close()
+
+
+@pytest.mark.parametrize("k,b,renorm", [(10, 100, 1), (10, 64, 1), (20, 256, 1), (12, 128, 0), (70, 200, 1)])
+def test_stgd_steps_match_oracle(mmsb, k, b, renorm):
+    n = 1000
+    rng = np.random.default_rng(k + b)
+    ya, yb, yc = (rng.standard_normal((n, k)).astype(np.float32).astype(np.float64) for _ in range(3))
+    mua, mub, muc = (0.3 * rng.standard_normal(k) for _ in range(3))
+    phi0 = core.init_phi(7, k)
+    cfg = mmsb.config(nx=n, na=n, nb=n, nc=n, ncom=k, alpha_0=1.3, learn_rate=2e-3, minibatch=b,
+                      renorm_every=renorm, theta=1.0, oversample=0)
+    print(f"case k={k} b={b} renorm={renorm}", flush=True)
+    p = mmsb.Pipeline(cfg)
+    p.set_yvectors(ya, yb, yc, mua, mub, muc)
+    p.set_phi(phi0)
+    steps = 25
+    p.stgd(steps, "fp32")
+    phi_ref, log_ref = core.stgd(ya, yb, yc, mua, mub, muc, phi0, alpha0=1.3, eta=2e-3, batch=b, epochs=1,
+                                 renorm_every=renorm, steps=steps)
+    assert rel_err(p.get("phi"), phi_ref) < 1e-4
+    np.testing.assert_allclose(p.get("frob"), log_ref, rtol=2e-3, atol=1e-12)
+    p.close()
+
+
+def test_stgd_full_epochs_and_wraparound(mmsb):
+    """Several epochs over a batch size that does not divide n_X (last batch partial)."""
+    n, k, b = 1037, 8, 100
+    rng = np.random.default_rng(3)
+    ya, yb, yc = (rng.standard_normal((n, k)).astype(np.float32).astype(np.float64) for _ in range(3))
+    mus = [0.2 * rng.standard_normal(k) for _ in range(3)]
+    phi0 = core.init_phi(1, k)
+    cfg = mmsb.config(nx=n, na=n, nb=n, nc=n, ncom=k, alpha_0=0.7, learn_rate=1e-3, minibatch=b, epochs=3,
+                      oversample=0)
+    p = mmsb.Pipeline(cfg)
+    p.set_yvectors(ya, yb, yc, *mus)
+    p.set_phi(phi0)
+    p.stgd(-1, "fp32")
+    phi_ref, log_ref = core.stgd(ya, yb, yc, *mus, phi0, alpha0=0.7, eta=1e-3, batch=b, epochs=3)
+    assert len(p.get("frob")) == len(log_ref) == 3 * 11
+    assert rel_err(p.get("phi"), phi_ref) < 1e-4
+    p.close()
+
+
+def test_seeded_phi0_matches_oracle(mmsb):
+    n, k = 256, 9
+    rng = np.random.default_rng(0)
+    ys = [rng.standard_normal((n, k)) for _ in range(3)]
+    cfg = mmsb.config(nx=n, na=n, nb=n, nc=n, ncom=k, seed=11, oversample=0, minibatch=64)
+    p = mmsb.Pipeline(cfg)
+    p.set_yvectors(*ys, np.zeros(k), np.zeros(k), np.zeros(k))
+    p.stgd(0, "fp32")
+    np.testing.assert_allclose(p.get("phi"), core.init_phi(11, k), atol=1e-6)
+    p.close()
+
+
+def test_eigen_post_match_oracle(mmsb):
+    g, cfg, bl, wcfg = make_case(mmsb, n=400, k=5, p_in=0.6, p_out=0.02, seed=2)
+    p = mmsb.Pipeline(cfg)
+    p.set_graph(g)
+    p.whiten()
+    ref = core.whiten(bl["XA"], bl["XB"], bl["XC"], wcfg)
+    # a non-orthonormal Phi exercises lambda = ||phi||^3 and Lambda' = ||lambda|| / lambda
+    rng = np.random.default_rng(9)
+    phi = np.linalg.qr(rng.standard_normal((5, 5)))[0] * np.array([1.0, 1.1, 0.9, 1.3, 0.8])
+    p.set_phi(phi)
+    p.eigen()
+    lam, lam_p, ah = core.eigenvalues(phi)
+    np.testing.assert_allclose(p.get("lambda"), lam, rtol=1e-6)
+    np.testing.assert_allclose(p.get("lambda_p"), lam_p, rtol=1e-6)
+    np.testing.assert_allclose(p.get("alpha_hat"), ah, rtol=1e-6)
+    p.post()
+    pis = core.post_proc(ref, phi, lam_p, bl["XA"], bl["BA"], bl["CA"])
+    for nm, P in zip(("pi_x", "pi_a", "pi_b", "pi_c"), pis):
+        assert rel_err(p.get(nm), P) < 2e-4, nm
+        Pn = core.norm_prob_rows(P)
+        got = p.get("pin_" + nm[-1])
+        assert np.abs(got - Pn).max() < 2e-4, nm
+        # membership counts agree wherever no entry sits within 1e-3 of the threshold (tau = 0)
+        cnt = p.get("count_" + nm[-1])
+        want = (Pn > 0).sum(1)
+        clear = ~((Pn > 0) & (Pn < 1e-3)).any(1)
+        assert np.array_equal(cnt[clear], want[clear]), nm
+    p.close()
+
+
+def test_read_inputs_and_write_outputs(mmsb, tmp_path):
+    """The drop-in file path: FOLDER_NAME/DBLP_Gx_a.txt ... in, Pi_* / evecs / evals / frob / sim_info out."""
+    from oracle import io as oio
+    n, k = 200, 3
+    g = mmsb.generate(n, n, n, n, k, 1.0, 0.5, 0.03, seed=4)
+    for nm, stem in (("XA", "Gx_a"), ("XB", "Gx_b"), ("XC", "Gx_c"), ("BA", "Gb_a"), ("CA", "Gc_a")):
+        oio.write_coo(str(tmp_path / f"DBLP_{stem}.txt"), g[nm][0], g[nm][1])
+    cfg = mmsb.config(nx=n, na=n, nb=n, nc=n, ncom=k, folder_name=str(tmp_path) + "/", minibatch=64, epochs=2,
+                      alpha_0_str="1.0", learn_rate_str="1e-3")
+    p = mmsb.Pipeline(cfg)
+    p.read_inputs()
+    p.whiten()
+    p.stgd(-1)
+    p.eigen()
+    p.post()
+    p.write_outputs()
+    for stem in ("Pi_X", "Pi_A", "Pi_B", "Pi_C", "evecs", "evals", "frob_norm", "sim_info"):
+        assert (tmp_path / f"{stem}_{k}_1.0_1e-3.txt").exists(), stem
+    vals = np.loadtxt(tmp_path / f"Pi_X_{k}_1.0_1e-3.txt")
+    np.testing.assert_allclose(vals.reshape(n, k), p.get("pi_x"), rtol=1e-6, atol=1e-9)
+    ev = np.loadtxt(tmp_path / f"evecs_{k}_1.0_1e-3.txt")
+    np.testing.assert_allclose(ev.reshape(k, k).T, p.get("phi"), rtol=1e-6)   # column-major
+    assert np.isclose(np.loadtxt(tmp_path / f"evals_{k}_1.0_1e-3.txt").sum(), 1.0)
+    p.close()
