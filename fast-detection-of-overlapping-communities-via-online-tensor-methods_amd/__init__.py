@@ -75,7 +75,7 @@ class StpmConfigC(ctypes.Structure):
                 ("polish_epochs", ctypes.c_int32), ("oversample", ctypes.c_int32), ("power_iters", ctypes.c_int32),
                 ("m2_scaling", ctypes.c_int32), ("member_tau", ctypes.c_double), ("seed", ctypes.c_uint64),
                 ("ba_first_is_row", ctypes.c_int32), ("feof_repeat", ctypes.c_int32), ("device", ctypes.c_int32),
-                ("verbose", ctypes.c_int32)]
+                ("verbose", ctypes.c_int32), ("use_graphs", ctypes.c_int32)]
 
 
 PREC = {"fp32": 0, "bf16": 1}
@@ -116,8 +116,19 @@ SIGNATURES: dict = {
     "mmsb_write_outputs": (ctypes.c_int, [_vp]),
     "mmsb_get": (ctypes.c_int, [_vp, ctypes.c_int32, _vp, ctypes.c_int64]),
     "mmsb_get_size": (ctypes.c_int64, [_vp, ctypes.c_int32]),
+    "mmsb_get_rows": (ctypes.c_int, [_vp, ctypes.c_int32, ctypes.c_int64, ctypes.c_int64, _vp, ctypes.c_int64]),
+    "mmsb_get_f32": (ctypes.c_int, [_vp, ctypes.c_int32, _vp, ctypes.c_int64]),
     "mmsb_sync": (ctypes.c_int, [_vp]),
     "stpm_set_config": (ctypes.c_int, [ctypes.POINTER(StpmConfigC)]),
+    "mmsb_tensor_coeffs": (ctypes.c_int, [ctypes.c_double, ctypes.c_int32, _vp]),
+    "mmsb_stgd_grad_workspace": (ctypes.c_size_t, [ctypes.c_int32, ctypes.c_int32]),
+    "mmsb_stgd_grad_f64": (ctypes.c_int, [_vp, ctypes.c_int32, ctypes.c_int32, _vp, _vp, _vp, ctypes.c_int32, _vp, _vp, _vp,
+                                          _vp, _vp, ctypes.c_double, _vp, _vp, _vp]),
+    "mmsb_stgd_grad_f32": (ctypes.c_int, [_vp, ctypes.c_int32, ctypes.c_int32, _vp, _vp, _vp, ctypes.c_int32, _vp, _vp, _vp,
+                                          _vp, _vp, ctypes.c_double, _vp, _vp, _vp]),
+    "mmsb_power_method": (ctypes.c_int, [_vp, ctypes.c_int32, ctypes.c_int32, _vp]),
+    "mmsb_grad": (ctypes.c_int, [_vp, ctypes.c_int64, ctypes.c_int32, _vp, ctypes.c_int32, ctypes.c_double, ctypes.c_int32,
+                                 _vp, _vp]),
     "pre_proc": (ctypes.c_int, []),
     "GPU_Task": (_vp, [_vp]),
     "post_proc": (ctypes.c_int, []),
@@ -313,3 +324,61 @@ class Pipeline:
         if name in ("W", "U", "V", "LB", "LC", "YA", "YB", "YC", "phi") or name.startswith("pi"):
             out = out.reshape(-1, self.k)
         return out
+
+    def grad(self, phi, row0, nrows, scaling="gpu", theta=1.0, recompute=False):
+        """Loss and adjoint over X rows [row0, row0 + nrows) (through mmsb_grad)."""
+        import numpy as np
+        a = np.ascontiguousarray(phi, np.float64)
+        g = np.empty_like(a)
+        loss = np.zeros(1)
+        check(lib.mmsb_grad(self.ctx, int(row0), int(nrows), a.ctypes.data, M2_SCALING[scaling], float(theta),
+                            int(recompute), g.ctypes.data, loss.ctypes.data))
+        return float(loss[0]), g
+
+    def power_method(self, iters, orthogonalize=True):
+        import numpy as np
+        lam = np.zeros(self.k)
+        check(lib.mmsb_power_method(self.ctx, int(iters), int(orthogonalize), lam.ctypes.data))
+        return lam
+
+    def get_f32(self, name):
+        import numpy as np
+        n = lib.mmsb_get_size(self.ctx, Q[name])
+        if n <= 0:
+            raise MMSBError(MMSB_ERR_STATE, f"{name} not available")
+        out = np.empty(n, np.float32)
+        check(lib.mmsb_get_f32(self.ctx, Q[name], out.ctypes.data, n))
+        return out.reshape(-1, self.k)
+
+    def get_rows(self, name, row0, nrows):
+        import numpy as np
+        q = Q[name]
+        tall = name in ("W", "U", "V", "LB", "LC", "YA", "YB", "YC") or name.startswith("pi")
+        cols = self.k if tall else 1
+        out = np.empty(nrows * cols, np.float64)
+        check(lib.mmsb_get_rows(self.ctx, q, int(row0), int(nrows), out.ctypes.data, out.size))
+        return out.reshape(nrows, cols) if tall else out
+
+
+def tensor_coeffs(alpha0: float, scaling: str = "gpu"):
+    import numpy as np
+    out = np.zeros(3)
+    check(lib.mmsb_tensor_coeffs(float(alpha0), M2_SCALING[scaling], out.ctypes.data))
+    return out
+
+
+def stgd_grad(ya, yb, yc, mu_a, mu_b, mu_c, phi, coeffs, theta=1.0, stream=None):
+    """Loss and adjoint on device tensors (FP64 or FP32 y, FP64 mu/phi) through mmsb_stgd_grad_*.
+    Returns (loss tensor [1], grad tensor [k, k]) on the same device."""
+    import numpy as np
+    nb, k = ya.shape
+    dev = ya.device
+    work = torch.empty(int(lib.mmsb_stgd_grad_workspace(k, nb)), dtype=torch.uint8, device=dev)
+    grad = torch.empty((k, k), dtype=torch.float64, device=dev)
+    loss = torch.empty(1, dtype=torch.float64, device=dev)
+    co = np.ascontiguousarray(coeffs, np.float64)
+    fn = lib.mmsb_stgd_grad_f64 if ya.dtype == torch.float64 else lib.mmsb_stgd_grad_f32
+    check(fn(stream_handle(stream), k, nb, ptr(ya), ptr(yb), ptr(yc), ya.stride(0), ptr(mu_a), ptr(mu_b), ptr(mu_c),
+             ptr(phi), co.ctypes.data, float(theta), ptr(grad), ptr(loss), ptr(work)))
+    torch.cuda.current_stream().synchronize() if stream is None else stream.synchronize()
+    return loss, grad

@@ -44,4 +44,15 @@ int copy2d_f32(hipStream_t s, int32_t rows, int32_t cols, const float* in, int32
 int spmm_f32(hipStream_t s, int32_t m, int32_t n, int64_t nnz, const int32_t* rowptr, const int32_t* colidx, const float* vals,
              float alpha, const float* B, int32_t ldb, int32_t r, float* Y, int32_t ldy);
 
+size_t grad_workspace_bytes(int32_t k, int32_t nb);
+template <typename TY>
+int stgd_grad(hipStream_t s, int32_t k, int32_t nb, const TY* ya, const TY* yb, const TY* yc, int32_t ldy,
+              const double* mua, const double* mub, const double* muc, const double* phi, double c3, double c1, double c0,
+              double theta, double* grad, double* loss, void* work);
+
+size_t power_workspace_bytes(int32_t k, int32_t chunk);
+int power_iteration(hipStream_t s, int32_t k, int64_t n, const float* ya, const float* yb, const float* yct, int32_t ldy,
+                    int64_t ldyc, const double* mua, const double* mub, const double* muc, double c3, double c1, double c0,
+                    double* phi, double* lam, int orthogonalize, int32_t chunk, void* work, float* yc_rows);
+
 }  // namespace mmsb

@@ -101,6 +101,8 @@ struct mmsb_ctx {
   double* frob = nullptr;
   int64_t frob_cap = 0, steps_done = 0;
   bf16_bits *YA16 = nullptr, *YB16 = nullptr, *YCt16 = nullptr, *cgt16 = nullptr;
+  double* frob_epoch = nullptr;            // per-epoch log written by captured epochs
+  hipGraphExec_t epoch_graph[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};  // [precision][cur]
   float *phiT = nullptr, *phitT = nullptr, *phiaT = nullptr;
   bool y16_valid = false;
   bool have_phi = false;
@@ -463,12 +465,48 @@ static int run_stgd_steps(mmsb_ctx* c, int64_t nsteps, int32_t precision) {
   a.cgt = c->cgt;
   a.cs_part = c->cs_part;
   a.mab = c->mab;
-  if (precision == MMSB_PREC_BF16)
-    TRY(stgd_run_bf16(c->s, a, c->YA16, c->YB16, c->YCt16, c->steps_done, nsteps, c->cfg.renorm_every, c->cfg.ns_iters,
-                      c->phi, c->cur, c->phit, c->gt, c->phia, c->phiT, c->phitT, c->phiaT, c->cgt16, c->frob));
-  else
-    TRY(stgd_run_f32(c->s, a, c->YA, c->YB, c->YCt, c->steps_done, nsteps, c->cfg.renorm_every, c->cfg.ns_iters, c->phi,
-                     c->cur, c->phit, c->gt, c->phia, c->frob));
+  const int64_t spe = (c->n[0] + c->bl - 1) / c->bl;
+  auto launch = [&](int64_t step0, int64_t ns, double* frob_out, int& cur) -> int {
+    if (precision == MMSB_PREC_BF16)
+      return stgd_run_bf16(c->s, a, c->YA16, c->YB16, c->YCt16, step0, ns, c->cfg.renorm_every, c->cfg.ns_iters, c->phi, cur,
+                           c->phit, c->gt, c->phia, c->phiT, c->phitT, c->phiaT, c->cgt16, frob_out);
+    return stgd_run_f32(c->s, a, c->YA, c->YB, c->YCt, step0, ns, c->cfg.renorm_every, c->cfg.ns_iters, c->phi, cur, c->phit,
+                        c->gt, c->phia, frob_out);
+  };
+  // Whole epochs that start at an epoch boundary replay a captured HIP graph of the epoch's
+  // launches (one graph per precision and Phi ping-pong parity); anything else launches directly.
+  const bool renorm_aligned = c->cfg.renorm_every <= 1 || spe % c->cfg.renorm_every == 0;
+  int64_t done = 0;
+  while (done < nsteps) {
+    const int64_t step0 = c->steps_done + done;
+    const bool epoch_aligned = (step0 % spe == 0) && (nsteps - done >= spe) && renorm_aligned && c->cfg.use_graphs;
+    if (epoch_aligned) {
+      TRY(dalloc(c, c->frob_epoch, (size_t)spe));
+      hipGraphExec_t& ge = c->epoch_graph[precision][c->cur];
+      if (!ge) {
+        hipGraph_t g = nullptr;
+        int cur = c->cur;
+        HCHK(hipStreamBeginCapture(c->s, hipStreamCaptureModeThreadLocal));
+        const int st = launch(0, spe, c->frob_epoch, cur);
+        hipError_t e = hipStreamEndCapture(c->s, &g);
+        if (st != MMSB_OK) {
+          if (g) (void)hipGraphDestroy(g);
+          return st;
+        }
+        HCHK(e);
+        HCHK(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
+        HCHK(hipGraphDestroy(g));
+      }
+      HCHK(hipGraphLaunch(ge, c->s));
+      HCHK(hipMemcpyAsync(c->frob + step0, c->frob_epoch, sizeof(double) * spe, hipMemcpyDeviceToDevice, c->s));
+      if (spe & 1) c->cur ^= 1;
+      done += spe;
+    } else {
+      const int64_t ns = std::min<int64_t>(nsteps - done, spe - step0 % spe);
+      TRY(launch(step0, ns, c->frob + step0, c->cur));
+      done += ns;
+    }
+  }
   c->steps_done += nsteps;
   return MMSB_OK;
 }
@@ -578,22 +616,56 @@ static View view_of(mmsb_ctx* c, int what) {
   }
 }
 
-static int copy_out(mmsb_ctx* c, const View& v, double* host) {
+// Device -> host copy of a (possibly strided or transposed) quantity: compacted on the device
+// into one contiguous staging buffer, then a single linear copy (strided 2D copies of ~1e6 short
+// rows into pageable memory are orders of magnitude slower).
+static int stage_compact(mmsb_ctx* c, const View& v, void** staged, size_t* bytes) {
   const size_t esz = v.dtype == 1 ? 8 : 4;
   const int64_t outer = v.trans ? v.cols : v.rows;
   const int64_t inner = v.trans ? v.rows : v.cols;
-  std::vector<unsigned char> buf((size_t)outer * inner * esz);
-  HCHK(hipMemcpy2DAsync(buf.data(), inner * esz, v.p, v.ld * esz, inner * esz, outer, hipMemcpyDeviceToHost, c->s));
-  HCHK(hipStreamSynchronize(c->s));
-  for (int64_t i = 0; i < v.rows; ++i)
-    for (int32_t j = 0; j < v.cols; ++j) {
-      const int64_t src = v.trans ? (int64_t)j * v.rows + i : i * v.cols + j;
+  *bytes = (size_t)outer * inner * esz;
+  if (v.ld == inner) {
+    *staged = nullptr;
+    return MMSB_OK;
+  }
+  void* d = nullptr;
+  HCHK(hipMalloc(&d, std::max<size_t>(*bytes, 1)));
+  hipError_t e = hipMemcpy2DAsync(d, inner * esz, v.p, v.ld * esz, inner * esz, outer, hipMemcpyDeviceToDevice, c->s);
+  if (e != hipSuccess) {
+    (void)hipFree(d);
+    return set_error(MMSB_ERR_HIP, "copy_out: %s", hipGetErrorString(e));
+  }
+  *staged = d;
+  return MMSB_OK;
+}
+
+static int copy_raw(mmsb_ctx* c, const View& v, void* host) {
+  void* staged = nullptr;
+  size_t bytes = 0;
+  TRY(stage_compact(c, v, &staged, &bytes));
+  hipError_t e = hipMemcpyAsync(host, staged ? staged : v.p, bytes, hipMemcpyDeviceToHost, c->s);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->s);
+  if (staged) (void)hipFree(staged);
+  if (e != hipSuccess) return set_error(MMSB_ERR_HIP, "copy_out: %s", hipGetErrorString(e));
+  return MMSB_OK;
+}
+
+static int copy_out(mmsb_ctx* c, const View& v, double* host) {
+  const size_t esz = v.dtype == 1 ? 8 : 4;
+  std::vector<unsigned char> buf((size_t)v.rows * v.cols * esz);
+  TRY(copy_raw(c, v, buf.data()));
+  const int64_t total = v.rows * v.cols;
+  parallel_for(total, default_threads(), [&](int64_t lo, int64_t hi) {
+    for (int64_t t = lo; t < hi; ++t) {
+      const int64_t i = t / v.cols, j = t % v.cols;
+      const int64_t src = v.trans ? j * v.rows + i : t;
       double x;
       if (v.dtype == 0) x = reinterpret_cast<const float*>(buf.data())[src];
       else if (v.dtype == 1) x = reinterpret_cast<const double*>(buf.data())[src];
       else x = reinterpret_cast<const int32_t*>(buf.data())[src];
-      host[i * v.cols + j] = x;
+      host[t] = x;
     }
+  });
   return MMSB_OK;
 }
 
@@ -657,6 +729,7 @@ extern "C" void stpm_config_default(stpm_config* c) {
   c->feof_repeat = 0;
   c->device = 0;
   c->verbose = 0;
+  c->use_graphs = 1;
 }
 
 extern "C" int mmsb_create(const stpm_config* cfg, mmsb_ctx** out) {
@@ -686,6 +759,9 @@ extern "C" void mmsb_destroy(mmsb_ctx* c) {
   if (!c) return;
   (void)hipSetDevice(c->cfg.device);
   if (c->s) (void)hipStreamSynchronize(c->s);
+  for (auto& row : c->epoch_graph)
+    for (hipGraphExec_t& g : row)
+      if (g) (void)hipGraphExecDestroy(g);
   for (void* p : c->allocs) (void)hipFree(p);
   if (c->s) (void)hipStreamDestroy(c->s);
   delete c;
@@ -825,6 +901,127 @@ extern "C" int mmsb_stgd(mmsb_ctx* c, int64_t steps, int32_t precision) {
   return MMSB_OK;
 }
 
+extern "C" int mmsb_grad(mmsb_ctx* c, int64_t row0, int32_t nrows, const double* phi, int32_t scaling, double theta,
+                         int32_t recompute, double* grad, double* loss) {
+  clear_error();
+  MMSB_CHECK_ARG(c && phi && grad && loss, "grad: null argument");
+  MMSB_CHECK_ARG(nrows >= 1 && row0 >= 0 && row0 + nrows <= c->n[0], "grad: rows [%lld, %lld) outside X", (long long)row0,
+                 (long long)(row0 + nrows));
+  if (recompute && !c->whitened) return set_error(MMSB_ERR_STATE, "grad: recomputing needs the whitening factors");
+  if (!recompute && !c->have_y) return set_error(MMSB_ERR_STATE, "grad: neighbour vectors not available");
+  HCHK(hipSetDevice(c->cfg.device));
+  hipStream_t s = c->s;
+  const int32_t k = c->k, kp = c->kp;
+  float *ya = nullptr, *yb = nullptr, *yc = nullptr;
+  double *dphi = nullptr, *dgrad = nullptr, *dloss = nullptr;
+  unsigned char* work = nullptr;
+  const size_t wb = grad_workspace_bytes(k, nrows);
+  // scratch of this call (freed before returning)
+  auto cleanup = [&]() {
+    for (void* p : {(void*)ya, (void*)yb, (void*)yc, (void*)dphi, (void*)dgrad, (void*)dloss, (void*)work})
+      if (p) (void)hipFree(p);
+  };
+  hipError_t e = hipSuccess;
+  const size_t ybytes = sizeof(float) * (size_t)nrows * kp;
+  for (float** p : {&ya, &yb, &yc})
+    if (e == hipSuccess) e = hipMalloc(p, ybytes);
+  if (e == hipSuccess) e = hipMalloc(&dphi, sizeof(double) * k * k);
+  if (e == hipSuccess) e = hipMalloc(&dgrad, sizeof(double) * k * k);
+  if (e == hipSuccess) e = hipMalloc(&dloss, sizeof(double));
+  if (e == hipSuccess) e = hipMalloc(&work, wb);
+  if (e != hipSuccess) {
+    cleanup();
+    return set_error(MMSB_ERR_NOMEM, "grad: %s", hipGetErrorString(e));
+  }
+  int st = MMSB_OK;
+  if (recompute) {  // y rows = (G_X* rows) x (W, Z_B^T W, Z_C^T W)
+    const DevCsr* g[3] = {&c->blk[0], &c->blk[1], &c->blk[2]};
+    const float* rhs[3] = {c->W, c->MB, c->MC};
+    float* out[3] = {ya, yb, yc};
+    for (int i = 0; i < 3 && st == MMSB_OK; ++i)
+      st = spmm_f32(s, nrows, g[i]->n_cols, g[i]->nnz, g[i]->rowptr + row0, g[i]->colidx, g[i]->vals, 1.f, rhs[i], kp, k,
+                    out[i], kp);
+  } else {
+    e = hipMemcpy2DAsync(ya, sizeof(float) * kp, c->YA + row0 * kp, sizeof(float) * kp, sizeof(float) * kp, nrows,
+                         hipMemcpyDeviceToDevice, s);
+    if (e == hipSuccess)
+      e = hipMemcpy2DAsync(yb, sizeof(float) * kp, c->YB + row0 * kp, sizeof(float) * kp, sizeof(float) * kp, nrows,
+                           hipMemcpyDeviceToDevice, s);
+    if (e != hipSuccess) st = set_error(MMSB_ERR_HIP, "grad: %s", hipGetErrorString(e));
+    if (st == MMSB_OK) st = transpose_f32(s, k, nrows, c->YCt + row0, c->n_pad, yc, kp);  // [k x n] -> [n x k]
+  }
+  double co[3];
+  if (st == MMSB_OK) st = mmsb_tensor_coeffs(c->cfg.alpha_0, scaling, co);
+  if (st == MMSB_OK) {
+    e = hipMemcpyAsync(dphi, phi, sizeof(double) * k * k, hipMemcpyHostToDevice, s);
+    if (e != hipSuccess) st = set_error(MMSB_ERR_HIP, "grad: %s", hipGetErrorString(e));
+  }
+  if (st == MMSB_OK)
+    st = stgd_grad<float>(s, k, nrows, ya, yb, yc, kp, c->muw[0], c->muw[1], c->muw[2], dphi, co[0], co[1], co[2], theta, dgrad,
+                          dloss, work);
+  if (st == MMSB_OK) {
+    e = hipMemcpyAsync(grad, dgrad, sizeof(double) * k * k, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(loss, dloss, sizeof(double), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) st = set_error(MMSB_ERR_HIP, "grad: %s", hipGetErrorString(e));
+  } else {
+    (void)hipStreamSynchronize(s);
+  }
+  cleanup();
+  return st;
+}
+
+extern "C" int mmsb_power_method(mmsb_ctx* c, int32_t iters, int32_t orthogonalize, double* lambda) {
+  clear_error();
+  MMSB_CHECK_ARG(c && lambda && iters >= 0, "power_method: bad arguments");
+  if (!c->have_y) return set_error(MMSB_ERR_STATE, "power_method: neighbour vectors not available");
+  HCHK(hipSetDevice(c->cfg.device));
+  TRY(alloc_stgd(c));
+  if (!c->have_phi) TRY(init_phi(c));
+  const int32_t k = c->k, kp = c->kp;
+  const int32_t chunk = (int32_t)std::min<int64_t>(c->n[0], 65536);
+  double co[3];
+  TRY(mmsb_tensor_coeffs(c->cfg.alpha_0, MMSB_M2_PAPER, co));
+  // Phi (FP32, kp-padded) <-> compact FP64 k x k
+  std::vector<float> hf((size_t)kp * kp);
+  std::vector<double> hd((size_t)k * k);
+  HCHK(hipMemcpy(hf.data(), c->phi[c->cur], sizeof(float) * hf.size(), hipMemcpyDeviceToHost));
+  for (int32_t m = 0; m < k; ++m)
+    for (int32_t i = 0; i < k; ++i) hd[(size_t)m * k + i] = hf[(size_t)m * kp + i];
+  double *dphi = nullptr, *dlam = nullptr;
+  float* ycr = nullptr;
+  void* work = nullptr;
+  auto cleanup = [&]() {
+    for (void* p : {(void*)dphi, (void*)dlam, (void*)ycr, work})
+      if (p) (void)hipFree(p);
+  };
+  hipError_t e = hipMalloc(&dphi, sizeof(double) * k * k);
+  if (e == hipSuccess) e = hipMalloc(&dlam, sizeof(double) * k);
+  if (e == hipSuccess) e = hipMalloc(&ycr, sizeof(float) * (size_t)chunk * kp);
+  if (e == hipSuccess) e = hipMalloc(&work, power_workspace_bytes(k, chunk));
+  if (e == hipSuccess) e = hipMemcpyAsync(dphi, hd.data(), sizeof(double) * k * k, hipMemcpyHostToDevice, c->s);
+  if (e != hipSuccess) {
+    cleanup();
+    return set_error(MMSB_ERR_NOMEM, "power_method: %s", hipGetErrorString(e));
+  }
+  int st = MMSB_OK;
+  for (int32_t it = 0; it < iters && st == MMSB_OK; ++it)
+    st = power_iteration(c->s, k, c->n[0], c->YA, c->YB, c->YCt, kp, c->n_pad, c->muw[0], c->muw[1], c->muw[2], co[0], co[1],
+                         co[2], dphi, dlam, orthogonalize, chunk, work, ycr);
+  if (st == MMSB_OK) {
+    e = hipMemcpyAsync(hd.data(), dphi, sizeof(double) * k * k, hipMemcpyDeviceToHost, c->s);
+    if (e == hipSuccess) e = hipMemcpyAsync(lambda, dlam, sizeof(double) * k, hipMemcpyDeviceToHost, c->s);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->s);
+    if (e != hipSuccess) st = set_error(MMSB_ERR_HIP, "power_method: %s", hipGetErrorString(e));
+  } else {
+    (void)hipStreamSynchronize(c->s);
+  }
+  cleanup();
+  if (st != MMSB_OK) return st;
+  if (iters == 0) std::fill(lambda, lambda + k, 0.0);
+  return mmsb_set_phi(c, hd.data());
+}
+
 extern "C" int mmsb_eigen(mmsb_ctx* c) {
   clear_error();
   MMSB_CHECK_ARG(c, "eigen: null context");
@@ -868,6 +1065,35 @@ extern "C" int mmsb_get(mmsb_ctx* c, int32_t what, double* host, int64_t count) 
   MMSB_CHECK_ARG(count >= v.rows * v.cols, "get: buffer of %lld doubles, need %lld", (long long)count, (long long)(v.rows * v.cols));
   HCHK(hipSetDevice(c->cfg.device));
   return copy_out(c, v, host);
+}
+
+extern "C" int mmsb_get_rows(mmsb_ctx* c, int32_t what, int64_t row0, int64_t nrows, double* host, int64_t count) {
+  clear_error();
+  MMSB_CHECK_ARG(c && host, "get_rows: null argument");
+  View v = view_of(c, what);
+  if (!v.p) return set_error(MMSB_ERR_STATE, "get_rows: quantity %d not computed", what);
+  MMSB_CHECK_ARG(row0 >= 0 && nrows >= 0 && row0 + nrows <= v.rows, "get_rows: rows [%lld, %lld) outside [0, %lld)",
+                 (long long)row0, (long long)(row0 + nrows), (long long)v.rows);
+  MMSB_CHECK_ARG(count >= nrows * v.cols, "get_rows: buffer too small");
+  const size_t esz = v.dtype == 1 ? 8 : 4;
+  if (v.trans)
+    v.p = static_cast<const unsigned char*>(v.p) + row0 * esz;
+  else
+    v.p = static_cast<const unsigned char*>(v.p) + row0 * v.ld * esz;
+  v.rows = nrows;
+  HCHK(hipSetDevice(c->cfg.device));
+  return copy_out(c, v, host);
+}
+
+extern "C" int mmsb_get_f32(mmsb_ctx* c, int32_t what, float* host, int64_t count) {
+  clear_error();
+  MMSB_CHECK_ARG(c && host, "get_f32: null argument");
+  const View v = view_of(c, what);
+  if (!v.p) return set_error(MMSB_ERR_STATE, "get_f32: quantity %d not computed", what);
+  MMSB_CHECK_ARG(v.dtype == 0 && !v.trans, "get_f32: quantity %d is not a row-major FP32 matrix", what);
+  MMSB_CHECK_ARG(count >= v.rows * v.cols, "get_f32: buffer too small");
+  HCHK(hipSetDevice(c->cfg.device));
+  return copy_raw(c, v, host);
 }
 
 extern "C" int mmsb_sync(mmsb_ctx* c) {
@@ -975,5 +1201,6 @@ extern "C" void* GPU_Task(void*) {
   if (st == MMSB_OK) st = mmsb_eigen(g_ctx);
   if (st == MMSB_OK) st = post_proc();
   g_task_status = st;
+  if (st != MMSB_OK) std::fprintf(stderr, "GPU_Task: %s\n", mmsb_last_error());  // the message is thread-local
   return st == MMSB_OK ? nullptr : &g_task_status;
 }

@@ -27,8 +27,8 @@ struct StgdArgs {
 };
 
 // Runs nsteps minibatch steps starting at global step step0 (batch = step mod steps-per-epoch).
-// phi_buf[cur] holds Phi on entry; on exit phi_buf[cur] holds the result. frob_log[step] receives
-// ||Phi_{t+1} - Phi_t||_F^2.
+// phi_buf[cur] holds Phi on entry; on exit phi_buf[cur] holds the result. frob_log[st] receives
+// ||Phi_{t+1} - Phi_t||_F^2 of the st-th step of this call. Every launch is capturable in a HIP graph.
 int stgd_run_f32(hipStream_t s, const StgdArgs& args, const float* YA, const float* YB, const float* YCt, int64_t step0,
                  int64_t nsteps, int32_t renorm_every, int32_t ns_iters, float* phi_buf[2], int& cur, float* phit, float* gt,
                  float* phia, double* frob_log);

@@ -280,7 +280,7 @@ static int run_steps(hipStream_t s, StgdArgs a, const TY* YA, const TY* YB, cons
     hipLaunchKernelGGL((stgd_phase2_kernel<TY>), dim3(ntk * ntk), dim3(256), 0, s, a, YCt, row0, nb);
     MMSB_LAUNCH_CHECK();
     float* out = phi_buf[cur ^ 1];
-    double* frob = frob_log + gstep;
+    double* frob = frob_log + st;
     const bool renorm = renorm_every > 0 && ((gstep + 1) % renorm_every == 0) && ns_iters > 0;
     if (renorm) {
       const float* x = phit;
@@ -312,7 +312,7 @@ int stgd_run_f32(hipStream_t s, const StgdArgs& args, const float* YA, const flo
   const int64_t spe = ceil_div(args.n_x, args.bl);
   MMSB_CHECK_ARG(args.n_pad >= (spe - 1) * args.bl + args.b, "stgd: n_pad too small");
   const int64_t nfrob = nsteps;
-  hipLaunchKernelGGL(zero_f64_kernel, dim3((unsigned)ceil_div(nfrob, 256)), dim3(256), 0, s, nfrob, frob_log + step0);
+  hipLaunchKernelGGL(zero_f64_kernel, dim3((unsigned)ceil_div(nfrob, 256)), dim3(256), 0, s, nfrob, frob_log);
   MMSB_LAUNCH_CHECK();
   return run_steps<float>(s, args, YA, YB, YCt, step0, nsteps, spe, renorm_every, ns_iters, phi_buf, cur, phit, gt, phia, frob_log);
 }

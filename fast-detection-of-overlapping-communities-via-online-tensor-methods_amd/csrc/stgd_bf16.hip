@@ -285,7 +285,7 @@ int stgd_run_bf16(hipStream_t s, const StgdArgs& args, const bf16_bits* YA_, con
   MMSB_CHECK_ARG(args.n_pad >= (spe - 1) * args.bl + args.b, "stgd: n_pad too small");
   const int kp = args.kp, ntk = kp / 64, ntb = args.b / 64;
   const int64_t nel = (int64_t)kp * kp;
-  hipLaunchKernelGGL(bf_zero_f64, dim3((unsigned)ceil_div(nsteps, 256)), dim3(256), 0, s, nsteps, frob_log + step0);
+  hipLaunchKernelGGL(bf_zero_f64, dim3((unsigned)ceil_div(nsteps, 256)), dim3(256), 0, s, nsteps, frob_log);
   MMSB_LAUNCH_CHECK();
   // Phi^T of the current Phi
   hipLaunchKernelGGL(transpose_sq_kernel, dim3((unsigned)ceil_div(nel, 256)), dim3(256), 0, s, kp, phi_buf[cur], phiT);
@@ -302,7 +302,7 @@ int stgd_run_bf16(hipStream_t s, const StgdArgs& args, const bf16_bits* YA_, con
     hipLaunchKernelGGL(bf_phase2_kernel, dim3(ntk * ntk), dim3(256), 0, s, a, YCt, cgt16, phitT, row0, nb);
     MMSB_LAUNCH_CHECK();
     float* out = phi_buf[cur ^ 1];
-    double* frob = frob_log + gstep;
+    double* frob = frob_log + st;
     const bool renorm = renorm_every > 0 && ((gstep + 1) % renorm_every == 0) && ns_iters > 0;
     if (renorm) {
       const float* x = phit;

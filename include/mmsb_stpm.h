@@ -61,6 +61,35 @@ int mmsb_spmm_f32(void* stream, int32_t m, int32_t n, int64_t nnz, const int32_t
                   int32_t ldb, int32_t r, float* Y, int32_t ldy);
 
 
+/*
+ * Coefficients (c3, c1, c0) of the empirical third moment
+ *   T = c3 E[y_A x y_B x y_C] + c1 (E[y_A x y_B x mu_C] + E[y_A x mu_B x y_C] + E[mu_A x y_B x y_C])
+ *       + c0 mu_A x mu_B x mu_C
+ * scaling MMSB_M2_GPU: (1, -a0/(a0+2), 2a0^2/((a0+1)(a0+2))), the 4x4 matrices of init_buff
+ * (stpm.c:1026-1038); MMSB_M2_PAPER: ((a0+1)(a0+2)/2, -a0(a0+1)/2, a0^2) (The_second_term,
+ * CPU Util.cpp:289-298).
+ */
+int mmsb_tensor_coeffs(double alpha0, int32_t scaling, double* out3);
+
+/*
+ * Loss and exact adjoint of the implicit tensor objective over one minibatch of nb nodes:
+ *   L = theta/2 sum_ij <phi_i,phi_j>^3 - sum_i T_b(phi_i, phi_i, phi_i)
+ *   grad = 3 theta Phi (G o G) - [T(I,phi,phi) + T(phi,I,phi) + T(phi,phi,I)],   G = Phi^T Phi
+ * Replaces the reference's analytic gradient Diff_Loss / The_second_term (CPU Util.cpp:257-304),
+ * which keeps only 3 T(phi,phi,I) and so equals this adjoint for symmetric inputs only; the GPU
+ * update of stpm.c:649-691 is that expression / 3. Device pointers: y_* [nb x ldy] row-major
+ * (FP64 or FP32), mu_* [k], phi [k x k] row-major (column i = eigenvector i), coeffs[3] (host),
+ * grad [k x k], loss [1]; work holds mmsb_stgd_grad_workspace(k, nb) bytes. Allocates nothing.
+ * Accumulation in FP64.
+ */
+size_t mmsb_stgd_grad_workspace(int32_t k, int32_t nb);
+int mmsb_stgd_grad_f64(void* stream, int32_t k, int32_t nb, const double* ya, const double* yb, const double* yc,
+                       int32_t ldy, const double* mu_a, const double* mu_b, const double* mu_c, const double* phi,
+                       const double* coeffs, double theta, double* grad, double* loss, void* work);
+int mmsb_stgd_grad_f32(void* stream, int32_t k, int32_t nb, const float* ya, const float* yb, const float* yc,
+                       int32_t ldy, const double* mu_a, const double* mu_b, const double* mu_c, const double* phi,
+                       const double* coeffs, double theta, double* grad, double* loss, void* work);
+
 /* ------------------------------------------------------------------------------------------
  * Host graph containers and the seeded MMSB generator
  * ------------------------------------------------------------------------------------------ */
@@ -169,6 +198,7 @@ typedef struct stpm_config {
   int32_t feof_repeat;        /* re-apply the last entry of each input file (while(!feof) quirk) */
   int32_t device;             /* HIP device ordinal                                              */
   int32_t verbose;            /* progress lines on stdout                                        */
+  int32_t use_graphs;         /* replay whole STGD epochs as captured HIP graphs                 */
 } stpm_config;
 
 /* Fills the reference defaults of stpm.h:47-106 (DBLP sizes, NCOM 100, ALPHA_0 1, LEARN_RATE 1e-3,
@@ -219,6 +249,18 @@ int mmsb_set_yvectors(mmsb_ctx* ctx, int64_t n_x, int32_t k, const double* ya, c
  * runs the configured epochs (BF16 epochs followed by polish_epochs in FP32 when precision is
  * BF16) with the THRESH / SGD_ITER_MIN / SGD_ITER_MAX stopping rules checked once per epoch. */
 int mmsb_stgd(mmsb_ctx* ctx, int64_t steps, int32_t precision);
+/* Loss and adjoint (mmsb_stgd_grad_f32) over X rows [row0, row0 + nrows) for a host Phi [k x k].
+ * recompute != 0 rebuilds those whitened rows from CSR (y = G_X* rows times W, Z_B^T W, Z_C^T W)
+ * instead of reading the stored neighbour vectors -- the memory-saving form for large n k. */
+int mmsb_grad(mmsb_ctx* ctx, int64_t row0, int32_t nrows, const double* phi, int32_t scaling, double theta,
+              int32_t recompute, double* grad, double* loss);
+/* Batch implicit tensor power method (CPU tensorDecom_alpha0, Util.cpp:131-206) on the stored
+ * neighbour vectors, as an alternative to STGD: `iters` iterations of phi_j <- T(I, phi_j, phi_j)
+ * over all of X with the paper coefficients, lambda_j = ||.||, column normalisation, and, when
+ * orthogonalize != 0, the symmetric orthonormalisation Phi (Phi^T Phi)^-1/2 the reference lacks
+ * (without it the columns collapse onto the dominant component). Starts from the current Phi
+ * (or the seeded Phi0); leaves Phi in the context and lambda in *lambda (host, k values). */
+int mmsb_power_method(mmsb_ctx* ctx, int32_t iters, int32_t orthogonalize, double* lambda);
 /* Eigenvalues, Lambda' and alpha_hat (stpm.c:796-870). */
 int mmsb_eigen(mmsb_ctx* ctx);
 /* Pi_X, Pi_A, Pi_B, Pi_C, normalisation and membership counts (post_proc.c:62-119). */
@@ -227,6 +269,11 @@ int mmsb_post(mmsb_ctx* ctx);
 int mmsb_write_outputs(mmsb_ctx* ctx);
 /* Copies a quantity to host memory as double; count is the number of doubles available. */
 int mmsb_get(mmsb_ctx* ctx, int32_t what, double* host, int64_t count);
+/* Rows [row0, row0 + nrows) of a row-indexed quantity (tall matrices, vectors, counts). */
+int mmsb_get_rows(mmsb_ctx* ctx, int32_t what, int64_t row0, int64_t nrows, double* host, int64_t count);
+/* Copies an FP32-resident quantity (tall matrices: W, U, V, L_B, L_C, Y_A, Y_B, Pi_*) in its
+ * native precision, compact row-major [rows x k]. */
+int mmsb_get_f32(mmsb_ctx* ctx, int32_t what, float* host, int64_t count);
 /* Number of doubles mmsb_get(what) writes (0 if not computed yet). */
 int64_t mmsb_get_size(mmsb_ctx* ctx, int32_t what);
 /* Blocks until all work of the context has finished. */

@@ -163,3 +163,102 @@ def test_read_inputs_and_write_outputs(mmsb, tmp_path):
     np.testing.assert_allclose(ev.reshape(k, k).T, p.get("phi"), rtol=1e-6)   # column-major
     assert np.isclose(np.loadtxt(tmp_path / f"evals_{k}_1.0_1e-3.txt").sum(), 1.0)
     p.close()
+
+
+@pytest.mark.parametrize("k,b", [(10, 128), (70, 200), (24, 64)])
+def test_stgd_bf16_tracks_oracle(mmsb, k, b):
+    """BF16 contractions (FP32 accumulation, split-BF16 k^3 products): Phi within 5e-3 of the FP64
+    oracle after 25 steps, and the FP32 polish epoch brings it back to the FP32 trajectory's level."""
+    n = 1000
+    rng = np.random.default_rng(100 + k)
+    ya, yb, yc = (rng.standard_normal((n, k)).astype(np.float32).astype(np.float64) for _ in range(3))
+    mus = [0.3 * rng.standard_normal(k) for _ in range(3)]
+    phi0 = core.init_phi(3, k)
+    cfg = mmsb.config(nx=n, na=n, nb=n, nc=n, ncom=k, alpha_0=1.3, learn_rate=2e-3, minibatch=b, oversample=0)
+    print(f"case k={k} b={b}", flush=True)
+    p = mmsb.Pipeline(cfg)
+    p.set_yvectors(ya, yb, yc, *mus)
+    p.set_phi(phi0)
+    p.stgd(25, "bf16")
+    phi_ref, log_ref = core.stgd(ya, yb, yc, *mus, phi0, alpha0=1.3, eta=2e-3, batch=b, epochs=1, steps=25)
+    got = p.get("phi")
+    assert rel_err(got, phi_ref) < 5e-3
+    assert np.abs(got.T @ got - np.eye(k)).max() < 1e-4      # split-BF16 renormalisation keeps Phi orthonormal
+    assert rel_err(p.get("frob"), log_ref) < 0.05
+    p.close()
+
+
+def test_get_rows_matches_get(mmsb):
+    n, k = 300, 5
+    rng = np.random.default_rng(1)
+    ys = [rng.standard_normal((n, k)) for _ in range(3)]
+    cfg = mmsb.config(nx=n, na=n, nb=n, nc=n, ncom=k, oversample=0, minibatch=64)
+    p = mmsb.Pipeline(cfg)
+    p.set_yvectors(*ys, np.zeros(k), np.zeros(k), np.zeros(k))
+    for nm, y in zip(("YA", "YB", "YC"), ys):
+        np.testing.assert_allclose(p.get_rows(nm, 37, 100), y[37:137], rtol=1e-6)
+    p.close()
+
+
+@pytest.mark.parametrize("prec", ["fp32", "bf16"])
+def test_epoch_graph_replays_match_eager(mmsb, prec):
+    """Captured epoch graphs, replayed several times with eager work and host syncs in between,
+    give bit-identical Phi to direct launches after every epoch (odd and even steps per epoch)."""
+    for n in (1024, 960):                         # 16 and 15 steps per epoch at b = 64
+        k, b = 16, 64
+        rng = np.random.default_rng(n)
+        ys = [rng.standard_normal((n, k)) for _ in range(3)]
+        mus = [0.2 * rng.standard_normal(k) for _ in range(3)]
+        phi0 = core.init_phi(5, k)
+        ps = []
+        for use in (1, 0):
+            cfg = mmsb.config(nx=n, na=n, nb=n, nc=n, ncom=k, oversample=0, minibatch=b, learn_rate=3e-3,
+                              use_graphs=use)
+            p = mmsb.Pipeline(cfg)
+            p.set_yvectors(*ys, *mus)
+            p.set_phi(phi0)
+            ps.append(p)
+        spe = n // b
+        for epoch in range(4):
+            for p in ps:
+                p.stgd(spe, prec)
+                p.eigen()                       # eager work between replays
+            a, e = ps[0].get("phi"), ps[1].get("phi")
+            assert np.array_equal(a, e), f"n={n} epoch {epoch}"
+        np.testing.assert_allclose(ps[0].get("frob"), ps[1].get("frob"), rtol=1e-12)
+        for p in ps:
+            p.close()
+
+
+@pytest.mark.parametrize("orth", [True, False])
+def test_power_method_matches_oracle(mmsb, orth):
+    """Batch implicit power method (CPU tensorDecom_alpha0 with the orthogonalisation added)."""
+    n, k = 700, 6
+    rng = np.random.default_rng(21)
+    ys = [rng.standard_normal((n, k)).astype(np.float32).astype(np.float64) for _ in range(3)]
+    mus = [0.3 * rng.standard_normal(k) for _ in range(3)]
+    phi0 = core.init_phi(2, k)
+    cfg = mmsb.config(nx=n, na=n, nb=n, nc=n, ncom=k, alpha_0=0.8, oversample=0, minibatch=64)
+    p = mmsb.Pipeline(cfg)
+    p.set_yvectors(*ys, *mus)
+    p.set_phi(phi0)
+    lam = p.power_method(5, orthogonalize=orth)
+    phi_ref, lam_ref = core.power_method(*ys, *mus, phi0, alpha0=0.8, iters=5, orthogonalize=orth)
+    np.testing.assert_allclose(lam, lam_ref, rtol=1e-5)
+    assert rel_err(p.get("phi"), phi_ref) < 1e-5
+    p.close()
+
+
+def test_power_method_recovers_odeco(mmsb):
+    from tests.test_oracle import odeco_data
+    V, Y = odeco_data(k=4, n=2000, seed=3)
+    z = np.zeros(4)
+    rng = np.random.default_rng(4)
+    phi0 = np.linalg.qr(V + 0.2 * rng.standard_normal((4, 4)))[0]
+    cfg = mmsb.config(nx=2000, na=2000, nb=2000, nc=2000, ncom=4, alpha_0=1e-9, oversample=0, minibatch=64)
+    p = mmsb.Pipeline(cfg)
+    p.set_yvectors(Y, Y, Y, z, z, z)
+    p.set_phi(phi0)
+    p.power_method(30)
+    assert np.allclose(np.sort(np.abs(V.T @ p.get("phi")).max(0)), 1.0, atol=1e-5)
+    p.close()
