@@ -113,7 +113,7 @@ def main():
     # ---- timed STGD epochs in both precisions --------------------------------------------------
     spe = math.ceil(n / b)
     rates = {}
-    precs = ("fp32", "bf16") if b % 8 == 0 else ("fp32",)
+    precs = ("fp32", "bf16")
     for prec in precs:
         for _ in range(args.warmup):
             p.stgd(spe, prec)
@@ -129,8 +129,17 @@ def main():
     nodes_per_s, ms_per_epoch = rates[prec_used]
     flops_epoch = 6.0 * k * k * n + 4.0 * k ** 3 * spe           # algorithmic STGD flops / epoch
     ns_flops_epoch = 2 * 2 * 2.0 * k ** 3 * spe                  # Newton-Schulz renormalisation (2 iters)
-    achieved = flops_epoch / (ms_per_epoch * 1e-3)
+    epoch_achieved = flops_epoch / (ms_per_epoch * 1e-3)
     peak = PEAK[prec_used]
+
+    # ---- per-kernel device time: one more epoch launched directly, HIP events around each phase ----
+    # on the context stream (the stream every STGD kernel runs on). P1 -- the Y_A Phi, Y_B Phi
+    # contractions fused with the C_g epilogue, plus (Phi^T Phi)^o2 -- is the largest kernel.
+    ph_ms = p.stgd_timed(spe, prec_used)
+    phases_us = {nm: 1e3 * t / spe for nm, t in zip(("p1", "p2", "renorm"), ph_ms)}
+    p1_flops = (4.0 * k * k * n + 2.0 * k ** 3 * spe) / spe       # per launch: 2 b k^2 (P_A) + 2 b k^2 (P_B) + 2 k^3
+    p1_achieved = p1_flops / (phases_us["p1"] * 1e-6)
+    log(f"phases (us/step): {phases_us}; P1 {p1_achieved / 1e12:.1f} TFLOP/s")
 
     # ---- Pi agreement: 2 epochs FP32 vs 1 epoch BF16 + 1 FP32 polish epoch from the same Phi0 ------
     pis = {}
@@ -173,10 +182,12 @@ def main():
         "stgd_nodes_per_s": {pr: v[0] for pr, v in rates.items()},
         "bf16_speedup_vs_fp32": rates["bf16"][0] / rates["fp32"][0] if "bf16" in rates else None,
         "pi_rel_diff_bf16_polish_vs_fp32": pi_err,
-        "roofline": {"bound": "mfma", "achieved": achieved / 1e12, "peak": peak / 1e12, "unit": "TFLOP/s",
-                     "frac": achieved / peak, "traffic": None,
-                     "kernel": "STGD epoch (all phase kernels)", "flops_per_epoch": flops_epoch,
-                     "renorm_flops_per_epoch": ns_flops_epoch},
+        "roofline": {"bound": "mfma", "achieved": p1_achieved / 1e12, "peak": peak / 1e12, "unit": "TFLOP/s",
+                     "frac": p1_achieved / peak, "traffic": None,
+                     "kernel": f"p1_kernel ({prec_used}): Y_A Phi, Y_B Phi + C_g epilogue, (Phi^T Phi)^o2",
+                     "flops_per_launch": p1_flops, "avg_us": phases_us["p1"], "phases_us_per_step": phases_us,
+                     "epoch_tflops": epoch_achieved / 1e12, "epoch_frac": epoch_achieved / peak,
+                     "flops_per_epoch": flops_epoch, "renorm_flops_per_epoch": ns_flops_epoch},
         "spmm": spmm,
     }
     if not args.no_cpu_baseline:

@@ -111,6 +111,8 @@ SIGNATURES: dict = {
     "mmsb_set_phi": (ctypes.c_int, [_vp, _vp]),
     "mmsb_set_yvectors": (ctypes.c_int, [_vp, ctypes.c_int64, ctypes.c_int32, _vp, _vp, _vp, _vp, _vp, _vp]),
     "mmsb_stgd": (ctypes.c_int, [_vp, ctypes.c_int64, ctypes.c_int32]),
+    "mmsb_stgd_timed": (ctypes.c_int, [_vp, ctypes.c_int64, ctypes.c_int32, _vp]),
+    "mmsb_set_learn_rate": (ctypes.c_int, [_vp, ctypes.c_double]),
     "mmsb_eigen": (ctypes.c_int, [_vp]),
     "mmsb_post": (ctypes.c_int, [_vp]),
     "mmsb_write_outputs": (ctypes.c_int, [_vp]),
@@ -303,6 +305,18 @@ class Pipeline:
 
     def stgd(self, steps=-1, precision="fp32"):
         check(lib.mmsb_stgd(self.ctx, int(steps), PREC[precision] if isinstance(precision, str) else precision))
+
+    def set_learn_rate(self, eta):
+        check(lib.mmsb_set_learn_rate(self.ctx, float(eta)))
+
+    def stgd_timed(self, steps, precision="fp32"):
+        """Runs `steps` direct STGD steps with HIP events around each phase; returns the summed
+        device milliseconds of (P1, P2, renormalisation)."""
+        import numpy as np
+        out = np.zeros(3, np.float64)
+        check(lib.mmsb_stgd_timed(self.ctx, int(steps), PREC[precision] if isinstance(precision, str) else precision,
+                                  out.ctypes.data))
+        return tuple(float(v) for v in out)
 
     def eigen(self):
         check(lib.mmsb_eigen(self.ctx))

@@ -235,11 +235,21 @@ template int gemm_nn<double, float>(hipStream_t, int64_t, int32_t, int32_t, doub
 // One workgroup of 1024 threads; G is r x r row-major FP64 in global memory (L2-resident).
 // Right-looking Crout on the lower triangle, then R = L^T and R^{-1} by column-parallel back
 // substitution. status[0] = 0 on success, j+1 if the pivot of column j is not positive.
-__global__ __launch_bounds__(1024) void chol_inv_kernel(int32_t r, double* __restrict__ G, double shift,
-                                                        double* __restrict__ R, double* __restrict__ Rinv,
-                                                        int32_t* __restrict__ status) {
-  __shared__ double red[1024];
-  __shared__ double piv;
+// ---- Cholesky factor and its inverse (CholeskyQR of the tall whitening sketches) ----------------
+// One workgroup per factorisation; the r x r FP64 matrices stay in L2.
+// chol_kernel: right-looking blocked Cholesky of the lower triangle of G + shift trace(G) I in
+// 16-column panels: the diagonal block is factored in LDS, the rows below it solved one row per
+// thread, and the trailing lower triangle updated once per panel in 4 x 4 register tiles.
+// trinv_kernel: X = L^{-1} by forward substitution, PI rows at a time; each thread owns one column
+// c of X (a column of L^{-1} depends only on itself), so the only barriers are between row panels,
+// which make the panel's rows of L visible in LDS. R = L^T and R^{-1} = X^T are written out.
+constexpr int kCholP = 16;
+constexpr int kCholThreads = 512;
+
+__global__ __launch_bounds__(kCholThreads) void chol_kernel(int32_t r, double* __restrict__ G, double shift,
+                                                            int32_t* __restrict__ status) {
+  __shared__ double red[kCholThreads];
+  __shared__ double D[kCholP][kCholP + 1];
   __shared__ int bad;
   const int t = threadIdx.x;
   double tr = 0.0;
@@ -251,63 +261,161 @@ __global__ __launch_bounds__(1024) void chol_inv_kernel(int32_t r, double* __res
     if (t < off) red[t] += red[t + off];
     __syncthreads();
   }
-  double s = shift * red[0];
-  if (!(s > 0.0)) s = 1e-300;
-  for (int i = t; i < r; i += blockDim.x) G[(int64_t)i * r + i] += s;
+  double sh = shift * red[0];
+  if (!(sh > 0.0)) sh = 1e-300;
+  for (int i = t; i < r; i += blockDim.x) G[(int64_t)i * r + i] += sh;
   __syncthreads();
-  for (int j = 0; j < r; ++j) {
-    if (t == 0) {
-      const double d = G[(int64_t)j * r + j];
-      if (!(d > 0.0)) {
-        bad = j + 1;
-        piv = 1.0;
-      } else {
-        piv = sqrt(d);
+  for (int j0 = 0; j0 < r; j0 += kCholP) {
+    const int pb = min(kCholP, r - j0);
+    // 1. diagonal block
+    if (t < kCholP * kCholP) {
+      const int a = t / kCholP, b = t % kCholP;
+      if (a < pb && b <= a) D[a][b] = G[(int64_t)(j0 + a) * r + j0 + b];
+    }
+    __syncthreads();
+    for (int c = 0; c < pb; ++c) {
+      if (t == 0) {
+        const double d = D[c][c];
+        if (!(d > 0.0)) {
+          if (!bad) bad = j0 + c + 1;
+          D[c][c] = 1.0;
+        } else {
+          D[c][c] = sqrt(d);
+        }
       }
-      G[(int64_t)j * r + j] = piv;
+      __syncthreads();
+      if (t > c && t < pb) D[t][c] /= D[c][c];
+      __syncthreads();
+      if (t < kCholP * kCholP) {
+        const int a = t / kCholP, b = t % kCholP;
+        if (a < pb && b > c && b <= a) D[a][b] -= D[a][c] * D[b][c];
+      }
+      __syncthreads();
     }
-    __syncthreads();
-    const double pj = piv;
-    for (int i = j + 1 + t; i < r; i += blockDim.x) G[(int64_t)i * r + j] /= pj;  // column j of L
-    __syncthreads();
-    // trailing update of the lower triangle: G[i][l] -= L[i][j] L[l][j], j < l <= i
-    const int m = r - j - 1;
-    const int64_t cnt = (int64_t)m * (m + 1) / 2;
-    for (int64_t e = t; e < cnt; e += blockDim.x) {
-      // e -> (ii, ll) with 0 <= ll <= ii < m, row-wise enumeration of the lower triangle
-      int ii = (int)((sqrt(8.0 * (double)e + 1.0) - 1.0) * 0.5);
-      while ((int64_t)ii * (ii + 1) / 2 > e) --ii;
-      while ((int64_t)(ii + 1) * (ii + 2) / 2 <= e) ++ii;
-      const int ll = (int)(e - (int64_t)ii * (ii + 1) / 2);
-      const int i = j + 1 + ii, l = j + 1 + ll;
-      G[(int64_t)i * r + l] -= G[(int64_t)i * r + j] * G[(int64_t)l * r + j];
+    if (t < kCholP * kCholP) {
+      const int a = t / kCholP, b = t % kCholP;
+      if (a < pb && b <= a) G[(int64_t)(j0 + a) * r + j0 + b] = D[a][b];
     }
-    __syncthreads();
-  }
-  // R = L^T (upper), zero below the diagonal
-  for (int64_t e = t; e < (int64_t)r * r; e += blockDim.x) {
-    const int i = (int)(e / r), l = (int)(e % r);
-    R[e] = (l >= i) ? G[(int64_t)l * r + i] : 0.0;
-  }
-  __syncthreads();
-  // Rinv: solve R X = I column by column (one thread per column), upper triangular
-  for (int c = t; c < r; c += blockDim.x) {
-    for (int i = r - 1; i >= 0; --i) {
-      double acc = (i == c) ? 1.0 : 0.0;
-      if (i <= c) {
-        for (int l = i + 1; l <= c; ++l) acc -= R[(int64_t)i * r + l] * Rinv[(int64_t)l * r + c];
-        Rinv[(int64_t)i * r + c] = acc / R[(int64_t)i * r + i];
-      } else {
-        Rinv[(int64_t)i * r + c] = 0.0;
+    // 2. rows below the diagonal block: L21 = A21 L11^{-T}
+    for (int i = j0 + pb + t; i < r; i += blockDim.x) {
+      __asm__ volatile("" ::: "memory");  // re-read D per row instead of holding 136 values in registers
+      double* row = G + (int64_t)i * r + j0;
+      double x[kCholP];
+#pragma unroll
+      for (int c = 0; c < kCholP; ++c) x[c] = c < pb ? row[c] : 0.0;
+#pragma unroll
+      for (int c = 0; c < kCholP; ++c) {
+        if (c < pb) {
+          double v = x[c];
+#pragma unroll
+          for (int cc = 0; cc < c; ++cc) v -= x[cc] * D[c][cc];
+          x[c] = v / D[c][c];
+          row[c] = x[c];
+        }
       }
     }
+    __syncthreads();
+    // 3. trailing lower triangle: G[i][l] -= sum_c L[i][c] L[l][c], j0 + pb <= l <= i < r
+    const int base = j0 + pb;
+    const int m = r - base;
+    const int nb4 = (m + 3) / 4;
+    const int ty = t >> 5, tx = t & 31;
+    for (int bi = ty; bi < nb4; bi += kCholThreads / 32) {
+      for (int bl = tx; bl <= bi; bl += 32) {
+        const double* pi[4];
+        const double* pl[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {  // rows past r read row r - 1 and are not stored
+          pi[u] = G + (int64_t)min(base + 4 * bi + u, r - 1) * r + j0;
+          pl[u] = G + (int64_t)min(base + 4 * bl + u, r - 1) * r + j0;
+        }
+        double acc[4][4] = {};
+#pragma unroll 4
+        for (int c = 0; c < pb; ++c) {
+          double a[4], b[4];
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            a[u] = pi[u][c];
+            b[u] = pl[u][c];
+          }
+#pragma unroll
+          for (int u = 0; u < 4; ++u)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) acc[u][v] = fma(a[u], b[v], acc[u][v]);
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            const int i = base + 4 * bi + u, l = base + 4 * bl + v;
+            if (i < r && l <= i) G[(int64_t)i * r + l] -= acc[u][v];
+          }
+      }
+    }
+    __syncthreads();
   }
   if (t == 0) status[0] = bad;
 }
 
+template <int PI>
+__global__ __launch_bounds__(1024) void trinv_kernel(int32_t r, const double* __restrict__ L, double* __restrict__ R,
+                                                     double* __restrict__ Rinv) {
+  __shared__ double Lr[16384];  // PI rows of L, r entries each
+  const int t = threadIdx.x;
+  const int64_t nel = (int64_t)r * r;
+  for (int64_t e = t; e < nel; e += blockDim.x) {
+    const int i = (int)(e / r), l = (int)(e % r);
+    R[e] = (l >= i) ? L[(int64_t)l * r + i] : 0.0;  // R = L^T
+    Rinv[e] = 0.0;
+  }
+  for (int i0 = 0; i0 < r; i0 += PI) {
+    const int pb = min(PI, r - i0);
+    __syncthreads();  // previous panel's Rinv entries and Lr reads are complete
+    for (int e = t; e < pb * r; e += blockDim.x) {
+      const int ii = e / r, l = e % r;
+      Lr[ii * r + l] = (l <= i0 + ii) ? L[(int64_t)(i0 + ii) * r + l] : 0.0;
+    }
+    __syncthreads();
+    for (int c = t; c < i0 + pb; c += blockDim.x) {
+      double acc[PI];
+#pragma unroll
+      for (int ii = 0; ii < PI; ++ii) acc[ii] = 0.0;
+      const double* xc = Rinv + (int64_t)c * r;  // row c of R^{-1} = column c of X
+      for (int l = c; l < i0; ++l) {
+        const double x = xc[l];
+#pragma unroll
+        for (int ii = 0; ii < PI; ++ii) acc[ii] = fma(Lr[ii * r + l], x, acc[ii]);
+      }
+      double xs[PI];
+#pragma unroll
+      for (int ii = 0; ii < PI; ++ii) {
+        const int i = i0 + ii;
+        double v = 0.0;
+        if (ii < pb && i >= c) {
+          v = (i == c ? 1.0 : 0.0) - acc[ii];
+#pragma unroll
+          for (int jj = 0; jj < ii; ++jj) v = fma(-Lr[ii * r + i0 + jj], xs[jj], v);
+          v /= Lr[ii * r + i];
+        }
+        xs[ii] = v;
+      }
+#pragma unroll
+      for (int ii = 0; ii < PI; ++ii)
+        if (ii < pb) Rinv[(int64_t)c * r + i0 + ii] = xs[ii];
+    }
+  }
+}
+
 int chol_inv(hipStream_t s, int32_t r, double* G, double shift, double* R, double* Rinv, int32_t* status) {
   MMSB_CHECK_ARG(r >= 1 && r <= 4096, "chol_inv: r=%d out of range", r);
-  hipLaunchKernelGGL(chol_inv_kernel, dim3(1), dim3(1024), 0, s, r, G, shift, R, Rinv, status);
+  hipLaunchKernelGGL(chol_kernel, dim3(1), dim3(kCholThreads), 0, s, r, G, shift, status);
+  MMSB_LAUNCH_CHECK();
+  if (r <= 1024)
+    hipLaunchKernelGGL(trinv_kernel<16>, dim3(1), dim3(1024), 0, s, r, G, R, Rinv);
+  else if (r <= 2048)
+    hipLaunchKernelGGL(trinv_kernel<8>, dim3(1), dim3(1024), 0, s, r, G, R, Rinv);
+  else
+    hipLaunchKernelGGL(trinv_kernel<4>, dim3(1), dim3(1024), 0, s, r, G, R, Rinv);
   MMSB_LAUNCH_CHECK();
   return MMSB_OK;
 }

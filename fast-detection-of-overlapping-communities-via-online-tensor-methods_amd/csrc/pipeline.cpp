@@ -98,9 +98,12 @@ struct mmsb_ctx {
   float* phi[2] = {nullptr, nullptr};
   int cur = 0;
   float *phit = nullptr, *gt = nullptr, *phia = nullptr, *h = nullptr, *cgt = nullptr, *cs_part = nullptr, *mab = nullptr;
+  float *ns_rowsum = nullptr, *mpart = nullptr;
+  bf16_bits *phi16[2] = {nullptr, nullptr}, *phit16 = nullptr, *phia16 = nullptr, *phiT16 = nullptr, *phitT16 = nullptr,
+            *phiaT16 = nullptr, *gt16 = nullptr, *h16 = nullptr;
   double* frob = nullptr;
   int64_t frob_cap = 0, steps_done = 0;
-  bf16_bits *YA16 = nullptr, *YB16 = nullptr, *YCt16 = nullptr, *cgt16 = nullptr;
+  bf16_bits *YA16 = nullptr, *YB16 = nullptr, *YCt16 = nullptr;
   double* frob_epoch = nullptr;            // per-epoch log written by captured epochs
   hipGraphExec_t epoch_graph[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};  // [precision][cur]
   float *phiT = nullptr, *phitT = nullptr, *phiaT = nullptr;
@@ -389,8 +392,13 @@ static int alloc_stgd(mmsb_ctx* c) {
   TRY(dalloc(c, c->phia, kk));
   TRY(dalloc(c, c->h, kk));
   TRY(dalloc(c, c->cgt, (size_t)c->kp * c->b));
-  TRY(dalloc(c, c->cs_part, (size_t)(c->b / 64) * c->kp));
+  TRY(dalloc(c, c->cs_part, (size_t)(c->b / 32) * c->kp));
   TRY(dalloc(c, c->mab, (size_t)2 * c->kp));
+  TRY(dalloc(c, c->ns_rowsum, (size_t)2 * c->kp));
+  TRY(dalloc(c, c->mpart, (size_t)(c->kp / 32) * 2 * c->kp));
+  TRY(dalloc(c, c->phiT, kk));
+  TRY(dalloc(c, c->phitT, kk));
+  TRY(dalloc(c, c->phiaT, kk));
   return MMSB_OK;
 }
 
@@ -406,15 +414,14 @@ static int init_phi(mmsb_ctx* c) {
 }
 
 static int prepare_bf16(mmsb_ctx* c) {
-  const size_t kk = (size_t)c->kp * c->kp;
   const size_t ny = (size_t)c->n_pad * c->kp;
   TRY(dalloc(c, c->YA16, ny));
   TRY(dalloc(c, c->YB16, ny));
   TRY(dalloc(c, c->YCt16, ny));
-  TRY(dalloc(c, c->cgt16, (size_t)c->kp * c->b));
-  TRY(dalloc(c, c->phiT, kk));
-  TRY(dalloc(c, c->phitT, kk));
-  TRY(dalloc(c, c->phiaT, kk));
+  const size_t kk2 = (size_t)2 * c->kp * c->kp;
+  for (bf16_bits** p : {&c->phi16[0], &c->phi16[1], &c->phit16, &c->phia16, &c->phiT16, &c->phitT16, &c->phiaT16, &c->gt16,
+                        &c->h16})
+    TRY(dalloc(c, *p, kk2));
   if (!c->y16_valid) {
     TRY(f32_to_bf16(c->s, (int64_t)ny, c->YA, c->YA16));
     TRY(f32_to_bf16(c->s, (int64_t)ny, c->YB, c->YB16));
@@ -424,7 +431,7 @@ static int prepare_bf16(mmsb_ctx* c) {
   return MMSB_OK;
 }
 
-static int run_stgd_steps(mmsb_ctx* c, int64_t nsteps, int32_t precision) {
+static int run_stgd_steps(mmsb_ctx* c, int64_t nsteps, int32_t precision, StgdTiming* timing = nullptr) {
   if (nsteps <= 0) return MMSB_OK;
   if (precision != MMSB_PREC_FP32 && precision != MMSB_PREC_BF16)
     return set_error(MMSB_ERR_ARG, "stgd: unknown precision %d", precision);
@@ -461,17 +468,37 @@ static int run_stgd_steps(mmsb_ctx* c, int64_t nsteps, int32_t precision) {
   a.mu_a = c->muw[0];
   a.mu_b = c->muw[1];
   a.mu_c = c->muw[2];
-  a.h = c->h;
-  a.cgt = c->cgt;
   a.cs_part = c->cs_part;
   a.mab = c->mab;
   const int64_t spe = (c->n[0] + c->bl - 1) / c->bl;
-  auto launch = [&](int64_t step0, int64_t ns, double* frob_out, int& cur) -> int {
-    if (precision == MMSB_PREC_BF16)
-      return stgd_run_bf16(c->s, a, c->YA16, c->YB16, c->YCt16, step0, ns, c->cfg.renorm_every, c->cfg.ns_iters, c->phi, cur,
-                           c->phit, c->gt, c->phia, c->phiT, c->phitT, c->phiaT, c->cgt16, frob_out);
-    return stgd_run_f32(c->s, a, c->YA, c->YB, c->YCt, step0, ns, c->cfg.renorm_every, c->cfg.ns_iters, c->phi, cur, c->phit,
-                        c->gt, c->phia, frob_out);
+  StgdBuffers bufs{};
+  bufs.phi[0] = c->phi[0];
+  bufs.phi[1] = c->phi[1];
+  bufs.phit = c->phit;
+  bufs.phia = c->phia;
+  bufs.gt = c->gt;
+  bufs.phiT = c->phiT;
+  bufs.phitT = c->phitT;
+  bufs.phiaT = c->phiaT;
+  bufs.h = c->h;
+  bufs.phi16[0] = c->phi16[0];
+  bufs.phi16[1] = c->phi16[1];
+  bufs.phit16 = c->phit16;
+  bufs.phia16 = c->phia16;
+  bufs.phiT16 = c->phiT16;
+  bufs.phitT16 = c->phitT16;
+  bufs.phiaT16 = c->phiaT16;
+  bufs.gt16 = c->gt16;
+  bufs.h16 = c->h16;
+  bufs.cgt = c->cgt;
+  bufs.rowsum = c->ns_rowsum;
+  bufs.mpart = c->mpart;
+  const bool bf = precision == MMSB_PREC_BF16;
+  const void* ya = bf ? (const void*)c->YA16 : (const void*)c->YA;
+  const void* yb = bf ? (const void*)c->YB16 : (const void*)c->YB;
+  const void* yc = bf ? (const void*)c->YCt16 : (const void*)c->YCt;
+  auto launch = [&](int64_t step0, int64_t ns, double* frob_out, int& cur, StgdTiming* tm) -> int {
+    return stgd_run(c->s, precision, a, ya, yb, yc, step0, ns, c->cfg.renorm_every, c->cfg.ns_iters, bufs, cur, frob_out, tm);
   };
   // Whole epochs that start at an epoch boundary replay a captured HIP graph of the epoch's
   // launches (one graph per precision and Phi ping-pong parity); anything else launches directly.
@@ -479,7 +506,8 @@ static int run_stgd_steps(mmsb_ctx* c, int64_t nsteps, int32_t precision) {
   int64_t done = 0;
   while (done < nsteps) {
     const int64_t step0 = c->steps_done + done;
-    const bool epoch_aligned = (step0 % spe == 0) && (nsteps - done >= spe) && renorm_aligned && c->cfg.use_graphs;
+    const bool epoch_aligned =
+        (step0 % spe == 0) && (nsteps - done >= spe) && renorm_aligned && c->cfg.use_graphs && !timing;
     if (epoch_aligned) {
       TRY(dalloc(c, c->frob_epoch, (size_t)spe));
       hipGraphExec_t& ge = c->epoch_graph[precision][c->cur];
@@ -487,7 +515,7 @@ static int run_stgd_steps(mmsb_ctx* c, int64_t nsteps, int32_t precision) {
         hipGraph_t g = nullptr;
         int cur = c->cur;
         HCHK(hipStreamBeginCapture(c->s, hipStreamCaptureModeThreadLocal));
-        const int st = launch(0, spe, c->frob_epoch, cur);
+        const int st = launch(0, spe, c->frob_epoch, cur, nullptr);
         hipError_t e = hipStreamEndCapture(c->s, &g);
         if (st != MMSB_OK) {
           if (g) (void)hipGraphDestroy(g);
@@ -503,7 +531,7 @@ static int run_stgd_steps(mmsb_ctx* c, int64_t nsteps, int32_t precision) {
       done += spe;
     } else {
       const int64_t ns = std::min<int64_t>(nsteps - done, spe - step0 % spe);
-      TRY(launch(step0, ns, c->frob + step0, c->cur));
+      TRY(launch(step0, ns, c->frob + step0, c->cur, timing));
       done += ns;
     }
   }
@@ -755,6 +783,28 @@ extern "C" int mmsb_create(const stpm_config* cfg, mmsb_ctx** out) {
   return MMSB_OK;
 }
 
+// Captured epoch graphs hold the step parameters in their kernel arguments.
+static int drop_epoch_graphs(mmsb_ctx* c) {
+  HCHK(hipStreamSynchronize(c->s));
+  for (auto& row : c->epoch_graph)
+    for (hipGraphExec_t& g : row)
+      if (g) {
+        HCHK(hipGraphExecDestroy(g));
+        g = nullptr;
+      }
+  return MMSB_OK;
+}
+
+extern "C" int mmsb_set_learn_rate(mmsb_ctx* c, double eta) {
+  clear_error();
+  MMSB_CHECK_ARG(c, "set_learn_rate: null context");
+  MMSB_CHECK_ARG(std::isfinite(eta) && eta > 0.0, "set_learn_rate: learning rate %g must be positive", eta);
+  HCHK(hipSetDevice(c->cfg.device));
+  TRY(drop_epoch_graphs(c));
+  c->cfg.learn_rate = eta;
+  return MMSB_OK;
+}
+
 extern "C" void mmsb_destroy(mmsb_ctx* c) {
   if (!c) return;
   (void)hipSetDevice(c->cfg.device);
@@ -866,6 +916,27 @@ extern "C" int mmsb_set_yvectors(mmsb_ctx* c, int64_t n_x, int32_t k, const doub
   HCHK(hipStreamSynchronize(c->s));
   c->have_y = true;
   c->y16_valid = false;
+  return MMSB_OK;
+}
+
+extern "C" int mmsb_stgd_timed(mmsb_ctx* c, int64_t steps, int32_t precision, double* phase_ms) {
+  clear_error();
+  MMSB_CHECK_ARG(c && phase_ms, "stgd_timed: null argument");
+  MMSB_CHECK_ARG(steps >= 0, "stgd_timed: negative step count");
+  if (!c->have_y) return set_error(MMSB_ERR_STATE, "stgd_timed: neighbour vectors not computed (call mmsb_whiten)");
+  HCHK(hipSetDevice(c->cfg.device));
+  TRY(alloc_stgd(c));
+  if (!c->have_phi) TRY(init_phi(c));
+  c->eig_done = c->post_done = false;
+  StgdTiming tm{};
+  int st = MMSB_OK;
+  for (int i = 0; i < 4 && st == MMSB_OK; ++i)
+    if (hipEventCreate(&tm.ev[i]) != hipSuccess) st = set_error(MMSB_ERR_HIP, "stgd_timed: hipEventCreate failed");
+  if (st == MMSB_OK) st = run_stgd_steps(c, steps, precision, &tm);
+  for (int i = 0; i < 4; ++i)
+    if (tm.ev[i]) (void)hipEventDestroy(tm.ev[i]);
+  TRY(st);
+  for (int i = 0; i < 3; ++i) phase_ms[i] = tm.ms[i];
   return MMSB_OK;
 }
 

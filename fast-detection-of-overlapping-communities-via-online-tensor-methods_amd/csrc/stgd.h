@@ -1,4 +1,4 @@
-// Internal declarations of the STGD kernels (stgd.hip, stgd_bf16.hip).
+// Internal declarations of the STGD kernels (stgd.hip).
 #pragma once
 
 #include <hip/hip_runtime_api.h>
@@ -18,29 +18,58 @@ struct StgdArgs {
   const double* mu_a;   // [kp] whitened means
   const double* mu_b;
   const double* mu_c;
-  const float* phi;     // [kp x kp] current Phi (row-major, column i = eigenvector i)
-  float* phit;          // [kp x kp] Phi~ after the gradient step
-  float* h;             // [kp x kp] (Phi^T Phi) o (Phi^T Phi)
-  float* cgt;           // [kp x b] C_g transposed
-  float* cs_part;       // [b/64 x kp] per-row-tile column sums of P_A o P_B
+  const float* phi;     // [kp x kp] current Phi (row-major, column i = eigenvector i); set per step
+  float* phit;          // [kp x kp] Phi~ after the gradient step; set per step
+  float* cs_part;       // [b/32 x kp] per-row-tile column sums of P_A o P_B
   float* mab;           // [2 x kp] m_A, m_B
 };
 
-// Runs nsteps minibatch steps starting at global step step0 (batch = step mod steps-per-epoch).
-// phi_buf[cur] holds Phi on entry; on exit phi_buf[cur] holds the result. frob_log[st] receives
-// ||Phi_{t+1} - Phi_t||_F^2 of the st-th step of this call. Every launch is capturable in a HIP graph.
-int stgd_run_f32(hipStream_t s, const StgdArgs& args, const float* YA, const float* YB, const float* YCt, int64_t step0,
-                 int64_t nsteps, int32_t renorm_every, int32_t ns_iters, float* phi_buf[2], int& cur, float* phit, float* gt,
-                 float* phia, double* frob_log);
+// Device buffers of a run (all [kp x kp] FP32 row-major unless noted). The *T buffers hold the
+// transposes of their partners so that every MFMA operand is read along contiguous rows. The *16
+// buffers ([2 x kp x kp] 16-bit words: the BF16 hi block, then the lo block) hold the split-BF16
+// pairs of their partners; BF16 mode only (may be null in FP32 mode).
+struct StgdBuffers {
+  float* phi[2];        // Phi ping-pong; phi[cur] is the current iterate
+  float* phit;          // Phi~ (gradient step) and Newton-Schulz scratch
+  float* phia;          // Newton-Schulz scratch
+  float* gt;            // X^T X of the current Newton-Schulz iterate
+  float* phiT;          // (phi[cur])^T
+  float* phitT;         // phit^T
+  float* phiaT;         // phia^T
+  float* h;             // (Phi^T Phi) o (Phi^T Phi)
+  uint16_t* phi16[2];
+  uint16_t* phit16;
+  uint16_t* phia16;
+  uint16_t* phiT16;
+  uint16_t* phitT16;
+  uint16_t* phiaT16;
+  uint16_t* gt16;
+  uint16_t* h16;
+  void* cgt;            // [kp x b] C_g^T: FP32, or BF16 in BF16 mode
+  float* rowsum;        // [2 x kp] row sums of |X^T X| (Newton-Schulz scale guard)
+  float* mpart;         // [kp/32 x 2 x kp] per-row-block partial sums of mu_A^T Phi, mu_B^T Phi
+};
 
-// BF16 variant (stgd_bf16.hip): Y in BF16; phiT/phitT/phiaT are [kp x kp] FP32 transposes kept beside
-// phi_buf/phit/phia; cgt16 is C_g^T in BF16. Requires bl % 8 == 0 (16-byte aligned Y_C^T slices).
+// Optional per-phase timing of direct (uncaptured) steps: ev[0..3] are caller-created events
+// recorded before the P1 kernel, before P2, before the renormalisation and after it; ms[ph] accumulates
+// the elapsed time of phase ph (P1, P2, renormalisation) over the steps of the call. The call then
+// waits for each step to finish before launching the next.
+struct StgdTiming {
+  hipEvent_t ev[4];
+  float ms[3];
+};
+
+// Runs nsteps minibatch steps starting at global step step0 (batch = step mod steps-per-epoch).
+// bufs.phi[cur] holds Phi on entry; cur is flipped once per step and bufs.phi[cur] holds the
+// result on exit. frob_log[st] receives ||Phi_{t+1} - Phi_t||_F^2 of the st-th step of this call.
+// precision is MMSB_PREC_FP32 (Y as float) or MMSB_PREC_BF16 (Y as BF16). Without timing every
+// launch is capturable in a HIP graph (no memset nodes, no host synchronisation).
+int stgd_run(hipStream_t s, int32_t precision, const StgdArgs& args, const void* YA, const void* YB, const void* YCt,
+             int64_t step0, int64_t nsteps, int32_t renorm_every, int32_t ns_iters, const StgdBuffers& bufs, int& cur,
+             double* frob_log, StgdTiming* timing = nullptr);
+
 // BF16 buffers cross the host boundary as opaque 16-bit words.
 typedef uint16_t bf16_bits;
-int stgd_run_bf16(hipStream_t s, const StgdArgs& args, const bf16_bits* YA, const bf16_bits* YB, const bf16_bits* YCt,
-                  int64_t step0, int64_t nsteps, int32_t renorm_every, int32_t ns_iters, float* phi_buf[2], int& cur,
-                  float* phit, float* gt, float* phia, float* phiT, float* phitT, float* phiaT, bf16_bits* cgt16,
-                  double* frob_log);
 int f32_to_bf16(hipStream_t s, int64_t n, const float* in, bf16_bits* out);
 
 }  // namespace mmsb

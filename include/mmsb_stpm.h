@@ -249,6 +249,14 @@ int mmsb_set_yvectors(mmsb_ctx* ctx, int64_t n_x, int32_t k, const double* ya, c
  * runs the configured epochs (BF16 epochs followed by polish_epochs in FP32 when precision is
  * BF16) with the THRESH / SGD_ITER_MIN / SGD_ITER_MAX stopping rules checked once per epoch. */
 int mmsb_stgd(mmsb_ctx* ctx, int64_t steps, int32_t precision);
+/* Changes LEARN_RATE (stpm.h:79, a compile-time constant in the reference) for the following
+ * mmsb_stgd calls of this context, e.g. a different rate for polishing epochs. */
+int mmsb_set_learn_rate(mmsb_ctx* ctx, double learn_rate);
+/* Measurement entry (no counterpart in the reference): `steps` STGD steps as mmsb_stgd, launched
+ * directly (never from a captured graph) with HIP events on the context stream around each phase;
+ * phase_ms[0..2] receive the summed device time of the P1 kernel (Y Phi contractions and
+ * (Phi^T Phi)^2), the P2 kernel (gradient step) and the renormalisation kernels. */
+int mmsb_stgd_timed(mmsb_ctx* ctx, int64_t steps, int32_t precision, double* phase_ms);
 /* Loss and adjoint (mmsb_stgd_grad_f32) over X rows [row0, row0 + nrows) for a host Phi [k x k].
  * recompute != 0 rebuilds those whitened rows from CSR (y = G_X* rows times W, Z_B^T W, Z_C^T W)
  * instead of reading the stored neighbour vectors -- the memory-saving form for large n k. */

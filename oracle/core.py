@@ -261,12 +261,19 @@ def t_hat_c(phi, YA, YB, YC, muA, muB, muC, coeffs):
 
 
 def polar_renorm(phi, ns_iters=2):
-    """Orthonormal renormalisation by Newton-Schulz iterations Phi <- Phi (3I - Phi^T Phi)/2,
-    which converge quadratically to the polar factor Phi (Phi^T Phi)^-1/2. The reference
-    re-orthonormalises every step with a QR factorisation instead (stpm.c:774-786)."""
+    """Orthonormal renormalisation by Newton-Schulz iterations X <- c X (3I - c^2 X^T X)/2, which
+    converge quadratically to the polar factor X (X^T X)^-1/2. The scale c = min(1, sqrt(2/||G||_inf))
+    (G = X^T X, infinity norm = max row sum of |G|, an upper bound of sigma_max(X)^2) keeps every
+    singular value of cX below sqrt(2) < sqrt(3), inside the region where the iteration converges;
+    the polar factor is scale-invariant, so c changes nothing when it is 1 and only rescues
+    iterates that would otherwise diverge. The reference re-orthonormalises every step with a QR
+    factorisation instead (stpm.c:774-786)."""
     I = np.eye(phi.shape[1])
     for _ in range(ns_iters):
-        phi = phi @ (1.5 * I - 0.5 * (phi.T @ phi))
+        G = phi.T @ phi
+        ginf = np.abs(G).sum(1).max()
+        c = np.sqrt(2.0 / ginf) if ginf > 2.0 else 1.0
+        phi = c * phi @ (1.5 * I - 0.5 * c * c * G)
     return phi
 
 

@@ -356,3 +356,26 @@ def test_norm_prob_rows_closed_forms():
     assert np.allclose(core.norm_prob_rows(P), want)
     rp, ids = core.memberships(core.norm_prob_rows(P))
     assert rp.tolist() == [0, 2, 4, 4, 5] and ids.tolist() == [0, 1, 0, 1, 0]
+
+
+def test_polar_renorm_converges_to_polar_factor():
+    """Newton-Schulz with the scale guard reaches U V^T of the SVD (the polar factor): from a
+    near-orthonormal start (guard inactive, cubic formula unchanged) and from an iterate whose
+    largest singular value is far beyond sqrt(3), where the unguarded iteration diverges."""
+    rng = np.random.default_rng(5)
+    k = 12
+    Q, _ = np.linalg.qr(rng.standard_normal((k, k)))
+    near = Q + 1e-3 * rng.standard_normal((k, k))
+    far = Q @ np.diag(np.geomspace(0.3, 80.0, k)) @ np.linalg.qr(rng.standard_normal((k, k)))[0]
+    for X, iters in ((near, 4), (far, 40)):
+        U, _, Vt = np.linalg.svd(X)
+        got = core.polar_renorm(X, iters)
+        assert np.abs(got - U @ Vt).max() < 1e-10
+    I = np.eye(k)
+    one = near @ (1.5 * I - 0.5 * (near.T @ near))            # plain cubic step: guard inactive
+    np.testing.assert_allclose(core.polar_renorm(near, 1), one, rtol=0, atol=1e-15)
+    with np.errstate(over="ignore", invalid="ignore"):
+        x = far.copy()
+        for _ in range(40):
+            x = x @ (1.5 * I - 0.5 * x.T @ x)
+    assert not np.isfinite(x).all() or np.abs(x).max() > 1e6    # the unguarded iteration blows up

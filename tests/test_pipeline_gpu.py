@@ -221,6 +221,8 @@ def test_epoch_graph_replays_match_eager(mmsb, prec):
         spe = n // b
         for epoch in range(4):
             for p in ps:
+                if epoch == 2:
+                    p.set_learn_rate(1e-3)      # drops the captured graphs; later epochs recapture
                 p.stgd(spe, prec)
                 p.eigen()                       # eager work between replays
             a, e = ps[0].get("phi"), ps[1].get("phi")
@@ -262,3 +264,27 @@ def test_power_method_recovers_odeco(mmsb):
     p.power_method(30)
     assert np.allclose(np.sort(np.abs(V.T @ p.get("phi")).max(0)), 1.0, atol=1e-5)
     p.close()
+
+
+@pytest.mark.parametrize("prec", ["fp32", "bf16"])
+def test_timed_steps_match_untimed(mmsb, prec):
+    """mmsb_stgd_timed runs the same steps as mmsb_stgd (bit-identical Phi) and reports positive
+    per-phase device times."""
+    n, k, b = 900, 24, 64
+    rng = np.random.default_rng(8)
+    ys = [rng.standard_normal((n, k)) for _ in range(3)]
+    mus = [0.2 * rng.standard_normal(k) for _ in range(3)]
+    phi0 = core.init_phi(4, k)
+    ps = []
+    for _ in range(2):
+        cfg = mmsb.config(nx=n, na=n, nb=n, nc=n, ncom=k, oversample=0, minibatch=b, learn_rate=2e-3, use_graphs=0)
+        p = mmsb.Pipeline(cfg)
+        p.set_yvectors(*ys, *mus)
+        p.set_phi(phi0)
+        ps.append(p)
+    ps[0].stgd(20, prec)
+    ms = ps[1].stgd_timed(20, prec)
+    assert all(t > 0 for t in ms), ms
+    assert np.array_equal(ps[0].get("phi"), ps[1].get("phi"))
+    for p in ps:
+        p.close()
