@@ -113,6 +113,10 @@ SIGNATURES: dict = {
     "mmsb_stgd": (ctypes.c_int, [_vp, ctypes.c_int64, ctypes.c_int32]),
     "mmsb_stgd_timed": (ctypes.c_int, [_vp, ctypes.c_int64, ctypes.c_int32, _vp]),
     "mmsb_set_learn_rate": (ctypes.c_int, [_vp, ctypes.c_double]),
+    "mmsb_batch_create": (ctypes.c_int, [_vp, ctypes.c_int32, ctypes.c_int32, _vp]),
+    "mmsb_batch_run": (ctypes.c_int, [_vp, ctypes.c_int32]),
+    "mmsb_batch_autotune": (ctypes.c_int, [_vp, _vp, _vp]),
+    "mmsb_batch_destroy": (None, [_vp]),
     "mmsb_eigen": (ctypes.c_int, [_vp]),
     "mmsb_post": (ctypes.c_int, [_vp]),
     "mmsb_write_outputs": (ctypes.c_int, [_vp]),
@@ -379,6 +383,37 @@ def tensor_coeffs(alpha0: float, scaling: str = "gpu"):
     out = np.zeros(3)
     check(lib.mmsb_tensor_coeffs(float(alpha0), M2_SCALING[scaling], out.ctypes.data))
     return out
+
+
+class Batch:
+    """Independent pipelines advanced together, one persistent launch per epoch (mmsb_batch_*)."""
+
+    def __init__(self, pipelines, precision="fp32"):
+        self.members = list(pipelines)
+        arr = (ctypes.c_void_p * len(self.members))(*[p.ctx for p in self.members])
+        out = ctypes.c_void_p()
+        check(lib.mmsb_batch_create(arr, len(self.members), PREC[precision] if isinstance(precision, str) else precision,
+                                    ctypes.byref(out)))
+        self.h = out
+
+    def run(self, epochs=1):
+        check(lib.mmsb_batch_run(self.h, int(epochs)))
+
+    def autotune(self):
+        g, ms = ctypes.c_int32(), ctypes.c_double()
+        check(lib.mmsb_batch_autotune(self.h, ctypes.byref(g), ctypes.byref(ms)))
+        return g.value, ms.value
+
+    def close(self):
+        if self.h:
+            lib.mmsb_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def stgd_grad(ya, yb, yc, mu_a, mu_b, mu_c, phi, coeffs, theta=1.0, stream=None):

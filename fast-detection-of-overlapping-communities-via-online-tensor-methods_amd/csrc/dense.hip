@@ -10,7 +10,10 @@
 // 78.6 TFLOP/s as its FP64 matrix cores, so these kernels use plain FP64 FMAs.
 #include <hip/hip_cooperative_groups.h>
 
+#include <type_traits>
+
 #include "common.h"
+
 #include "dense.h"
 #include "philox.h"
 
@@ -127,19 +130,169 @@ static int tn_slices(int64_t n, int32_t p, int32_t q) {
   return (int)std::max<int64_t>(1, std::min<int64_t>(s, 512));
 }
 
+// ---- FP64 MFMA tall products ---------------------------------------------------------------------
+// v_mfma_f64_16x16x4f64: lane l supplies A[l % 16][l / 16] and B[l / 16][l % 16] (one double each);
+// accumulator register v of lane l holds D[4 v + l / 16][l % 16].
+// Workgroup tile 128 x 128 of the output, four waves of 64 x 64 (4 x 4 MFMA tiles each); the
+// reduction runs in stages of 16 through LDS (one buffer, the next stage's global loads in flight
+// in registers during the current stage's MFMAs). Inputs are converted to FP64 on the way into LDS,
+// so FP32 data gives the same products as the FP64 vector kernels.
+constexpr int kMT = 128;        // workgroup tile
+constexpr int kMK = 16;         // reduction stage
+constexpr int kMPad = kMT + 16; // LDS row stride (doubles) of the k-major stages
+
+static bool use_mfma(int64_t n, int32_t p, int32_t q) { return n >= 4096 && p >= 32 && q >= 32; }
+
+static int tn_mfma_tiles(int32_t p, int32_t q, bool sym) {
+  const int tp = (int)ceil_div(p, kMT), tq = (int)ceil_div(q, kMT);
+  return sym ? tp * (tp + 1) / 2 : tp * tq;
+}
+
+static int tn_mfma_slices(int64_t n, int32_t p, int32_t q, bool sym) {
+  const int64_t tiles = tn_mfma_tiles(p, q, sym);
+  int64_t s = ceil_div(1024, tiles);
+  const int64_t cap = std::max<int64_t>(1, (int64_t)(128ll << 20) / ((int64_t)p * q * 8));  // partials <= 128 MiB
+  s = std::min<int64_t>(std::min<int64_t>(s, cap), std::max<int64_t>(1, n / 1024));
+  return (int)std::max<int64_t>(1, s);
+}
+
 size_t tn_workspace_bytes(int64_t n, int32_t p, int32_t q) {
-  return sizeof(double) * (size_t)tn_slices(n, p, q) * (size_t)p * (size_t)q;
+  size_t b = sizeof(double) * (size_t)tn_slices(n, p, q) * (size_t)p * (size_t)q;
+  if (use_mfma(n, p, q))
+    for (bool sym : {false, true})
+      b = std::max(b, sizeof(double) * (size_t)tn_mfma_slices(n, p, q, sym) * (size_t)p * (size_t)q);
+  return b;
+}
+
+__device__ inline doublex4 mfma_f64(double a, double b, doublex4 c) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
+// One wave's 64 x 64 block from k-major LDS stages: As[k][i] (rows of the output), Bs[k][j].
+__device__ inline void mfma_stage(doublex4 (&acc)[4][4], const double (*As)[kMPad], const double (*Bs)[kMPad], int ai,
+                                  int bj) {
+  const int l = threadIdx.x & 63, lr = l & 15, lk = l >> 4;
+#pragma unroll
+  for (int kk = 0; kk < kMK; kk += 4) {
+    double a[4], b[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      a[u] = As[kk + lk][ai + 16 * u + lr];
+      b[u] = Bs[kk + lk][bj + 16 * u + lr];
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) acc[u][v] = mfma_f64(a[u], b[v], acc[u][v]);
+  }
+}
+
+// C partial [slice] = A[r0:r1]^T B[r0:r1] for output tile (ti, tj); SYM: A == B, tiles ti <= tj only.
+template <typename TA, typename TB, bool SYM>
+__global__ __launch_bounds__(256) void tn_mfma_kernel(int64_t n, int32_t p, int32_t q, const TA* __restrict__ A,
+                                                      int32_t lda, const TB* __restrict__ B, int32_t ldb, int32_t slices,
+                                                      double* __restrict__ work) {
+  __shared__ double As[kMK][kMPad];
+  __shared__ double Bs[kMK][kMPad];
+  int ti, tj;
+  if constexpr (SYM) {  // upper-triangular tile pairs in row order
+    const int tp = (int)((p + kMT - 1) / kMT);
+    int rem = blockIdx.x;
+    ti = 0;
+    while (rem >= tp - ti) {
+      rem -= tp - ti;
+      ++ti;
+    }
+    tj = ti + rem;
+  } else {
+    const int tq = (int)((q + kMT - 1) / kMT);
+    ti = blockIdx.x / tq;
+    tj = blockIdx.x % tq;
+  }
+  const int slice = blockIdx.y;
+  const int64_t r0 = n * slice / slices, r1 = n * (slice + 1) / slices;
+  const int i0 = ti * kMT, j0 = tj * kMT;
+  const int t = threadIdx.x, w = t >> 6;
+  const int ai = 64 * (w >> 1), bj = 64 * (w & 1);
+  // staging map: thread -> stage row sr, columns sc + 16 e
+  const int sr = t >> 4, sc = t & 15;
+  double ra[8], rb[8];
+  auto load = [&](int64_t base) {
+    const int64_t row = base + sr;
+    const bool rv = row < r1;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      const int ci = i0 + sc + 16 * e, cj = j0 + sc + 16 * e;
+      ra[e] = (rv && ci < p) ? (double)A[row * lda + ci] : 0.0;
+      rb[e] = (rv && cj < q) ? (double)B[row * ldb + cj] : 0.0;
+    }
+  };
+  doublex4 acc[4][4];
+#pragma unroll
+  for (int u = 0; u < 4; ++u)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) acc[u][v] = doublex4{0.0, 0.0, 0.0, 0.0};
+  load(r0);
+  for (int64_t base = r0; base < r1; base += kMK) {
+    __syncthreads();
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      As[sr][sc + 16 * e] = ra[e];
+      Bs[sr][sc + 16 * e] = rb[e];
+    }
+    __syncthreads();
+    if (base + kMK < r1) load(base + kMK);
+    mfma_stage(acc, As, Bs, ai, bj);
+  }
+  double* wp = work + (int64_t)slice * p * q;
+  const int l = t & 63, lr = l & 15, lk = l >> 4;
+#pragma unroll
+  for (int u = 0; u < 4; ++u)
+#pragma unroll
+    for (int v = 0; v < 4; ++v)
+#pragma unroll
+      for (int g = 0; g < 4; ++g) {
+        const int i = i0 + ai + 16 * u + 4 * g + lk, j = j0 + bj + 16 * v + lr;
+        if (i < p && j < q) wp[(int64_t)i * q + j] = acc[u][v][g];
+      }
+}
+
+// C = alpha * sum_s work[s] + beta * C; SYM: entries below the diagonal tiles come from the mirror.
+__global__ void tn_mfma_reduce_kernel(int32_t p, int32_t q, int32_t slices, bool sym, const double* __restrict__ work,
+                                      double alpha, double beta, double* __restrict__ C) {
+  const int64_t pq = (int64_t)p * q;
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= pq) return;
+  const int i = (int)(t / q), j = (int)(t % q);
+  const int64_t src = (sym && i / kMT > j / kMT) ? (int64_t)j * q + i : t;
+  double acc = 0.0;
+  for (int s = 0; s < slices; ++s) acc += work[(int64_t)s * pq + src];
+  C[t] = (beta == 0.0) ? alpha * acc : alpha * acc + beta * C[t];
 }
 
 template <typename TA, typename TB>
 int gemm_tn(hipStream_t s, int64_t n, int32_t p, int32_t q, double alpha, const TA* A, int32_t lda, const TB* B,
             int32_t ldb, double beta, double* C, double* work) {
   MMSB_CHECK_ARG(n >= 0 && p >= 1 && q >= 1 && lda >= p && ldb >= q, "gemm_tn: bad shape");
+  const int64_t pq = (int64_t)p * q;
+  if (use_mfma(n, p, q)) {
+    const bool sym = std::is_same<TA, TB>::value && (const void*)A == (const void*)B && lda == ldb && p == q;
+    const int slices = tn_mfma_slices(n, p, q, sym);
+    dim3 grid((unsigned)tn_mfma_tiles(p, q, sym), (unsigned)slices);
+    if (sym)
+      hipLaunchKernelGGL((tn_mfma_kernel<TA, TB, true>), grid, dim3(256), 0, s, n, p, q, A, lda, B, ldb, slices, work);
+    else
+      hipLaunchKernelGGL((tn_mfma_kernel<TA, TB, false>), grid, dim3(256), 0, s, n, p, q, A, lda, B, ldb, slices, work);
+    MMSB_LAUNCH_CHECK();
+    hipLaunchKernelGGL(tn_mfma_reduce_kernel, dim3((unsigned)ceil_div(pq, 256)), dim3(256), 0, s, p, q, slices, sym, work,
+                       alpha, beta, C);
+    MMSB_LAUNCH_CHECK();
+    return MMSB_OK;
+  }
   const int slices = tn_slices(n, p, q);
   dim3 grid((unsigned)ceil_div(p, 64), (unsigned)ceil_div(q, 64), (unsigned)slices);
   hipLaunchKernelGGL((tn_partial_kernel<TA, TB>), grid, dim3(256), 0, s, n, p, q, A, lda, B, ldb, slices, work);
   MMSB_LAUNCH_CHECK();
-  const int64_t pq = (int64_t)p * q;
   hipLaunchKernelGGL(tn_reduce_kernel, dim3((unsigned)ceil_div(pq, 256)), dim3(256), 0, s, pq, slices, work, alpha,
                      beta, C);
   MMSB_LAUNCH_CHECK();
@@ -208,11 +361,91 @@ __global__ __launch_bounds__(256) void nn_kernel(int64_t n, int32_t p, int32_t q
   }
 }
 
+// Y tile [128 x 128] = alpha * X[rows, :] C[:, cols] (+ beta * Y) on FP64 MFMA; CT: C stored transposed.
+template <typename TX, typename TY, bool CT>
+__global__ __launch_bounds__(256) void nn_mfma_kernel(int64_t n, int32_t p, int32_t q, double alpha,
+                                                      const TX* __restrict__ X, int32_t ldx, const double* __restrict__ C,
+                                                      int32_t ldc, double beta, TY* __restrict__ Y, int32_t ldy) {
+  __shared__ double Xs[kMK][kMPad];  // k-major: Xs[k][row]
+  __shared__ double Cs[kMK][kMPad];  // Cs[k][col]
+  const int64_t r0 = (int64_t)blockIdx.x * kMT;
+  const int c0 = blockIdx.y * kMT;
+  const int t = threadIdx.x, w = t >> 6;
+  const int ai = 64 * (w >> 1), bj = 64 * (w & 1);
+  // X stage map: thread -> row xr + 32 e' ... : 128 rows x 16 k = 2048 values, 8 per thread
+  const int xk = t & 15, xr = t >> 4;  // k = xk, rows xr + 16 e
+  double rx[8], rc[8];
+  auto load = [&](int kb) {
+    const bool kv = kb + xk < p;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      const int64_t row = r0 + xr + 16 * e;
+      rx[e] = (kv && row < n) ? (double)X[row * ldx + kb + xk] : 0.0;
+    }
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      if constexpr (CT) {  // C[col][k]: thread -> k = t & 15, cols (t >> 4) + 16 e
+        const int col = c0 + xr + 16 * e;
+        rc[e] = (kv && col < q) ? C[(int64_t)col * ldc + kb + xk] : 0.0;
+      } else {             // C[k][col]: thread -> k = t >> 4, cols (t & 15) + 16 e
+        const int k = kb + xr, col = c0 + xk + 16 * e;
+        rc[e] = (k < p && col < q) ? C[(int64_t)k * ldc + col] : 0.0;
+      }
+    }
+  };
+  doublex4 acc[4][4];
+#pragma unroll
+  for (int u = 0; u < 4; ++u)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) acc[u][v] = doublex4{0.0, 0.0, 0.0, 0.0};
+  load(0);
+  for (int kb = 0; kb < p; kb += kMK) {
+    __syncthreads();
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      Xs[xk][xr + 16 * e] = rx[e];
+      if constexpr (CT)
+        Cs[xk][xr + 16 * e] = rc[e];
+      else
+        Cs[xr][xk + 16 * e] = rc[e];
+    }
+    __syncthreads();
+    if (kb + kMK < p) load(kb + kMK);
+    mfma_stage(acc, Xs, Cs, ai, bj);
+  }
+  const int l = t & 63, lr = l & 15, lk = l >> 4;
+#pragma unroll
+  for (int u = 0; u < 4; ++u)
+#pragma unroll
+    for (int g = 0; g < 4; ++g) {
+      const int64_t row = r0 + ai + 16 * u + 4 * g + lk;
+      if (row >= n) continue;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int col = c0 + bj + 16 * v + lr;
+        if (col < q) {
+          TY* y = Y + row * ldy + col;
+          const double o = alpha * acc[u][v][g];
+          *y = (TY)((beta == 0.0) ? o : o + beta * (double)*y);
+        }
+      }
+    }
+}
+
 template <typename TX, typename TY>
 int gemm_nn(hipStream_t s, int64_t n, int32_t p, int32_t q, double alpha, const TX* X, int32_t ldx, const double* C,
             int32_t ldc, double beta, TY* Y, int32_t ldy, bool c_trans) {
   MMSB_CHECK_ARG(n >= 0 && p >= 1 && q >= 1 && ldx >= p && ldy >= q && ldc >= (c_trans ? p : q), "gemm_nn: bad shape");
   if (n == 0) return MMSB_OK;
+  if (use_mfma(n, p, q)) {
+    dim3 g((unsigned)ceil_div(n, kMT), (unsigned)ceil_div(q, kMT));
+    if (c_trans)
+      hipLaunchKernelGGL((nn_mfma_kernel<TX, TY, true>), g, dim3(256), 0, s, n, p, q, alpha, X, ldx, C, ldc, beta, Y, ldy);
+    else
+      hipLaunchKernelGGL((nn_mfma_kernel<TX, TY, false>), g, dim3(256), 0, s, n, p, q, alpha, X, ldx, C, ldc, beta, Y, ldy);
+    MMSB_LAUNCH_CHECK();
+    return MMSB_OK;
+  }
   dim3 grid((unsigned)ceil_div(n, 64), (unsigned)ceil_div(q, 64));
   if (c_trans)
     hipLaunchKernelGGL((nn_kernel<TX, TY, true>), grid, dim3(256), 0, s, n, p, q, alpha, X, ldx, C, ldc, beta, Y, ldy);

@@ -151,11 +151,175 @@ __global__ __launch_bounds__(kJacobiThreads) void jacobi_svd_kernel(int32_t r, d
   if (blockIdx.x == 0 && threadIdx.x == 0) sweeps_out[0] = sweep + (sweep < kMaxSweeps ? 1 : 0);
 }
 
+// Block variant: blocks of B consecutive columns; a round pairs blocks by the same circle
+// schedule, and the workgroup holding blocks (P, Q) loads their 2B columns of A and of V into LDS,
+// orthogonalises every pair among them (one inner sweep, 2B - 1 inner rounds of B disjoint pairs,
+// 256 / B threads per pair), and writes them back. A sweep needs nblk - 1 grid barriers instead
+// of r - 1. Convergence test, ordering of the output and exit protocol as above.
+template <int B>
+__global__ __launch_bounds__(kJacobiThreads) void jacobi_block_kernel(int32_t r, double* __restrict__ A,
+                                                                       double* __restrict__ U, double* __restrict__ sigma,
+                                                                       double* __restrict__ V, double* __restrict__ Vw,
+                                                                       double* __restrict__ sw, int32_t* __restrict__ counters,
+                                                                       int32_t* __restrict__ sweeps_out) {
+  constexpr int TPG = kJacobiThreads / B;  // threads per pair (<= 64: one wave or half)
+  extern __shared__ double lds[];          // [2B][r] columns of A, then [2B][r] columns of V
+  cg::grid_group grid = cg::this_grid();
+  __shared__ double red[kJacobiThreads / 64];
+  __shared__ int go_on, nrot;
+  double* la = lds;
+  double* lv = lds + (int64_t)2 * B * r;
+  const int nwg = gridDim.x;
+  const int64_t rr = (int64_t)r * r;
+  const int64_t gtid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t gsz = (int64_t)nwg * blockDim.x;
+  for (int64_t e = gtid; e < rr; e += gsz) Vw[e] = ((e / r) == (e % r)) ? 1.0 : 0.0;
+  for (int64_t e = gtid; e < kMaxSweeps + 2; e += gsz) counters[e] = 0;
+  grid.sync();
+
+  const int nblk = (r + B - 1) / B;
+  const int nbp = nblk + (nblk & 1);
+  const int npairs = nbp / 2;
+  const int g = threadIdx.x / TPG, tg = threadIdx.x % TPG;
+  int sweep = 0;
+  for (; sweep < kMaxSweeps; ++sweep) {
+    for (int round = 0; round < nbp - 1; ++round) {
+      for (int pi = blockIdx.x; pi < npairs; pi += nwg) {
+        int P, Q;
+        pair_of(nbp, round, pi, P, Q);
+        auto gcol = [&](int lc) -> int {  // global column of local column lc, or -1
+          const int blkid = lc < B ? P : Q;
+          const int c = blkid * B + (lc % B);
+          return (blkid < nblk && c < r) ? c : -1;
+        };
+        for (int lc = 0; lc < 2 * B; ++lc) {
+          const int c = gcol(lc);
+          if (c < 0) continue;
+          for (int i = threadIdx.x; i < r; i += blockDim.x) {
+            la[(int64_t)lc * r + i] = A[(int64_t)c * r + i];
+            lv[(int64_t)lc * r + i] = Vw[(int64_t)c * r + i];
+          }
+        }
+        if (threadIdx.x == 0) nrot = 0;
+        __syncthreads();
+        int myrot = 0;
+        for (int ir = 0; ir < 2 * B - 1; ++ir) {
+          int lp, lq;
+          pair_of(2 * B, ir, g, lp, lq);
+          if (gcol(lp) >= 0 && gcol(lq) >= 0) {
+            double* ap = la + (int64_t)lp * r;
+            double* aq = la + (int64_t)lq * r;
+            double a = 0.0, b = 0.0, gg = 0.0;
+            for (int i = tg; i < r; i += TPG) {
+              const double x = ap[i], y = aq[i];
+              a = fma(x, x, a);
+              b = fma(y, y, b);
+              gg = fma(x, y, gg);
+            }
+            for (int off = TPG / 2; off > 0; off >>= 1) {
+              a += __shfl_xor(a, off);
+              b += __shfl_xor(b, off);
+              gg += __shfl_xor(gg, off);
+            }
+            if (gg != 0.0 && fabs(gg) > 1e-15 * sqrt(a * b)) {
+              const double zeta = (b - a) / (2.0 * gg);
+              const double t = (zeta >= 0.0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
+              const double c = 1.0 / sqrt(1.0 + t * t);
+              const double sn = c * t;
+              for (int i = tg; i < r; i += TPG) {
+                const double x = ap[i], y = aq[i];
+                ap[i] = c * x - sn * y;
+                aq[i] = sn * x + c * y;
+              }
+              double* vp = lv + (int64_t)lp * r;
+              double* vq = lv + (int64_t)lq * r;
+              for (int i = tg; i < r; i += TPG) {
+                const double x = vp[i], y = vq[i];
+                vp[i] = c * x - sn * y;
+                vq[i] = sn * x + c * y;
+              }
+              if (tg == 0) ++myrot;
+            }
+          }
+          __syncthreads();
+        }
+        if (tg == 0 && myrot) atomicAdd(&nrot, myrot);
+        for (int lc = 0; lc < 2 * B; ++lc) {
+          const int c = gcol(lc);
+          if (c < 0) continue;
+          for (int i = threadIdx.x; i < r; i += blockDim.x) {
+            A[(int64_t)c * r + i] = la[(int64_t)lc * r + i];
+            Vw[(int64_t)c * r + i] = lv[(int64_t)lc * r + i];
+          }
+        }
+        __syncthreads();
+        if (threadIdx.x == 0 && nrot) atomicAdd(counters + sweep, nrot);
+        __syncthreads();
+      }
+      grid.sync();
+    }
+    if (threadIdx.x == 0) go_on = atomicAdd(counters + sweep, 0) != 0;
+    __syncthreads();
+    if (!go_on) break;
+  }
+  for (int j = blockIdx.x; j < r; j += nwg) {
+    const double* aj = A + (int64_t)j * r;
+    double a = 0.0;
+    for (int i = threadIdx.x; i < r; i += blockDim.x) a = fma(aj[i], aj[i], a);
+    a = block_sum(a, red);
+    if (threadIdx.x == 0) sw[j] = sqrt(a);
+    __syncthreads();
+  }
+  grid.sync();
+  for (int j = blockIdx.x; j < r; j += nwg) {
+    const double sj = sw[j];
+    int cnt = 0;
+    for (int i = threadIdx.x; i < r; i += blockDim.x) {
+      const double si = sw[i];
+      cnt += (si > sj || (si == sj && i < j)) ? 1 : 0;
+    }
+    const int rank = (int)block_sum((double)cnt, red);
+    const double inv = sj > 0.0 ? 1.0 / sj : 0.0;
+    const double* aj = A + (int64_t)j * r;
+    const double* vj = Vw + (int64_t)j * r;
+    for (int i = threadIdx.x; i < r; i += blockDim.x) {
+      U[(int64_t)rank * r + i] = aj[i] * inv;
+      V[(int64_t)rank * r + i] = vj[i];
+    }
+    if (threadIdx.x == 0) sigma[rank] = sj;
+    __syncthreads();
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) sweeps_out[0] = sweep + (sweep < kMaxSweeps ? 1 : 0);
+}
+
+template <int B>
+static int launch_block(hipStream_t s, int32_t r, double* A, double* U, double* sigma, double* V, double* Vw, double* sw,
+                        int32_t* counters, int32_t* sweeps_out) {
+  const void* fn = reinterpret_cast<const void*>(jacobi_block_kernel<B>);
+  const size_t shm = sizeof(double) * (size_t)4 * B * r;
+  MMSB_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
+  int dev = 0, per_cu = 0, cus = 0;
+  MMSB_HIP(hipGetDevice(&dev));
+  MMSB_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, kJacobiThreads, shm));
+  MMSB_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  if (per_cu < 1) return set_error(MMSB_ERR_HIP, "jacobi: block kernel does not fit (%zu bytes of LDS)", shm);
+  const int nblk = (r + B - 1) / B;
+  const int npairs = (nblk + (nblk & 1)) / 2;
+  const int nwg = std::max(1, std::min(npairs, per_cu * cus));
+  void* args[] = {&r, &A, &U, &sigma, &V, &Vw, &sw, &counters, &sweeps_out};
+  MMSB_HIP(hipLaunchCooperativeKernel(fn, dim3(nwg), dim3(kJacobiThreads), args, shm, s));
+  return MMSB_OK;
+}
+
 int jacobi_svd(hipStream_t s, int32_t r, double* A, double* U, double* sigma, double* V, void* work, int32_t* sweeps_out) {
   MMSB_CHECK_ARG(r >= 1 && r <= 8192, "jacobi: r=%d out of range", r);
   double* Vw = static_cast<double*>(work);
   double* sw = Vw + (size_t)r * r;
   int32_t* counters = reinterpret_cast<int32_t*>(sw + r);
+  // block variant while 2B columns of A and V (4 B r doubles) fit in the 160 KiB of LDS
+  auto fits = [&](int b) { return sizeof(double) * (size_t)4 * b * r + 256 <= (size_t)160 * 1024; };
+  if (r >= 64 && fits(8)) return launch_block<8>(s, r, A, U, sigma, V, Vw, sw, counters, sweeps_out);
+  if (r >= 64 && fits(4)) return launch_block<4>(s, r, A, U, sigma, V, Vw, sw, counters, sweeps_out);
   int dev = 0;
   MMSB_HIP(hipGetDevice(&dev));
   int per_cu = 0, cus = 0;

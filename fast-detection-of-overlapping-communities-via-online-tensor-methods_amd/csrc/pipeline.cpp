@@ -431,28 +431,28 @@ static int prepare_bf16(mmsb_ctx* c) {
   return MMSB_OK;
 }
 
-static int run_stgd_steps(mmsb_ctx* c, int64_t nsteps, int32_t precision, StgdTiming* timing = nullptr) {
-  if (nsteps <= 0) return MMSB_OK;
-  if (precision != MMSB_PREC_FP32 && precision != MMSB_PREC_BF16)
-    return set_error(MMSB_ERR_ARG, "stgd: unknown precision %d", precision);
-  if (precision == MMSB_PREC_BF16) TRY(prepare_bf16(c));
-  if (c->steps_done + nsteps > c->frob_cap) {
-    int64_t cap = std::max<int64_t>(1024, 2 * (c->steps_done + nsteps));
-    double* nf = nullptr;
-    HCHK(hipMalloc(&nf, sizeof(double) * cap));
-    HCHK(hipMemsetAsync(nf, 0, sizeof(double) * cap, c->s));
-    if (c->frob && c->steps_done)
-      HCHK(hipMemcpyAsync(nf, c->frob, sizeof(double) * c->steps_done, hipMemcpyDeviceToDevice, c->s));
-    if (c->frob) {
-      HCHK(hipStreamSynchronize(c->s));
-      auto it = std::find(c->allocs.begin(), c->allocs.end(), (void*)c->frob);
-      if (it != c->allocs.end()) c->allocs.erase(it);
-      HCHK(hipFree(c->frob));
-    }
-    c->allocs.push_back(nf);
-    c->frob = nf;
-    c->frob_cap = cap;
+// Grows the ||dPhi||^2 log to hold `more` further steps.
+static int ensure_frob(mmsb_ctx* c, int64_t more) {
+  if (c->steps_done + more <= c->frob_cap) return MMSB_OK;
+  int64_t cap = std::max<int64_t>(1024, 2 * (c->steps_done + more));
+  double* nf = nullptr;
+  HCHK(hipMalloc(&nf, sizeof(double) * cap));
+  HCHK(hipMemsetAsync(nf, 0, sizeof(double) * cap, c->s));
+  if (c->frob && c->steps_done)
+    HCHK(hipMemcpyAsync(nf, c->frob, sizeof(double) * c->steps_done, hipMemcpyDeviceToDevice, c->s));
+  if (c->frob) {
+    HCHK(hipStreamSynchronize(c->s));
+    auto it = std::find(c->allocs.begin(), c->allocs.end(), (void*)c->frob);
+    if (it != c->allocs.end()) c->allocs.erase(it);
+    HCHK(hipFree(c->frob));
   }
+  c->allocs.push_back(nf);
+  c->frob = nf;
+  c->frob_cap = cap;
+  return MMSB_OK;
+}
+
+static StgdArgs stgd_args(const mmsb_ctx* c) {
   StgdArgs a{};
   a.kp = c->kp;
   a.b = c->b;
@@ -470,7 +470,10 @@ static int run_stgd_steps(mmsb_ctx* c, int64_t nsteps, int32_t precision, StgdTi
   a.mu_c = c->muw[2];
   a.cs_part = c->cs_part;
   a.mab = c->mab;
-  const int64_t spe = (c->n[0] + c->bl - 1) / c->bl;
+  return a;
+}
+
+static StgdBuffers stgd_bufs(const mmsb_ctx* c) {
   StgdBuffers bufs{};
   bufs.phi[0] = c->phi[0];
   bufs.phi[1] = c->phi[1];
@@ -493,10 +496,27 @@ static int run_stgd_steps(mmsb_ctx* c, int64_t nsteps, int32_t precision, StgdTi
   bufs.cgt = c->cgt;
   bufs.rowsum = c->ns_rowsum;
   bufs.mpart = c->mpart;
+  return bufs;
+}
+
+static void y_ptrs(const mmsb_ctx* c, int32_t precision, const void*& ya, const void*& yb, const void*& yc) {
   const bool bf = precision == MMSB_PREC_BF16;
-  const void* ya = bf ? (const void*)c->YA16 : (const void*)c->YA;
-  const void* yb = bf ? (const void*)c->YB16 : (const void*)c->YB;
-  const void* yc = bf ? (const void*)c->YCt16 : (const void*)c->YCt;
+  ya = bf ? (const void*)c->YA16 : (const void*)c->YA;
+  yb = bf ? (const void*)c->YB16 : (const void*)c->YB;
+  yc = bf ? (const void*)c->YCt16 : (const void*)c->YCt;
+}
+
+static int run_stgd_steps(mmsb_ctx* c, int64_t nsteps, int32_t precision, StgdTiming* timing = nullptr) {
+  if (nsteps <= 0) return MMSB_OK;
+  if (precision != MMSB_PREC_FP32 && precision != MMSB_PREC_BF16)
+    return set_error(MMSB_ERR_ARG, "stgd: unknown precision %d", precision);
+  if (precision == MMSB_PREC_BF16) TRY(prepare_bf16(c));
+  TRY(ensure_frob(c, nsteps));
+  const StgdArgs a = stgd_args(c);
+  const StgdBuffers bufs = stgd_bufs(c);
+  const void *ya, *yb, *yc;
+  y_ptrs(c, precision, ya, yb, yc);
+  const int64_t spe = (c->n[0] + c->bl - 1) / c->bl;
   auto launch = [&](int64_t step0, int64_t ns, double* frob_out, int& cur, StgdTiming* tm) -> int {
     return stgd_run(c->s, precision, a, ya, yb, yc, step0, ns, c->cfg.renorm_every, c->cfg.ns_iters, bufs, cur, frob_out, tm);
   };
@@ -538,6 +558,238 @@ static int run_stgd_steps(mmsb_ctx* c, int64_t nsteps, int32_t precision, StgdTi
   c->steps_done += nsteps;
   return MMSB_OK;
 }
+
+}  // namespace mmsb
+
+// ---- many instances: one persistent launch per epoch -------------------------------------------
+// The plan of each epoch parity covers one epoch of every member (members may have different n, k
+// and minibatch); each plan is captured once as a HIP graph (queue reset kernel + persistent kernel)
+// and replayed per epoch. A member's Phi ping-pong flips every epoch when its steps per epoch are
+// odd, so parity 1 differs from parity 0 exactly in those members' cur.
+struct mmsb_batch {
+  std::vector<mmsb_ctx*> ctx;
+  int32_t precision = MMSB_PREC_FP32;
+  hipStream_t s = nullptr;
+  int device = 0;
+  StgdBatch* plan[2] = {nullptr, nullptr};
+  hipGraphExec_t graph[2] = {nullptr, nullptr};
+  std::vector<int> cur0;       // members' cur at even batch epochs
+  std::vector<int64_t> spe;    // steps per epoch
+  int64_t epochs_done = 0;
+  std::vector<float*> save;    // autotuning: snapshots of each member's Phi
+};
+
+static int64_t steps_per_epoch(const mmsb_ctx* c) { return (c->n[0] + c->bl - 1) / c->bl; }
+
+static void batch_drop_graphs(mmsb_batch* b) {
+  for (hipGraphExec_t& g : b->graph)
+    if (g) {
+      (void)hipGraphExecDestroy(g);
+      g = nullptr;
+    }
+}
+
+extern "C" void mmsb_batch_destroy(mmsb_batch* b) {
+  if (!b) return;
+  (void)hipSetDevice(b->device);
+  if (b->s) (void)hipStreamSynchronize(b->s);
+  batch_drop_graphs(b);
+  for (StgdBatch*& p : b->plan) stgd_batch_free(p);
+  for (float* p : b->save) (void)hipFree(p);
+  delete b;
+}
+
+extern "C" int mmsb_batch_create(mmsb_ctx* const* ctxs, int32_t count, int32_t precision, mmsb_batch** out) {
+  clear_error();
+  MMSB_CHECK_ARG(ctxs && out && count >= 1, "batch_create: bad arguments");
+  *out = nullptr;
+  MMSB_CHECK_ARG(precision == MMSB_PREC_FP32 || precision == MMSB_PREC_BF16, "batch_create: unknown precision %d", precision);
+  const mmsb_ctx* c0 = ctxs[0];
+  MMSB_CHECK_ARG(c0, "batch_create: null context 0");
+  for (int32_t i = 0; i < count; ++i) {
+    const mmsb_ctx* c = ctxs[i];
+    MMSB_CHECK_ARG(c, "batch_create: null context %d", i);
+    if (!c->have_y) return set_error(MMSB_ERR_STATE, "batch_create: context %d has no neighbour vectors", i);
+    MMSB_CHECK_ARG(c->cfg.device == c0->cfg.device, "batch_create: contexts on different devices");
+    MMSB_CHECK_ARG(c->cfg.ns_iters == c0->cfg.ns_iters && c->cfg.renorm_every == c0->cfg.renorm_every,
+                   "batch_create: contexts disagree on ns_iters / renorm_every");
+    const int64_t spe = steps_per_epoch(c);
+    MMSB_CHECK_ARG(c->steps_done % spe == 0, "batch_create: context %d is not at an epoch boundary", i);
+    MMSB_CHECK_ARG(c->cfg.renorm_every <= 1 || spe % c->cfg.renorm_every == 0,
+                   "batch_create: context %d: renorm_every must divide the steps per epoch", i);
+  }
+  HCHK(hipSetDevice(c0->cfg.device));
+  auto* b = new mmsb_batch();
+  b->precision = precision;
+  b->device = c0->cfg.device;
+  b->s = c0->s;
+  auto fail = [&](int st) {
+    mmsb_batch_destroy(b);
+    return st;
+  };
+  std::vector<StgdBatchMember> mem[2];
+  for (int32_t i = 0; i < count; ++i) {
+    mmsb_ctx* c = ctxs[i];
+    int st = alloc_stgd(c);
+    if (st == MMSB_OK && !c->have_phi) st = init_phi(c);
+    if (st == MMSB_OK && precision == MMSB_PREC_BF16) st = prepare_bf16(c);
+    const int64_t spe = steps_per_epoch(c);
+    if (st == MMSB_OK) st = dalloc(c, c->frob_epoch, (size_t)spe);
+    if (st != MMSB_OK) return fail(st);
+    if (hipStreamSynchronize(c->s) != hipSuccess) return fail(set_error(MMSB_ERR_HIP, "batch_create: stream sync failed"));
+    b->ctx.push_back(c);
+    b->cur0.push_back(c->cur);
+    b->spe.push_back(spe);
+    for (int p = 0; p < 2; ++p) {
+      StgdBatchMember m{};
+      m.args = stgd_args(c);
+      m.bufs = stgd_bufs(c);
+      y_ptrs(c, precision, m.YA, m.YB, m.YCt);
+      m.cur = c->cur ^ (int)((p * spe) & 1);
+      m.step0 = 0;
+      m.nsteps = spe;
+      m.frob_log = c->frob_epoch;
+      mem[p].push_back(m);
+    }
+  }
+  for (int p = 0; p < 2; ++p) {
+    const int st = stgd_batch_plan(b->plan[p], precision, mem[p], c0->cfg.ns_iters, c0->cfg.renorm_every);
+    if (st != MMSB_OK) return fail(st);
+  }
+  *out = b;
+  return MMSB_OK;
+}
+
+static int batch_check_state(mmsb_batch* b) {
+  const int parity = (int)(b->epochs_done & 1);
+  for (size_t i = 0; i < b->ctx.size(); ++i) {
+    const mmsb_ctx* c = b->ctx[i];
+    if (c->steps_done % b->spe[i] != 0 || c->cur != (b->cur0[i] ^ (int)((parity * b->spe[i]) & 1)))
+      return set_error(MMSB_ERR_STATE, "batch: member %zu advanced outside the batch; create a new batch", i);
+  }
+  return MMSB_OK;
+}
+
+extern "C" int mmsb_batch_run(mmsb_batch* b, int32_t epochs) {
+  clear_error();
+  MMSB_CHECK_ARG(b && epochs >= 0, "batch_run: bad arguments");
+  HCHK(hipSetDevice(b->device));
+  TRY(batch_check_state(b));
+  for (mmsb_ctx* c : b->ctx) {
+    TRY(ensure_frob(c, (int64_t)epochs * steps_per_epoch(c)));
+    HCHK(hipStreamSynchronize(c->s));
+  }
+  for (int32_t e = 0; e < epochs; ++e) {
+    const int p = (int)((b->epochs_done + e) & 1);
+    if (!b->graph[p]) {
+      hipGraph_t g = nullptr;
+      HCHK(hipStreamBeginCapture(b->s, hipStreamCaptureModeThreadLocal));
+      const int st = stgd_batch_launch(b->s, b->plan[p]);
+      hipError_t ce = hipStreamEndCapture(b->s, &g);
+      if (st != MMSB_OK) {
+        if (g) (void)hipGraphDestroy(g);
+        return st;
+      }
+      HCHK(ce);
+      HCHK(hipGraphInstantiate(&b->graph[p], g, nullptr, nullptr, 0));
+      HCHK(hipGraphDestroy(g));
+    }
+    HCHK(hipGraphLaunch(b->graph[p], b->s));
+    for (size_t i = 0; i < b->ctx.size(); ++i) {
+      mmsb_ctx* c = b->ctx[i];
+      HCHK(hipMemcpyAsync(c->frob + c->steps_done, c->frob_epoch, sizeof(double) * b->spe[i], hipMemcpyDeviceToDevice, b->s));
+      c->steps_done += b->spe[i];
+      if (b->spe[i] & 1) c->cur ^= 1;
+      c->eig_done = c->post_done = false;
+    }
+  }
+  b->epochs_done += epochs;
+  int32_t err = 0;
+  TRY(stgd_batch_error(b->s, b->plan[b->epochs_done & 1], &err));
+  if (epochs > 0) {
+    int32_t err2 = 0;
+    TRY(stgd_batch_error(b->s, b->plan[(b->epochs_done + 1) & 1], &err2));
+    err |= err2;
+  }
+  if (err) return set_error(MMSB_ERR_NUMERIC, "batch_run: a dependency wait exceeded its spin limit; results are invalid");
+  return MMSB_OK;
+}
+
+extern "C" int mmsb_batch_autotune(mmsb_batch* b, int32_t* best_grid, double* best_ms) {
+  clear_error();
+  MMSB_CHECK_ARG(b, "batch_autotune: null batch");
+  HCHK(hipSetDevice(b->device));
+  TRY(batch_check_state(b));
+  for (mmsb_ctx* c : b->ctx) HCHK(hipStreamSynchronize(c->s));
+  const int p = (int)(b->epochs_done & 1);
+  StgdBatch* plan = b->plan[p];
+  if (b->save.empty())
+    for (mmsb_ctx* c : b->ctx) {
+      float* q = nullptr;
+      HCHK(hipMalloc(&q, sizeof(float) * (size_t)c->kp * c->kp));
+      b->save.push_back(q);
+    }
+  for (size_t i = 0; i < b->ctx.size(); ++i) {
+    const mmsb_ctx* c = b->ctx[i];
+    HCHK(hipMemcpyAsync(b->save[i], c->phi[c->cur], sizeof(float) * (size_t)c->kp * c->kp, hipMemcpyDeviceToDevice, b->s));
+  }
+  auto restore = [&]() -> int {
+    for (size_t i = 0; i < b->ctx.size(); ++i) {
+      const mmsb_ctx* c = b->ctx[i];
+      HCHK(hipMemcpyAsync(c->phi[c->cur], b->save[i], sizeof(float) * (size_t)c->kp * c->kp, hipMemcpyDeviceToDevice,
+                          b->s));
+    }
+    return MMSB_OK;
+  };
+  int dev = 0, cus = 0;
+  HCHK(hipGetDevice(&dev));
+  HCHK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  const int32_t gmax = stgd_batch_grid(plan);
+  std::vector<int32_t> cands;
+  for (int32_t g : {cus / 2, cus, 2 * cus, 3 * cus, 4 * cus})
+    if (g >= 1 && g <= gmax && std::find(cands.begin(), cands.end(), g) == cands.end()) cands.push_back(g);
+  if (cands.empty()) cands.push_back(gmax);
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  HCHK(hipEventCreate(&e0));
+  HCHK(hipEventCreate(&e1));
+  int32_t best = cands[0];
+  double bms = 1e300;
+  int st = MMSB_OK;
+  for (int32_t g : cands) {
+    stgd_batch_set_grid(plan, g);
+    double ms_min = 1e300;
+    for (int rep = 0; rep < 2 && st == MMSB_OK; ++rep) {
+      if ((st = restore()) != MMSB_OK) break;
+      if (hipEventRecord(e0, b->s) != hipSuccess) st = set_error(MMSB_ERR_HIP, "batch_autotune: event record failed");
+      if (st == MMSB_OK) st = stgd_batch_launch(b->s, plan);
+      if (st == MMSB_OK && hipEventRecord(e1, b->s) != hipSuccess) st = set_error(MMSB_ERR_HIP, "batch_autotune: event record failed");
+      if (st == MMSB_OK && hipEventSynchronize(e1) != hipSuccess) st = set_error(MMSB_ERR_HIP, "batch_autotune: sync failed");
+      float ms = 0.f;
+      if (st == MMSB_OK && hipEventElapsedTime(&ms, e0, e1) != hipSuccess) st = set_error(MMSB_ERR_HIP, "batch_autotune: timing failed");
+      int32_t err = 0;
+      if (st == MMSB_OK) st = stgd_batch_error(b->s, plan, &err);
+      if (st == MMSB_OK && err) st = set_error(MMSB_ERR_NUMERIC, "batch_autotune: a dependency wait exceeded its spin limit");
+      ms_min = std::min(ms_min, (double)ms);
+    }
+    if (st != MMSB_OK) break;
+    if (ms_min < bms) {
+      bms = ms_min;
+      best = g;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  TRY(st);
+  TRY(restore());
+  HCHK(hipStreamSynchronize(b->s));
+  for (StgdBatch* q : b->plan) stgd_batch_set_grid(q, best);
+  batch_drop_graphs(b);
+  if (best_grid) *best_grid = best;
+  if (best_ms) *best_ms = bms;
+  return MMSB_OK;
+}
+
+namespace mmsb {
 
 // ---- eigenvalues and Pi ------------------------------------------------------------------------
 static int eigen_impl(mmsb_ctx* c) {

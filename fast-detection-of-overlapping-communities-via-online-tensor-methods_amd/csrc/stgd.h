@@ -2,7 +2,9 @@
 #pragma once
 
 #include <hip/hip_runtime_api.h>
+
 #include <cstdint>
+#include <vector>
 
 namespace mmsb {
 
@@ -67,6 +69,35 @@ struct StgdTiming {
 int stgd_run(hipStream_t s, int32_t precision, const StgdArgs& args, const void* YA, const void* YB, const void* YCt,
              int64_t step0, int64_t nsteps, int32_t renorm_every, int32_t ns_iters, const StgdBuffers& bufs, int& cur,
              double* frob_log, StgdTiming* timing = nullptr);
+
+// ---- many instances in one persistent launch ----------------------------------------------------
+// One member of a batch: its step arguments, buffers and neighbour vectors (FP32 or BF16 as the
+// batch precision), the state of its Phi ping-pong, and the steps of one launch.
+struct StgdBatchMember {
+  StgdArgs args;
+  StgdBuffers bufs;
+  const void* YA;
+  const void* YB;
+  const void* YCt;
+  int32_t cur;        // bufs.phi[cur] holds Phi at the start of every launch of the plan
+  int64_t step0;      // global step index of the launch's first step
+  int64_t nsteps;     // steps of one launch (e.g. one epoch)
+  double* frob_log;   // [nsteps]
+};
+
+struct StgdBatch;
+// Lays out the tiles of every member's steps as one ticket sequence and uploads it. The plan is
+// fixed: every launch replays the same steps (the members' cur must match at each launch).
+int stgd_batch_plan(StgdBatch*& out, int32_t precision, const std::vector<StgdBatchMember>& members, int32_t ns_iters,
+                    int32_t renorm_every);
+// Resets the queue counters (a kernel, so the pair is capturable in a HIP graph) and launches the
+// persistent kernel.
+int stgd_batch_launch(hipStream_t s, StgdBatch* b);
+// Reads the error flag (non-zero: a dependency wait hit its spin limit; the results are invalid).
+int stgd_batch_error(hipStream_t s, StgdBatch* b, int32_t* err);
+void stgd_batch_set_grid(StgdBatch* b, int32_t grid);
+int32_t stgd_batch_grid(const StgdBatch* b);
+void stgd_batch_free(StgdBatch* b);
 
 // BF16 buffers cross the host boundary as opaque 16-bit words.
 typedef uint16_t bf16_bits;

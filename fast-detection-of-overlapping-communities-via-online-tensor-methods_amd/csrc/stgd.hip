@@ -28,7 +28,9 @@
 // in FP32. In BF16 mode every k x k operand is written by the kernel that produces it both in
 // FP32 and as its (hi, lo) BF16 pair, so the MFMA loops load ready operands and convert nothing.
 // m_A and m_B of the next step are reduced from the tiles of the new Phi where it is written.
+#include <algorithm>
 #include <type_traits>
+#include <vector>
 
 #include "common.h"
 #include "stgd.h"
@@ -55,13 +57,13 @@ __host__ __device__ inline Opnd<T> opnd(const T* hi, const T* lo, int64_t ld) {
 }
 
 template <bool AL>
-__device__ inline float4 ld4(const float* __restrict__ p) {
+__device__ inline float4 ld4(const float* p) {
   if constexpr (AL) return *reinterpret_cast<const float4*>(p);
   return make_float4(p[0], p[1], p[2], p[3]);
 }
 
 template <bool AL>
-__device__ inline bf16x8 ld8(const __bf16* __restrict__ p) {
+__device__ inline bf16x8 ld8(const __bf16* p) {
   if constexpr (AL) return *reinterpret_cast<const bf16x8*>(p);
   bf16x8 v;
 #pragma unroll
@@ -307,51 +309,131 @@ __device__ inline void store4(T* p, const float v[4]) {
   }
 }
 
-// ---- P1 --------------------------------------------------------------------------------------------
-// Blocks [0, ntb * ntk): a 32-row slice of the minibatch x 32 columns of C_g; the rest: H blocks.
+// ---- the phases of one step, as per-tile device functions ----------------------------------------
+// A step is the phase sequence P1, P2, then ns_iters x (GRAM, UPDATE), or COPY when the step does not
+// renormalise; SETUP precedes the first step of a run. The same tile functions serve the per-phase
+// launches of one instance (stgd_phase_kernel, blockIdx.x = tile) and the persistent launch over many
+// instances (stgd_batch_kernel, tiles from a work queue).
+enum StgdPhase { PH_SETUP = 0, PH_P1 = 1, PH_P2 = 2, PH_GRAM = 3, PH_UPDATE = 4, PH_COPY = 5 };
+
+// Everything the phases of one step of one instance read or write.
+struct StepCtx {
+  StgdArgs a;          // a.phi = B.phi[cur], a.phit = B.phit
+  StgdBuffers B;
+  const void* YA;
+  const void* YB;
+  const void* YCt;
+  int32_t cur;         // B.phi[cur] holds the step's input Phi
+  int32_t ns_iters;
+  int64_t row0;        // first X row of the minibatch
+  int32_t nb;          // rows in the minibatch
+  double* frob;        // this step's ||Phi_{t+1} - Phi_t||_F^2
+  double* frob_zero;   // SETUP: the run's frob entries, cleared by tile 0
+  int64_t nfrob;
+};
+
+union StgdSmem {
+  struct {
+    Red4 red;
+    float mvec[2][32];
+  } p1;
+  struct {
+    Red4 red;
+    TileShm S;
+    float cmu[32];
+    float cscale;
+  } p2;
+  TileShm S;
+};
+
+__host__ __device__ inline int phase_tiles(int phase, int kp, int b) {
+  const int ntk = kp / 32;
+  return phase == PH_P1 ? (b / 32) * ntk + ntk * ntk : ntk * ntk;
+}
+
+// Newton-Schulz iteration `it` reads (x, xT) and writes (dst, dstT): phit -> phia -> phit -> ...;
+// the last iteration writes phi[cur ^ 1] and phiT.
+struct NsBufs {
+  float *x, *xT, *dst, *dstT;
+  uint16_t *x16, *xT16, *dst16, *dstT16;
+};
+
+__host__ __device__ inline NsBufs ns_bufs(const StgdBuffers& B, int cur, int it, int ns_iters) {
+  NsBufs n;
+  const bool from_t = (it & 1) == 0;
+  const bool last = it == ns_iters - 1;
+  n.x = from_t ? B.phit : B.phia;
+  n.xT = from_t ? B.phitT : B.phiaT;
+  n.x16 = from_t ? B.phit16 : B.phia16;
+  n.xT16 = from_t ? B.phitT16 : B.phiaT16;
+  n.dst = last ? B.phi[cur ^ 1] : (from_t ? B.phia : B.phit);
+  n.dstT = last ? B.phiT : (from_t ? B.phiaT : B.phitT);
+  n.dst16 = last ? B.phi16[cur ^ 1] : (from_t ? B.phia16 : B.phit16);
+  n.dstT16 = last ? B.phiT16 : (from_t ? B.phiaT16 : B.phitT16);
+  return n;
+}
+
+// The operand view of a kp x kp matrix for one mode: FP32 data, or the BF16 (hi, lo) pair.
+template <int MODE>
+__host__ __device__ inline Opnd<op_t<MODE>> pick(const float* f32, const uint16_t* s16, int64_t nel, int64_t ld) {
+  if constexpr (MODE == MM_F32) {
+    (void)s16;
+    (void)nel;
+    return opnd<float>(f32, nullptr, ld);
+  } else {
+    (void)f32;
+    const __bf16* p = reinterpret_cast<const __bf16*>(s16);
+    return opnd<__bf16>(p, p + nel, ld);
+  }
+}
+
+template <bool SPLIT>
+__device__ inline uint16_t* sp(uint16_t* p) {
+  return SPLIT ? p : nullptr;
+}
+
+// P1 tiles [0, ntb * ntk): a 32-row slice of the minibatch x 32 columns of C_g; the rest: H tiles.
 template <int DMODE, int KMODE, typename TY, typename TPH>
-__global__ __launch_bounds__(256) void p1_kernel(StgdArgs a, const TY* __restrict__ YA, const TY* __restrict__ YB,
-                                                 Opnd<op_t<DMODE>> phiT_d, Opnd<op_t<KMODE>> phiT_k,
-                                                 const float* __restrict__ mpart, TPH* __restrict__ cgt, float* __restrict__ h,
-                                                 uint16_t* __restrict__ h16, int64_t row0, int32_t nb) {
-  __shared__ Red4 red;
-  __shared__ float mvec[2][32];
+__device__ void p1_tile(const StepCtx& P, int tile, StgdSmem& sm) {
+  constexpr bool SPLIT = KMODE == MM_BF16X3;
+  const StgdArgs& a = P.a;
   const int kp = a.kp;
   const int ntk = kp / 32, ntb = a.b / 32;
+  const int64_t nel = (int64_t)kp * kp;
   const int w = threadIdx.x >> 6;
-  int tile = blockIdx.x;
+  Red4& red = sm.p1.red;
   floatx16 acc = {};
   int64_t k0, kl;
   if (tile < ntb * ntk) {
     const int tb = tile / ntk, tj = tile % ntk;
     const int64_t x0 = (int64_t)tb * 32, j0 = (int64_t)tj * 32;
     // waves 0, 1: P_A over two halves of the k reduction; waves 2, 3: P_B
-    const Opnd<TY> Y = opnd<TY>((w < 2 ? YA : YB) + row0 * (int64_t)kp, nullptr, kp);
+    const TY* Yb = static_cast<const TY*>(w < 2 ? P.YA : P.YB) + P.row0 * (int64_t)kp;
     kslice<DMODE>(kp, 2, w & 1, k0, kl);
-    wave_mm<DMODE, true>(acc, Y, phiT_d, x0, j0, k0, kl);
+    wave_mm<DMODE, true>(acc, opnd<TY>(Yb, nullptr, kp), pick<DMODE>(P.B.phiT, P.B.phiT16, nel, kp), x0, j0, k0, kl);
     if (threadIdx.x < 64) {  // m_A, m_B of the block's columns from the per-row-block partials
       const int s = threadIdx.x >> 5, c = threadIdx.x & 31;
       float m = 0.f;
-      for (int ti = 0; ti < ntk; ++ti) m += mpart[((int64_t)ti * 2 + s) * kp + j0 + c];
-      mvec[s][c] = m;
+      for (int ti = 0; ti < ntk; ++ti) m += P.B.mpart[((int64_t)ti * 2 + s) * kp + j0 + c];
+      sm.p1.mvec[s][c] = m;
     }
     red_put(red, acc);
     __syncthreads();
     // C_g^T rows are columns j of the block: thread -> column c, rows r4 .. r4 + 3
     const int c = threadIdx.x >> 3, r4 = 4 * (threadIdx.x & 7);
-    const float mA = mvec[0][c], mB = mvec[1][c];
+    const float mA = sm.p1.mvec[0][c], mB = sm.p1.mvec[1][c];
     float cg[4], cs = 0.f;
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
       const int r = r4 + e;
       const float p = red_get(red, 0, r, c) + red_get(red, 1, r, c);
       const float q = red_get(red, 2, r, c) + red_get(red, 3, r, c);
-      const bool valid = (x0 + r) < nb;
+      const bool valid = (x0 + r) < P.nb;
       const float ab = p * q;
       cs += valid ? ab : 0.f;
       cg[e] = valid ? a.c3 * ab + a.c1 * (p * mB + mA * q) : 0.f;
     }
-    store4(cgt + (j0 + c) * (int64_t)a.b + x0 + r4, cg);
+    store4(static_cast<TPH*>(P.B.cgt) + (j0 + c) * (int64_t)a.b + x0 + r4, cg);
     for (int off = 1; off < 8; off <<= 1) cs += __shfl_xor(cs, off);
     if ((threadIdx.x & 7) == 0) {
       a.cs_part[(int64_t)tb * kp + j0 + c] = cs;
@@ -364,6 +446,7 @@ __global__ __launch_bounds__(256) void p1_kernel(StgdArgs a, const TY* __restric
     tile -= ntb * ntk;
     const int ti = tile / ntk, tj = tile % ntk;
     const int64_t i0 = (int64_t)ti * 32, j0 = (int64_t)tj * 32;
+    const Opnd<op_t<KMODE>> phiT_k = pick<KMODE>(P.B.phiT, P.B.phiT16, nel, kp);
     kslice<KMODE>(kp, 4, w, k0, kl);
     wave_mm<KMODE, true>(acc, phiT_k, phiT_k, i0, j0, k0, kl);  // G = Phi^T Phi
     red_put(red, acc);
@@ -375,43 +458,47 @@ __global__ __launch_bounds__(256) void p1_kernel(StgdArgs a, const TY* __restric
       g[e] = red_sum4(red, r, c4 + e);
       g[e] *= g[e];
     }
-    put4<KMODE == MM_BF16X3>(h, h16, (int64_t)kp * kp, (i0 + r) * kp + j0 + c4, g);
+    put4<SPLIT>(P.B.h, sp<SPLIT>(P.B.h16), nel, (i0 + r) * kp + j0 + c4, g);
   }
 }
 
-// ---- P2 --------------------------------------------------------------------------------------------
-template <int DMODE, int KMODE, bool YAL, typename TY, typename TPH>
-__global__ __launch_bounds__(256) void p2_kernel(StgdArgs a, const TY* __restrict__ YCt, const TPH* __restrict__ cgt,
-                                                 Opnd<op_t<KMODE>> phi_k, Opnd<op_t<KMODE>> h_k, uint16_t* __restrict__ phit16,
-                                                 float* __restrict__ phitT, uint16_t* __restrict__ phitT16,
-                                                 float* __restrict__ rowsum0, int64_t row0, int32_t nb) {
-  __shared__ Red4 red;
-  __shared__ TileShm S;
-  __shared__ float cmu[32];
+template <int DMODE, int KMODE, typename TY, typename TPH>
+__device__ void p2_tile(const StepCtx& P, int tile, StgdSmem& sm) {
+  constexpr bool SPLIT = KMODE == MM_BF16X3;
+  constexpr int VEC = 16 / sizeof(TY);  // Y_C^T elements per 16-byte load
+  const StgdArgs& a = P.a;
   const int kp = a.kp, ntk = kp / 32;
+  const int64_t nel = (int64_t)kp * kp;
   const int w = threadIdx.x >> 6;
-  const int ti = blockIdx.x / ntk, tj = blockIdx.x % ntk;
+  const int ti = tile / ntk, tj = tile % ntk;
   const int64_t i0 = (int64_t)ti * 32, j0 = (int64_t)tj * 32;
+  Red4& red = sm.p2.red;
   floatx16 acc = {};
   int64_t k0, kl;
   if (w < 2) {  // Phi H over two halves of k (H symmetric: row j of H is column j)
     kslice<KMODE>(kp, 2, w, k0, kl);
-    wave_mm<KMODE, true>(acc, phi_k, h_k, i0, j0, k0, kl);
+    wave_mm<KMODE, true>(acc, pick<KMODE>(a.phi, P.B.phi16[P.cur], nel, kp), pick<KMODE>(P.B.h, P.B.h16, nel, kp), i0, j0,
+                         k0, kl);
   } else {      // Y_C^T C_g over two halves of the minibatch tile
     kslice<DMODE>(a.b, 2, w & 1, k0, kl);
-    wave_mm<DMODE, YAL>(acc, opnd<TY>(YCt + row0, nullptr, a.n_pad), opnd<TPH>(cgt, nullptr, a.b), i0, j0, k0, kl);
+    const Opnd<TY> yc = opnd<TY>(static_cast<const TY*>(P.YCt) + P.row0, nullptr, a.n_pad);
+    const Opnd<TPH> cg = opnd<TPH>(static_cast<const TPH*>(P.B.cgt), nullptr, a.b);
+    if (P.row0 % VEC == 0)
+      wave_mm<DMODE, true>(acc, yc, cg, i0, j0, k0, kl);
+    else
+      wave_mm<DMODE, false>(acc, yc, cg, i0, j0, k0, kl);
   }
   if (threadIdx.x < 32) {
     const int c = threadIdx.x;
     float s = 0.f;
     for (int tb = 0; tb < a.b / 32; ++tb) s += a.cs_part[(int64_t)tb * kp + j0 + c];
-    cmu[c] = a.c0 * a.mab[j0 + c] * a.mab[kp + j0 + c] + a.c1 * s / (float)nb;
+    sm.p2.cmu[c] = a.c0 * a.mab[j0 + c] * a.mab[kp + j0 + c] + a.c1 * s / (float)P.nb;
   }
-  if (blockIdx.x == 0)
-    for (int t = threadIdx.x; t < kp; t += blockDim.x) rowsum0[t] = 0.f;
+  if (tile == 0)
+    for (int t = threadIdx.x; t < kp; t += blockDim.x) P.B.rowsum[t] = 0.f;
   red_put(red, acc);
   __syncthreads();
-  const float eta = a.eta, inv_b = 1.f / (float)nb;
+  const float eta = a.eta, inv_b = 1.f / (float)P.nb;
   const int r = threadIdx.x >> 3, c4 = 4 * (threadIdx.x & 7);
   const float muc = (float)a.mu_c[i0 + r];
   const float4 ph = *reinterpret_cast<const float4*>(a.phi + (i0 + r) * kp + j0 + c4);
@@ -422,24 +509,27 @@ __global__ __launch_bounds__(256) void p2_kernel(StgdArgs a, const TY* __restric
     const int c = c4 + e;
     const float phh = red_get(red, 0, r, c) + red_get(red, 1, r, c);
     const float t = red_get(red, 2, r, c) + red_get(red, 3, r, c);
-    v[e] = p0[e] - eta * a.theta * phh + eta * (t * inv_b + muc * cmu[c]);
+    v[e] = p0[e] - eta * a.theta * phh + eta * (t * inv_b + muc * sm.p2.cmu[c]);
   }
-  finish_tile<KMODE == MM_BF16X3>(S, kp, ti, i0, j0, v, a.phit, phit16, phitT, phitT16, nullptr, nullptr, nullptr, nullptr,
-                                   nullptr);
+  finish_tile<SPLIT>(sm.p2.S, kp, ti, i0, j0, v, a.phit, sp<SPLIT>(P.B.phit16), P.B.phitT, sp<SPLIT>(P.B.phitT16), nullptr,
+                     nullptr, nullptr, nullptr, nullptr);
 }
 
-// ---- Newton-Schulz ---------------------------------------------------------------------------------
-// G = X^T X from X^T, and rowsum[i] += sum over the block's columns of |G_ij|.
+// G = X^T X of Newton-Schulz iteration `it`, and rowsum[i] += sum over the tile's columns of |G_ij|.
 template <int KMODE>
-__global__ __launch_bounds__(256) void ns_gram_kernel(int32_t kp, Opnd<op_t<KMODE>> xT, float* __restrict__ G,
-                                                      uint16_t* __restrict__ G16, float* __restrict__ rowsum) {
-  __shared__ Red4 red;
-  const int ntk = kp / 32;
+__device__ void gram_tile(const StepCtx& P, int it, int tile, StgdSmem& sm) {
+  constexpr bool SPLIT = KMODE == MM_BF16X3;
+  const int kp = P.a.kp, ntk = kp / 32;
+  const int64_t nel = (int64_t)kp * kp;
   const int w = threadIdx.x >> 6;
-  const int ti = blockIdx.x / ntk, tj = blockIdx.x % ntk;
+  const int ti = tile / ntk, tj = tile % ntk;
   const int64_t i0 = (int64_t)ti * 32, j0 = (int64_t)tj * 32;
+  const NsBufs nb = ns_bufs(P.B, P.cur, it, P.ns_iters);
+  float* rowsum = P.B.rowsum + (it & 1) * kp;
+  Red4& red = sm.p2.red;
   floatx16 acc = {};
   int64_t k0, kl;
+  const Opnd<op_t<KMODE>> xT = pick<KMODE>(nb.xT, nb.xT16, nel, kp);
   kslice<KMODE>(kp, 4, w, k0, kl);
   wave_mm<KMODE, true>(acc, xT, xT, i0, j0, k0, kl);
   red_put(red, acc);
@@ -451,63 +541,197 @@ __global__ __launch_bounds__(256) void ns_gram_kernel(int32_t kp, Opnd<op_t<KMOD
     g[e] = red_sum4(red, r, c4 + e);
     s += fabsf(g[e]);
   }
-  put4<KMODE == MM_BF16X3>(G, G16, (int64_t)kp * kp, (i0 + r) * kp + j0 + c4, g);
+  put4<SPLIT>(P.B.gt, sp<SPLIT>(P.B.gt16), nel, (i0 + r) * kp + j0 + c4, g);
   for (int off = 1; off < 8; off <<= 1) s += __shfl_xor(s, off);
   if ((threadIdx.x & 7) == 0) atomicAdd(rowsum + i0 + r, s);
 }
 
-// Xout = c (1.5 X - 0.5 c^2 X G); the outputs are those of finish_tile.
+// Xout = c (1.5 X - 0.5 c^2 X G); the outputs are those of finish_tile (the last iteration also
+// logs ||Phi_next - Phi||^2 and the partial sums of m_A, m_B).
 template <int KMODE>
-__global__ __launch_bounds__(256) void ns_update_kernel(int32_t kp, Opnd<op_t<KMODE>> x_k, Opnd<op_t<KMODE>> g_k,
-                                                        const float* __restrict__ X, const float* __restrict__ rowsum,
-                                                        float* __restrict__ rowsum_next, float* Xout, uint16_t* Xout16,
-                                                        float* XoutT, uint16_t* XoutT16, const float* prev, double* frob,
-                                                        const double* mu_a, const double* mu_b, float* mpart) {
-  __shared__ Red4 red;
-  __shared__ TileShm S;
-  __shared__ float cscale;
-  const int ntk = kp / 32;
+__device__ void update_tile(const StepCtx& P, int it, int tile, StgdSmem& sm) {
+  constexpr bool SPLIT = KMODE == MM_BF16X3;
+  const int kp = P.a.kp, ntk = kp / 32;
+  const int64_t nel = (int64_t)kp * kp;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  const int ti = blockIdx.x / ntk, tj = blockIdx.x % ntk;
+  const int ti = tile / ntk, tj = tile % ntk;
   const int64_t i0 = (int64_t)ti * 32, j0 = (int64_t)tj * 32;
+  const NsBufs nb = ns_bufs(P.B, P.cur, it, P.ns_iters);
+  const bool last = it == P.ns_iters - 1;
+  const float* rowsum = P.B.rowsum + (it & 1) * kp;
+  float* rowsum_next = P.B.rowsum + ((it + 1) & 1) * kp;
+  Red4& red = sm.p2.red;
   floatx16 acc = {};
   int64_t k0, kl;
   kslice<KMODE>(kp, 4, w, k0, kl);
-  wave_mm<KMODE, true>(acc, x_k, g_k, i0, j0, k0, kl);  // X G (G symmetric)
+  wave_mm<KMODE, true>(acc, pick<KMODE>(nb.x, nb.x16, nel, kp), pick<KMODE>(P.B.gt, P.B.gt16, nel, kp), i0, j0, k0,
+                       kl);  // X G (G symmetric)
   if (w == 3) {  // ||G||_inf from the row sums of |G|
     float mx = 0.f;
     for (int t = lane; t < kp; t += 64) mx = fmaxf(mx, rowsum[t]);
     for (int off = 32; off > 0; off >>= 1) mx = fmaxf(mx, __shfl_xor(mx, off));
-    if (lane == 0) cscale = mx > 2.f ? sqrtf(2.f / mx) : 1.f;
+    if (lane == 0) sm.p2.cscale = mx > 2.f ? sqrtf(2.f / mx) : 1.f;
   }
-  if (blockIdx.x == 0)
+  if (tile == 0)
     for (int t = threadIdx.x; t < kp; t += blockDim.x) rowsum_next[t] = 0.f;
   red_put(red, acc);
   __syncthreads();
-  const float c1 = 1.5f * cscale, c3 = 0.5f * cscale * cscale * cscale;
+  const float cs = sm.p2.cscale;
+  const float c1 = 1.5f * cs, c3 = 0.5f * cs * cs * cs;
   const int r = threadIdx.x >> 3, c4 = 4 * (threadIdx.x & 7);
-  const float4 xv = *reinterpret_cast<const float4*>(X + (i0 + r) * kp + j0 + c4);
+  const float4 xv = *reinterpret_cast<const float4*>(nb.x + (i0 + r) * kp + j0 + c4);
   const float x[4] = {xv.x, xv.y, xv.z, xv.w};
   float v[4];
 #pragma unroll
   for (int e = 0; e < 4; ++e) v[e] = c1 * x[e] - c3 * red_sum4(red, r, c4 + e);
-  finish_tile<KMODE == MM_BF16X3>(S, kp, ti, i0, j0, v, Xout, Xout16, XoutT, XoutT16, prev, frob, mu_a, mu_b, mpart);
+  finish_tile<SPLIT>(sm.p2.S, kp, ti, i0, j0, v, nb.dst, sp<SPLIT>(nb.dst16), nb.dstT, sp<SPLIT>(nb.dstT16),
+                     last ? P.a.phi : nullptr, P.frob, P.a.mu_a, P.a.mu_b, last ? P.B.mpart : nullptr);
 }
 
-// Steps without renormalisation: Phi_next = Phi~, with the outputs of the last NS update. With
-// prev == nullptr and Xout == nullptr it is the run set-up: the derived copies of the current Phi.
+// COPY: Phi_next = Phi~ with the outputs of a last update. SETUP: the derived copies of the
+// current Phi (split pairs, Phi^T, the m partial sums), and tile 0 clears the run's frob entries.
 template <bool SPLIT>
-__global__ __launch_bounds__(256) void copy_tile_kernel(int32_t kp, const float* __restrict__ X, float* Xout, uint16_t* Xout16,
-                                                        float* XoutT, uint16_t* XoutT16, const float* prev, double* frob,
-                                                        const double* mu_a, const double* mu_b, float* mpart) {
-  __shared__ TileShm S;
-  const int ntk = kp / 32;
-  const int ti = blockIdx.x / ntk, tj = blockIdx.x % ntk;
+__device__ void copy_tile(const StepCtx& P, bool setup, int tile, StgdSmem& sm) {
+  const int kp = P.a.kp, ntk = kp / 32;
+  const int ti = tile / ntk, tj = tile % ntk;
   const int64_t i0 = (int64_t)ti * 32, j0 = (int64_t)tj * 32;
   const int r = threadIdx.x >> 3, c4 = 4 * (threadIdx.x & 7);
+  const float* X = setup ? P.B.phi[P.cur] : P.B.phit;
   const float4 xv = *reinterpret_cast<const float4*>(X + (i0 + r) * kp + j0 + c4);
   const float v[4] = {xv.x, xv.y, xv.z, xv.w};
-  finish_tile<SPLIT>(S, kp, ti, i0, j0, v, Xout, Xout16, XoutT, XoutT16, prev, frob, mu_a, mu_b, mpart);
+  if (setup && tile == 0)
+    for (int64_t t = threadIdx.x; t < P.nfrob; t += blockDim.x) P.frob_zero[t] = 0.0;
+  if (setup)
+    finish_tile<SPLIT>(sm.S, kp, ti, i0, j0, v, nullptr, sp<SPLIT>(P.B.phi16[P.cur]), P.B.phiT, sp<SPLIT>(P.B.phiT16),
+                       nullptr, nullptr, P.a.mu_a, P.a.mu_b, P.B.mpart);
+  else
+    finish_tile<SPLIT>(sm.S, kp, ti, i0, j0, v, P.B.phi[P.cur ^ 1], sp<SPLIT>(P.B.phi16[P.cur ^ 1]), P.B.phiT,
+                       sp<SPLIT>(P.B.phiT16), P.a.phi, P.frob, P.a.mu_a, P.a.mu_b, P.B.mpart);
+}
+
+template <int DMODE, int KMODE, typename TY, typename TPH>
+__device__ void run_tile(const StepCtx& P, int phase, int it, int tile, StgdSmem& sm) {
+  constexpr bool SPLIT = KMODE == MM_BF16X3;
+  switch (phase) {
+    case PH_SETUP: copy_tile<SPLIT>(P, true, tile, sm); break;
+    case PH_P1: p1_tile<DMODE, KMODE, TY, TPH>(P, tile, sm); break;
+    case PH_P2: p2_tile<DMODE, KMODE, TY, TPH>(P, tile, sm); break;
+    case PH_GRAM: gram_tile<KMODE>(P, it, tile, sm); break;
+    case PH_UPDATE: update_tile<KMODE>(P, it, tile, sm); break;
+    default: copy_tile<SPLIT>(P, false, tile, sm); break;
+  }
+}
+
+// One phase of one step: blockIdx.x = tile.
+template <int DMODE, int KMODE, typename TY, typename TPH>
+__global__ __launch_bounds__(256) void stgd_phase_kernel(StepCtx P, int32_t phase, int32_t it) {
+  __shared__ StgdSmem sm;
+  run_tile<DMODE, KMODE, TY, TPH>(P, phase, it, blockIdx.x, sm);
+}
+
+// ---- many instances in one persistent launch -------------------------------------------------------
+// The host lays the phases of every instance's steps out as segments of a global ticket sequence,
+// slot by slot (the s-th phase of every instance, then the (s+1)-th, ...), so that each segment
+// comes after every segment it depends on. A workgroup takes the next ticket, waits until its
+// instance has completed all tiles of its earlier segments (done[inst] >= pre), runs the tile and
+// publishes it. Tickets are taken in order by running workgroups only, so the smallest unfinished
+// ticket never waits: the launch cannot deadlock whatever the number of resident workgroups. A
+// wait that exceeds spin_limit polls sets *err, and every later tile is skipped but still counted,
+// so the grid always drains.
+struct BatchSeg {
+  int32_t inst;
+  int32_t phase;
+  int32_t it;
+  int32_t ntiles;
+  int64_t step;     // step index within the launch
+  int64_t tstart;   // first ticket
+  uint32_t pre;     // tiles of the instance in earlier segments
+};
+
+struct BatchInst {
+  StgdArgs a;
+  StgdBuffers B;
+  const void* YA;
+  const void* YB;
+  const void* YCt;
+  int32_t cur0;       // phi[cur0] holds Phi at the start of the launch
+  int32_t ns_iters;
+  int64_t step0;      // global index of the launch's first step
+  int64_t spe;        // steps per epoch
+  double* frob_log;   // [steps of the launch]
+  int64_t nfrob;
+};
+
+__device__ inline void make_step(const BatchInst& I, int64_t st, StepCtx& P) {
+  P.a = I.a;
+  P.B = I.B;
+  P.YA = I.YA;
+  P.YB = I.YB;
+  P.YCt = I.YCt;
+  P.cur = I.cur0 ^ (int)(st & 1);
+  P.ns_iters = I.ns_iters;
+  const int64_t g = I.step0 + st;
+  P.row0 = (g % I.spe) * I.a.bl;
+  P.nb = (int32_t)min((int64_t)I.a.bl, I.a.n_x - P.row0);
+  P.a.phi = I.B.phi[P.cur];
+  P.a.phit = I.B.phit;
+  P.frob = I.frob_log + st;
+  P.frob_zero = I.frob_log;
+  P.nfrob = I.nfrob;
+}
+
+template <int DMODE, int KMODE, typename TY, typename TPH>
+__global__ __launch_bounds__(256) void stgd_batch_kernel(const BatchInst* insts, const BatchSeg* segs, int32_t nsegs,
+                                                         int64_t ntickets, unsigned long long* ticket, uint32_t* done,
+                                                         int32_t* err, int64_t spin_limit) {
+  __shared__ StgdSmem sm;
+  __shared__ StepCtx P;
+  __shared__ int64_t T;
+  __shared__ int32_t sidx, skip;
+  for (;;) {
+    if (threadIdx.x == 0) {
+      T = (int64_t)atomicAdd(ticket, 1ull);
+      skip = 0;
+      if (T < ntickets) {
+        int lo = 0, hi = nsegs - 1;
+        while (lo < hi) {
+          const int mid = (lo + hi + 1) >> 1;
+          if (segs[mid].tstart <= T) lo = mid;
+          else hi = mid - 1;
+        }
+        sidx = lo;
+        const BatchSeg& s = segs[lo];
+        int64_t spins = 0;
+        while (__hip_atomic_load(done + s.inst, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < s.pre) {
+          if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) || ++spins > spin_limit) {
+            atomicExch(err, 1);
+            skip = 1;
+            break;
+          }
+          __builtin_amdgcn_s_sleep(2);
+        }
+        __threadfence();
+        make_step(insts[s.inst], s.step, P);
+      }
+    }
+    __syncthreads();
+    if (T >= ntickets) break;
+    __threadfence();  // acquire: this wave sees every tile published before the wait ended
+    const BatchSeg& s = segs[sidx];
+    if (!skip) run_tile<DMODE, KMODE, TY, TPH>(P, s.phase, s.it, (int)(T - s.tstart), sm);
+    __threadfence();  // release: this tile's stores before the count
+    __syncthreads();
+    if (threadIdx.x == 0) atomicAdd(done + s.inst, 1u);
+  }
+}
+
+__global__ void batch_reset_kernel(int32_t ninst, unsigned long long* ticket, uint32_t* done, int32_t* err) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t == 0) {
+    *ticket = 0ull;
+    *err = 0;
+  }
+  if (t < ninst) done[t] = 0u;
 }
 
 __global__ void zero_f64_kernel(int64_t n, double* p) {
@@ -529,95 +753,54 @@ int f32_to_bf16(hipStream_t s, int64_t n, const float* in, bf16_bits* out) {
 }
 
 // ---- host driver -----------------------------------------------------------------------------------
-// The operand view of a kp x kp matrix for one mode: FP32 data, or the BF16 (hi, lo) pair.
-template <int MODE>
-static Opnd<op_t<MODE>> pick(const float* f32, const uint16_t* s16, int64_t nel, int64_t ld) {
-  if constexpr (MODE == MM_F32) {
-    (void)s16;
-    (void)nel;
-    return opnd<float>(f32, nullptr, ld);
-  } else {
-    (void)f32;
-    const __bf16* p = reinterpret_cast<const __bf16*>(s16);
-    return opnd<__bf16>(p, p + nel, ld);
-  }
+static bool step_renorms(int64_t gstep, int32_t renorm_every, int32_t ns_iters) {
+  return renorm_every > 0 && ((gstep + 1) % renorm_every == 0) && ns_iters > 0;
 }
 
 template <int DMODE, int KMODE, typename TY, typename TPH>
-static int run_steps(hipStream_t s, const StgdArgs& args, const TY* YA, const TY* YB, const TY* YCt, int64_t step0,
+static int run_steps(hipStream_t s, const StgdArgs& args, const void* YA, const void* YB, const void* YCt, int64_t step0,
                      int64_t nsteps, int32_t renorm_every, int32_t ns_iters, const StgdBuffers& B, int& cur,
                      double* frob_log, StgdTiming* timing) {
-  constexpr bool SPLIT = KMODE == MM_BF16X3;
-  constexpr int VEC = 16 / sizeof(TY);  // Y_C^T elements per 16-byte load
-  const int kp = args.kp, ntk = kp / 32, ntb = args.b / 32;
-  const int64_t nel = (int64_t)kp * kp;
+  const int kp = args.kp;
   const int64_t spe = ceil_div(args.n_x, args.bl);
-  TPH* cgt = static_cast<TPH*>(B.cgt);
-  auto s16 = [&](uint16_t* p) -> uint16_t* { return SPLIT ? p : nullptr; };
-  hipLaunchKernelGGL(zero_f64_kernel, dim3((unsigned)ceil_div(nsteps, 256)), dim3(256), 0, s, nsteps, frob_log);
-  MMSB_LAUNCH_CHECK();
-  hipLaunchKernelGGL(copy_tile_kernel<SPLIT>, dim3(ntk * ntk), dim3(256), 0, s, kp, B.phi[cur], nullptr, s16(B.phi16[cur]),
-                     B.phiT, s16(B.phiT16), nullptr, nullptr, args.mu_a, args.mu_b, B.mpart);
-  MMSB_LAUNCH_CHECK();
-  StgdArgs a = args;
+  StepCtx P{};
+  P.a = args;
+  P.B = B;
+  P.YA = YA;
+  P.YB = YB;
+  P.YCt = YCt;
+  P.ns_iters = ns_iters;
+  auto launch = [&](int phase, int it) -> int {
+    hipLaunchKernelGGL((stgd_phase_kernel<DMODE, KMODE, TY, TPH>), dim3(phase_tiles(phase, kp, args.b)), dim3(256), 0, s, P,
+                       phase, it);
+    MMSB_LAUNCH_CHECK();
+    return MMSB_OK;
+  };
+  P.cur = cur;
+  P.a.phi = B.phi[cur];
+  P.a.phit = B.phit;
+  P.frob_zero = frob_log;
+  P.nfrob = nsteps;
+  MMSB_TRY(launch(PH_SETUP, 0));
   for (int64_t st = 0; st < nsteps; ++st) {
     const int64_t gstep = step0 + st;
-    const int64_t row0 = (gstep % spe) * a.bl;
-    const int32_t nb = (int32_t)std::min<int64_t>(a.bl, a.n_x - row0);
-    a.phi = B.phi[cur];
-    a.phit = B.phit;
+    P.cur = cur;
+    P.row0 = (gstep % spe) * args.bl;
+    P.nb = (int32_t)std::min<int64_t>(args.bl, args.n_x - P.row0);
+    P.a.phi = B.phi[cur];
+    P.frob = frob_log + st;
     if (timing) MMSB_HIP(hipEventRecord(timing->ev[0], s));
-    // the data contraction reads Phi^T as BF16 hi only; the k^3 products read the split pair
-    hipLaunchKernelGGL((p1_kernel<DMODE, KMODE, TY, TPH>), dim3(ntb * ntk + ntk * ntk), dim3(256), 0, s, a, YA, YB,
-                       pick<DMODE>(B.phiT, B.phiT16, nel, kp), pick<KMODE>(B.phiT, B.phiT16, nel, kp), B.mpart, cgt, B.h,
-                       s16(B.h16), row0, nb);
-    MMSB_LAUNCH_CHECK();
+    MMSB_TRY(launch(PH_P1, 0));
     if (timing) MMSB_HIP(hipEventRecord(timing->ev[1], s));
-    const Opnd<op_t<KMODE>> phi_k = pick<KMODE>(B.phi[cur], B.phi16[cur], nel, kp);
-    const Opnd<op_t<KMODE>> h_k = pick<KMODE>(B.h, B.h16, nel, kp);
-    if (row0 % VEC == 0)
-      hipLaunchKernelGGL((p2_kernel<DMODE, KMODE, true, TY, TPH>), dim3(ntk * ntk), dim3(256), 0, s, a, YCt, cgt, phi_k, h_k,
-                         s16(B.phit16), B.phitT, s16(B.phitT16), B.rowsum, row0, nb);
-    else
-      hipLaunchKernelGGL((p2_kernel<DMODE, KMODE, false, TY, TPH>), dim3(ntk * ntk), dim3(256), 0, s, a, YCt, cgt, phi_k, h_k,
-                         s16(B.phit16), B.phitT, s16(B.phitT16), B.rowsum, row0, nb);
-    MMSB_LAUNCH_CHECK();
+    MMSB_TRY(launch(PH_P2, 0));
     if (timing) MMSB_HIP(hipEventRecord(timing->ev[2], s));
-    float* out = B.phi[cur ^ 1];
-    uint16_t* out16 = s16(B.phi16[cur ^ 1]);
-    double* frob = frob_log + st;
-    const bool renorm = renorm_every > 0 && ((gstep + 1) % renorm_every == 0) && ns_iters > 0;
-    if (renorm) {
-      // iterates: (phit, phitT) -> (phia, phiaT) -> (phit, phitT) ...; the last writes Phi_next
-      float* x = B.phit;
-      float* xT = B.phitT;
-      uint16_t* x16 = B.phit16;
-      uint16_t* xT16 = B.phitT16;
+    if (step_renorms(gstep, renorm_every, ns_iters)) {
       for (int it = 0; it < ns_iters; ++it) {
-        const bool last = it == ns_iters - 1;
-        const bool to_a = x == B.phit;
-        float* dst = last ? out : (to_a ? B.phia : B.phit);
-        uint16_t* dst16 = last ? B.phi16[cur ^ 1] : (to_a ? B.phia16 : B.phit16);
-        float* dstT = last ? B.phiT : (to_a ? B.phiaT : B.phitT);
-        uint16_t* dstT16 = last ? B.phiT16 : (to_a ? B.phiaT16 : B.phitT16);
-        float* rs = B.rowsum + (it & 1) * kp;
-        float* rs_next = B.rowsum + ((it + 1) & 1) * kp;
-        hipLaunchKernelGGL((ns_gram_kernel<KMODE>), dim3(ntk * ntk), dim3(256), 0, s, kp, pick<KMODE>(xT, xT16, nel, kp), B.gt,
-                           s16(B.gt16), rs);
-        MMSB_LAUNCH_CHECK();
-        hipLaunchKernelGGL((ns_update_kernel<KMODE>), dim3(ntk * ntk), dim3(256), 0, s, kp, pick<KMODE>(x, x16, nel, kp),
-                           pick<KMODE>(B.gt, B.gt16, nel, kp), x, rs, rs_next, dst, s16(dst16), dstT, s16(dstT16),
-                           last ? a.phi : nullptr, frob, a.mu_a, a.mu_b, last ? B.mpart : nullptr);
-        MMSB_LAUNCH_CHECK();
-        x = dst;
-        xT = dstT;
-        x16 = dst16;
-        xT16 = dstT16;
+        MMSB_TRY(launch(PH_GRAM, it));
+        MMSB_TRY(launch(PH_UPDATE, it));
       }
     } else {
-      hipLaunchKernelGGL(copy_tile_kernel<SPLIT>, dim3(ntk * ntk), dim3(256), 0, s, kp, B.phit, out, out16, B.phiT,
-                         s16(B.phiT16), a.phi, frob, a.mu_a, a.mu_b, B.mpart);
-      MMSB_LAUNCH_CHECK();
+      MMSB_TRY(launch(PH_COPY, 0));
     }
     if (timing) {
       MMSB_HIP(hipEventRecord(timing->ev[3], s));
@@ -633,29 +816,186 @@ static int run_steps(hipStream_t s, const StgdArgs& args, const TY* YA, const TY
   return MMSB_OK;
 }
 
-int stgd_run(hipStream_t s, int32_t precision, const StgdArgs& args, const void* YA, const void* YB, const void* YCt,
-             int64_t step0, int64_t nsteps, int32_t renorm_every, int32_t ns_iters, const StgdBuffers& bufs, int& cur,
-             double* frob_log, StgdTiming* timing) {
+static int check_args(const StgdArgs& args, int32_t ns_iters) {
   MMSB_CHECK_ARG(args.kp % 64 == 0 && args.b % 64 == 0 && args.kp >= 64 && args.b >= 64,
                  "stgd: kp and b must be positive multiples of 64 (got %d, %d)", args.kp, args.b);
   MMSB_CHECK_ARG(args.bl >= 1 && args.bl <= args.b, "stgd: logical minibatch %d outside [1, %d]", args.bl, args.b);
   MMSB_CHECK_ARG(args.n_pad % 64 == 0, "stgd: n_pad must be a multiple of 64");
   const int64_t spe = ceil_div(args.n_x, args.bl);
   MMSB_CHECK_ARG(args.n_pad >= (spe - 1) * args.bl + args.b, "stgd: n_pad too small");
-  MMSB_CHECK_ARG(nsteps >= 0 && ns_iters >= 0, "stgd: negative step or iteration count");
+  MMSB_CHECK_ARG(ns_iters >= 0, "stgd: negative iteration count");
+  return MMSB_OK;
+}
+
+static int check_split(const StgdBuffers& bufs) {
+  MMSB_CHECK_ARG(bufs.phi16[0] && bufs.phi16[1] && bufs.phit16 && bufs.phia16 && bufs.phiT16 && bufs.phitT16 &&
+                     bufs.phiaT16 && bufs.gt16 && bufs.h16,
+                 "stgd: BF16 mode needs the split operand buffers");
+  return MMSB_OK;
+}
+
+int stgd_run(hipStream_t s, int32_t precision, const StgdArgs& args, const void* YA, const void* YB, const void* YCt,
+             int64_t step0, int64_t nsteps, int32_t renorm_every, int32_t ns_iters, const StgdBuffers& bufs, int& cur,
+             double* frob_log, StgdTiming* timing) {
+  MMSB_TRY(check_args(args, ns_iters));
+  MMSB_CHECK_ARG(nsteps >= 0, "stgd: negative step count");
   if (nsteps == 0) return MMSB_OK;
   if (precision == MMSB_PREC_BF16) {
-    MMSB_CHECK_ARG(bufs.phi16[0] && bufs.phi16[1] && bufs.phit16 && bufs.phia16 && bufs.phiT16 && bufs.phitT16 &&
-                       bufs.phiaT16 && bufs.gt16 && bufs.h16,
-                   "stgd: BF16 mode needs the split operand buffers");
-    return run_steps<MM_BF16, MM_BF16X3, __bf16, __bf16>(s, args, static_cast<const __bf16*>(YA), static_cast<const __bf16*>(YB),
-                                                         static_cast<const __bf16*>(YCt), step0, nsteps, renorm_every, ns_iters,
-                                                         bufs, cur, frob_log, timing);
+    MMSB_TRY(check_split(bufs));
+    return run_steps<MM_BF16, MM_BF16X3, __bf16, __bf16>(s, args, YA, YB, YCt, step0, nsteps, renorm_every, ns_iters, bufs,
+                                                         cur, frob_log, timing);
   }
   MMSB_CHECK_ARG(precision == MMSB_PREC_FP32, "stgd: unknown precision %d", precision);
-  return run_steps<MM_F32, MM_F32, float, float>(s, args, static_cast<const float*>(YA), static_cast<const float*>(YB),
-                                                 static_cast<const float*>(YCt), step0, nsteps, renorm_every, ns_iters, bufs,
-                                                 cur, frob_log, timing);
+  return run_steps<MM_F32, MM_F32, float, float>(s, args, YA, YB, YCt, step0, nsteps, renorm_every, ns_iters, bufs, cur,
+                                                 frob_log, timing);
+}
+
+// ---- batch ------------------------------------------------------------------------------------------
+struct StgdBatch {
+  int32_t precision = MMSB_PREC_FP32;
+  int32_t ninst = 0;
+  int32_t nsegs = 0;
+  int64_t ntickets = 0;
+  int32_t grid = 0;
+  BatchInst* d_inst = nullptr;   // [ninst]
+  BatchSeg* d_segs = nullptr;    // [nsegs]
+  unsigned long long* d_ticket = nullptr;
+  uint32_t* d_done = nullptr;
+  int32_t* d_err = nullptr;
+};
+
+int stgd_batch_plan(StgdBatch*& out, int32_t precision, const std::vector<StgdBatchMember>& members, int32_t ns_iters,
+                    int32_t renorm_every) {
+  out = nullptr;
+  MMSB_CHECK_ARG(!members.empty(), "stgd batch: no instances");
+  MMSB_CHECK_ARG(precision == MMSB_PREC_FP32 || precision == MMSB_PREC_BF16, "stgd batch: unknown precision %d", precision);
+  std::vector<BatchInst> insts(members.size());
+  std::vector<std::vector<BatchSeg>> per(members.size());
+  for (size_t i = 0; i < members.size(); ++i) {
+    const StgdBatchMember& m = members[i];
+    MMSB_TRY(check_args(m.args, ns_iters));
+    if (precision == MMSB_PREC_BF16) MMSB_TRY(check_split(m.bufs));
+    MMSB_CHECK_ARG(m.nsteps >= 1, "stgd batch: instance %zu has no steps", i);
+    BatchInst& I = insts[i];
+    I.a = m.args;
+    I.B = m.bufs;
+    I.YA = m.YA;
+    I.YB = m.YB;
+    I.YCt = m.YCt;
+    I.cur0 = m.cur;
+    I.ns_iters = ns_iters;
+    I.step0 = m.step0;
+    I.spe = ceil_div(m.args.n_x, m.args.bl);
+    I.frob_log = m.frob_log;
+    I.nfrob = m.nsteps;
+    auto add = [&](int phase, int it, int64_t st) {
+      BatchSeg sg{};
+      sg.inst = (int32_t)i;
+      sg.phase = phase;
+      sg.it = it;
+      sg.ntiles = phase_tiles(phase, m.args.kp, m.args.b);
+      sg.step = st;
+      per[i].push_back(sg);
+    };
+    add(PH_SETUP, 0, 0);
+    for (int64_t st = 0; st < m.nsteps; ++st) {
+      add(PH_P1, 0, st);
+      add(PH_P2, 0, st);
+      if (step_renorms(m.step0 + st, renorm_every, ns_iters)) {
+        for (int it = 0; it < ns_iters; ++it) {
+          add(PH_GRAM, it, st);
+          add(PH_UPDATE, it, st);
+        }
+      } else {
+        add(PH_COPY, 0, st);
+      }
+    }
+  }
+  // slot-major interleaving: the s-th segment of every instance, then the (s+1)-th
+  std::vector<BatchSeg> segs;
+  std::vector<uint32_t> pre(members.size(), 0);
+  size_t maxlen = 0;
+  for (auto& v : per) maxlen = std::max(maxlen, v.size());
+  int64_t t = 0;
+  for (size_t sl = 0; sl < maxlen; ++sl)
+    for (size_t i = 0; i < per.size(); ++i) {
+      if (sl >= per[i].size()) continue;
+      BatchSeg sg = per[i][sl];
+      sg.tstart = t;
+      sg.pre = pre[i];
+      pre[i] += (uint32_t)sg.ntiles;
+      t += sg.ntiles;
+      segs.push_back(sg);
+    }
+  MMSB_CHECK_ARG(segs.size() < (size_t)INT32_MAX, "stgd batch: too many segments");
+  auto* b = new StgdBatch();
+  b->precision = precision;
+  b->ninst = (int32_t)insts.size();
+  b->nsegs = (int32_t)segs.size();
+  b->ntickets = t;
+  auto fail = [&](hipError_t e) {
+    stgd_batch_free(b);
+    return set_error(MMSB_ERR_HIP, "stgd batch: %s", hipGetErrorString(e));
+  };
+  hipError_t e;
+  if ((e = hipMalloc(&b->d_inst, sizeof(BatchInst) * insts.size())) != hipSuccess) return fail(e);
+  if ((e = hipMalloc(&b->d_segs, sizeof(BatchSeg) * segs.size())) != hipSuccess) return fail(e);
+  if ((e = hipMalloc(&b->d_ticket, sizeof(unsigned long long))) != hipSuccess) return fail(e);
+  if ((e = hipMalloc(&b->d_done, sizeof(uint32_t) * insts.size())) != hipSuccess) return fail(e);
+  if ((e = hipMalloc(&b->d_err, sizeof(int32_t))) != hipSuccess) return fail(e);
+  if ((e = hipMemcpy(b->d_inst, insts.data(), sizeof(BatchInst) * insts.size(), hipMemcpyHostToDevice)) != hipSuccess)
+    return fail(e);
+  if ((e = hipMemcpy(b->d_segs, segs.data(), sizeof(BatchSeg) * segs.size(), hipMemcpyHostToDevice)) != hipSuccess)
+    return fail(e);
+  const void* fn = precision == MMSB_PREC_BF16
+                       ? reinterpret_cast<const void*>(stgd_batch_kernel<MM_BF16, MM_BF16X3, __bf16, __bf16>)
+                       : reinterpret_cast<const void*>(stgd_batch_kernel<MM_F32, MM_F32, float, float>);
+  int dev = 0, per_cu = 0, cus = 0;
+  if ((e = hipGetDevice(&dev)) != hipSuccess) return fail(e);
+  if ((e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, 256, 0)) != hipSuccess) return fail(e);
+  if ((e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev)) != hipSuccess) return fail(e);
+  b->grid = std::max(1, per_cu) * std::max(1, cus);
+  out = b;
+  return MMSB_OK;
+}
+
+void stgd_batch_set_grid(StgdBatch* b, int32_t grid) {
+  if (b && grid > 0) b->grid = grid;
+}
+
+int32_t stgd_batch_grid(const StgdBatch* b) { return b ? b->grid : 0; }
+
+int stgd_batch_launch(hipStream_t s, StgdBatch* b) {
+  MMSB_CHECK_ARG(b, "stgd batch: null plan");
+  hipLaunchKernelGGL(batch_reset_kernel, dim3((unsigned)ceil_div(b->ninst + 1, 256)), dim3(256), 0, s, b->ninst, b->d_ticket,
+                     b->d_done, b->d_err);
+  MMSB_LAUNCH_CHECK();
+  const int64_t spin_limit = 1ll << 22;
+  if (b->precision == MMSB_PREC_BF16)
+    hipLaunchKernelGGL((stgd_batch_kernel<MM_BF16, MM_BF16X3, __bf16, __bf16>), dim3(b->grid), dim3(256), 0, s, b->d_inst,
+                       b->d_segs, b->nsegs, b->ntickets, b->d_ticket, b->d_done, b->d_err, spin_limit);
+  else
+    hipLaunchKernelGGL((stgd_batch_kernel<MM_F32, MM_F32, float, float>), dim3(b->grid), dim3(256), 0, s, b->d_inst,
+                       b->d_segs, b->nsegs, b->ntickets, b->d_ticket, b->d_done, b->d_err, spin_limit);
+  MMSB_LAUNCH_CHECK();
+  return MMSB_OK;
+}
+
+int stgd_batch_error(hipStream_t s, StgdBatch* b, int32_t* err) {
+  MMSB_CHECK_ARG(b && err, "stgd batch: null argument");
+  MMSB_HIP(hipMemcpyAsync(err, b->d_err, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  MMSB_HIP(hipStreamSynchronize(s));
+  return MMSB_OK;
+}
+
+void stgd_batch_free(StgdBatch* b) {
+  if (!b) return;
+  (void)hipFree(b->d_inst);
+  (void)hipFree(b->d_segs);
+  (void)hipFree(b->d_ticket);
+  (void)hipFree(b->d_done);
+  (void)hipFree(b->d_err);
+  delete b;
 }
 
 }  // namespace mmsb

@@ -28,9 +28,12 @@ def make_case(mmsb, n=300, k=4, alpha0=1.0, p_in=0.5, p_out=0.02, seed=0, power_
     return g, cfg, bl, wcfg
 
 
-@pytest.mark.parametrize("k,n,scaling", [(4, 300, "paper"), (10, 1000, "paper"), (6, 400, "gpu")])
+# (60, 6000): partitions of >= 4096 rows with r, k >= 32 take the FP64-MFMA tall products, and r >= 64
+# takes the block Jacobi kernel
+@pytest.mark.parametrize("k,n,scaling", [(4, 300, "paper"), (10, 1000, "paper"), (6, 400, "gpu"), (60, 6000, "gpu")])
 def test_whitening_matches_oracle(mmsb, k, n, scaling):
-    g, cfg, bl, wcfg = make_case(mmsb, n=n, k=k, m2_scaling=scaling, p_in=0.6 if k > 4 else 0.5)
+    g, cfg, bl, wcfg = make_case(mmsb, n=n, k=k, m2_scaling=scaling, p_in=0.6 if k > 4 else 0.5,
+                                 p_out=0.02 if n < 4096 else 0.004)
     print(f"case k={k} n={n} scaling={scaling}", flush=True)
     p = mmsb.Pipeline(cfg)
     p.set_graph(g)
@@ -287,4 +290,38 @@ def test_timed_steps_match_untimed(mmsb, prec):
     assert all(t > 0 for t in ms), ms
     assert np.array_equal(ps[0].get("phi"), ps[1].get("phi"))
     for p in ps:
+        p.close()
+
+
+@pytest.mark.parametrize("prec", ["fp32", "bf16"])
+def test_batch_matches_single_instances(mmsb, prec):
+    """Uneven instances (n, k, minibatch, alpha0, eta all differ; odd and even steps per epoch)
+    advanced together by the persistent batch launch give the same Phi, bit for bit, as each
+    instance run alone, and the same ||dPhi||^2 log; also after autotuning the grid."""
+    specs = [(700, 8, 64, 0.7, 2e-3), (1300, 24, 128, 1.5, 1e-3), (520, 70, 64, 2.5, 3e-3), (2000, 10, 200, 1.0, 2e-3)]
+    singles, members = [], []
+    for idx, (n, k, b, a0, eta) in enumerate(specs):
+        rng = np.random.default_rng(idx)
+        ys = [rng.standard_normal((n, k)) for _ in range(3)]
+        mus = [0.2 * rng.standard_normal(k) for _ in range(3)]
+        phi0 = core.init_phi(idx + 1, k)
+        for lst in (singles, members):
+            cfg = mmsb.config(nx=n, na=n, nb=n, nc=n, ncom=k, alpha_0=a0, oversample=0, minibatch=b, learn_rate=eta)
+            p = mmsb.Pipeline(cfg)
+            p.set_yvectors(*ys, *mus)
+            p.set_phi(phi0)
+            lst.append(p)
+    print(f"case prec={prec} instances={len(specs)}", flush=True)
+    batch = mmsb.Batch(members, prec)
+    batch.run(2)
+    grid, ms = batch.autotune()
+    assert grid > 0 and ms > 0
+    batch.run(1)
+    for (n, k, b, a0, eta), p_single, p_batch in zip(specs, singles, members):
+        spe = -(-n // b)
+        p_single.stgd(3 * spe, prec)
+        assert np.array_equal(p_single.get("phi"), p_batch.get("phi")), (n, k, b)
+        np.testing.assert_allclose(p_batch.get("frob"), p_single.get("frob"), rtol=1e-9)
+    batch.close()
+    for p in singles + members:
         p.close()
