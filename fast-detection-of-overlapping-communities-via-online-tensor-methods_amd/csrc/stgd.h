@@ -86,17 +86,18 @@ struct StgdBatchMember {
 };
 
 struct StgdBatch;
-// Lays out the tiles of every member's steps as one ticket sequence and uploads it. The plan is
-// fixed: every launch replays the same steps (the members' cur must match at each launch).
+// Lays out the tiles of every member's steps slot by slot and uploads the plan. The plan is fixed:
+// every launch replays the same steps (the members' cur must match at each launch).
 int stgd_batch_plan(StgdBatch*& out, int32_t precision, const std::vector<StgdBatchMember>& members, int32_t ns_iters,
                     int32_t renorm_every);
-// Resets the queue counters (a kernel, so the pair is capturable in a HIP graph) and launches the
-// persistent kernel.
+// One cooperative launch of the persistent kernel over the whole plan.
 int stgd_batch_launch(hipStream_t s, StgdBatch* b);
-// Reads the error flag (non-zero: a dependency wait hit its spin limit; the results are invalid).
-int stgd_batch_error(hipStream_t s, StgdBatch* b, int32_t* err);
+// The same plan as one grouped launch per phase slot (capturable in a HIP graph).
+int stgd_batch_launch_slots(hipStream_t s, StgdBatch* b);
+// Workgroups of the launch (at most every workgroup that can be resident at once).
 void stgd_batch_set_grid(StgdBatch* b, int32_t grid);
 int32_t stgd_batch_grid(const StgdBatch* b);
+int32_t stgd_batch_grid_max(const StgdBatch* b);
 void stgd_batch_free(StgdBatch* b);
 
 // BF16 buffers cross the host boundary as opaque 16-bit words.

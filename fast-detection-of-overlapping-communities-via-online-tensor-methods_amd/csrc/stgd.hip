@@ -28,12 +28,16 @@
 // in FP32. In BF16 mode every k x k operand is written by the kernel that produces it both in
 // FP32 and as its (hi, lo) BF16 pair, so the MFMA loops load ready operands and convert nothing.
 // m_A and m_B of the next step are reduced from the tiles of the new Phi where it is written.
+#include <hip/hip_cooperative_groups.h>
+
 #include <algorithm>
 #include <type_traits>
 #include <vector>
 
 #include "common.h"
 #include "stgd.h"
+
+namespace cg = cooperative_groups;
 
 namespace mmsb {
 
@@ -141,9 +145,22 @@ struct Frag<MM_BF16X3> {
   }
 };
 
-// Ring depth per mode (fragment registers: 32 VGPRs per set for FP32 and BF16, 64 for split BF16).
+// Ring depth per mode (fragment registers: 32 VGPRs per set for FP32 and BF16, 64 for split BF16);
+// deeper rings hide more load latency per wave at the cost of waves per SIMD.
+#ifndef MMSB_RING_F32
+#define MMSB_RING_F32 2
+#endif
+#ifndef MMSB_RING_BF16
+#define MMSB_RING_BF16 2
+#endif
+#ifndef MMSB_RING_BF16X3
+#define MMSB_RING_BF16X3 2
+#endif
+static_assert(MMSB_RING_F32 >= 2 && MMSB_RING_BF16 >= 2 && MMSB_RING_BF16X3 >= 2, "ring depth must be at least 2");
 template <int MODE>
-constexpr int ring_depth() { return MODE == MM_BF16X3 ? 2 : 4; }
+constexpr int ring_depth() {
+  return MODE == MM_BF16X3 ? MMSB_RING_BF16X3 : (MODE == MM_BF16 ? MMSB_RING_BF16 : MMSB_RING_F32);
+}
 
 // The part of a reduction of length K (a multiple of the group) that wave `idx` of `parts` owns.
 template <int MODE>
@@ -622,30 +639,33 @@ __device__ void run_tile(const StepCtx& P, int phase, int it, int tile, StgdSmem
   }
 }
 
-// One phase of one step: blockIdx.x = tile.
-template <int DMODE, int KMODE, typename TY, typename TPH>
-__global__ __launch_bounds__(256) void stgd_phase_kernel(StepCtx P, int32_t phase, int32_t it) {
+// One phase of one step, one kernel per phase (each with its own register allocation):
+// blockIdx.x = tile.
+template <int DMODE, int KMODE, typename TY, typename TPH, int PHASE>
+__global__ __launch_bounds__(256) void stgd_phase_kernel(StepCtx P, int32_t it) {
   __shared__ StgdSmem sm;
-  run_tile<DMODE, KMODE, TY, TPH>(P, phase, it, blockIdx.x, sm);
+  constexpr bool SPLIT = KMODE == MM_BF16X3;
+  if constexpr (PHASE == PH_SETUP) copy_tile<SPLIT>(P, true, blockIdx.x, sm);
+  else if constexpr (PHASE == PH_P1) p1_tile<DMODE, KMODE, TY, TPH>(P, blockIdx.x, sm);
+  else if constexpr (PHASE == PH_P2) p2_tile<DMODE, KMODE, TY, TPH>(P, blockIdx.x, sm);
+  else if constexpr (PHASE == PH_GRAM) gram_tile<KMODE>(P, it, blockIdx.x, sm);
+  else if constexpr (PHASE == PH_UPDATE) update_tile<KMODE>(P, it, blockIdx.x, sm);
+  else copy_tile<SPLIT>(P, false, blockIdx.x, sm);
 }
 
 // ---- many instances in one persistent launch -------------------------------------------------------
-// The host lays the phases of every instance's steps out as segments of a global ticket sequence,
-// slot by slot (the s-th phase of every instance, then the (s+1)-th, ...), so that each segment
-// comes after every segment it depends on. A workgroup takes the next ticket, waits until its
-// instance has completed all tiles of its earlier segments (done[inst] >= pre), runs the tile and
-// publishes it. Tickets are taken in order by running workgroups only, so the smallest unfinished
-// ticket never waits: the launch cannot deadlock whatever the number of resident workgroups. A
-// wait that exceeds spin_limit polls sets *err, and every later tile is skipped but still counted,
-// so the grid always drains.
+// The host lays the phases of every instance's steps out slot by slot: slot s holds the s-th phase
+// segment of every instance that has one, so each segment's inputs were produced in an earlier
+// slot. The cooperative grid walks the slots; in each, the workgroups share the slot's tiles
+// (tile t of the slot goes to workgroup t mod grid), and a grid barrier separates consecutive
+// slots. Every workgroup executes every barrier, and no workgroup waits on another except there.
 struct BatchSeg {
   int32_t inst;
   int32_t phase;
   int32_t it;
   int32_t ntiles;
   int64_t step;     // step index within the launch
-  int64_t tstart;   // first ticket
-  uint32_t pre;     // tiles of the instance in earlier segments
+  int64_t first;    // index of the segment's first tile within its slot
 };
 
 struct BatchInst {
@@ -681,57 +701,57 @@ __device__ inline void make_step(const BatchInst& I, int64_t st, StepCtx& P) {
 }
 
 template <int DMODE, int KMODE, typename TY, typename TPH>
-__global__ __launch_bounds__(256) void stgd_batch_kernel(const BatchInst* insts, const BatchSeg* segs, int32_t nsegs,
-                                                         int64_t ntickets, unsigned long long* ticket, uint32_t* done,
-                                                         int32_t* err, int64_t spin_limit) {
+__global__ __launch_bounds__(256) void stgd_batch_kernel(const BatchInst* insts, const BatchSeg* segs,
+                                                         const int32_t* slot_seg, const int64_t* slot_tiles,
+                                                         int32_t nslots) {
+  cg::grid_group grid = cg::this_grid();
   __shared__ StgdSmem sm;
   __shared__ StepCtx P;
-  __shared__ int64_t T;
-  __shared__ int32_t sidx, skip;
-  for (;;) {
-    if (threadIdx.x == 0) {
-      T = (int64_t)atomicAdd(ticket, 1ull);
-      skip = 0;
-      if (T < ntickets) {
-        int lo = 0, hi = nsegs - 1;
+  __shared__ int32_t sidx;
+  for (int32_t sl = 0; sl < nslots; ++sl) {
+    const int32_t s0 = slot_seg[sl], s1 = slot_seg[sl + 1];
+    const int64_t ntiles = slot_tiles[sl];
+    for (int64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+      if (threadIdx.x == 0) {
+        int lo = s0, hi = s1 - 1;  // last segment with first <= t
         while (lo < hi) {
           const int mid = (lo + hi + 1) >> 1;
-          if (segs[mid].tstart <= T) lo = mid;
+          if (segs[mid].first <= t) lo = mid;
           else hi = mid - 1;
         }
         sidx = lo;
-        const BatchSeg& s = segs[lo];
-        int64_t spins = 0;
-        while (__hip_atomic_load(done + s.inst, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < s.pre) {
-          if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) || ++spins > spin_limit) {
-            atomicExch(err, 1);
-            skip = 1;
-            break;
-          }
-          __builtin_amdgcn_s_sleep(2);
-        }
-        __threadfence();
-        make_step(insts[s.inst], s.step, P);
+        make_step(insts[segs[lo].inst], segs[lo].step, P);
       }
+      __syncthreads();
+      const BatchSeg& sg = segs[sidx];
+      run_tile<DMODE, KMODE, TY, TPH>(P, sg.phase, sg.it, (int)(t - sg.first), sm);
+      __syncthreads();
     }
-    __syncthreads();
-    if (T >= ntickets) break;
-    __threadfence();  // acquire: this wave sees every tile published before the wait ended
-    const BatchSeg& s = segs[sidx];
-    if (!skip) run_tile<DMODE, KMODE, TY, TPH>(P, s.phase, s.it, (int)(T - s.tstart), sm);
-    __threadfence();  // release: this tile's stores before the count
-    __syncthreads();
-    if (threadIdx.x == 0) atomicAdd(done + s.inst, 1u);
+    grid.sync();
   }
 }
 
-__global__ void batch_reset_kernel(int32_t ninst, unsigned long long* ticket, uint32_t* done, int32_t* err) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t == 0) {
-    *ticket = 0ull;
-    *err = 0;
+// One phase slot of the plan as its own launch (blockIdx.x = tile of the slot): the grouped
+// alternative to the persistent kernel, where the kernel boundary orders consecutive slots.
+template <int DMODE, int KMODE, typename TY, typename TPH>
+__global__ __launch_bounds__(256) void stgd_slot_kernel(const BatchInst* insts, const BatchSeg* segs, int32_t s0, int32_t s1) {
+  __shared__ StgdSmem sm;
+  __shared__ StepCtx P;
+  __shared__ int32_t sidx;
+  const int64_t t = blockIdx.x;
+  if (threadIdx.x == 0) {
+    int lo = s0, hi = s1 - 1;  // last segment with first <= t
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (segs[mid].first <= t) lo = mid;
+      else hi = mid - 1;
+    }
+    sidx = lo;
+    make_step(insts[segs[lo].inst], segs[lo].step, P);
   }
-  if (t < ninst) done[t] = 0u;
+  __syncthreads();
+  const BatchSeg& sg = segs[sidx];
+  run_tile<DMODE, KMODE, TY, TPH>(P, sg.phase, sg.it, (int)(t - sg.first), sm);
 }
 
 __global__ void zero_f64_kernel(int64_t n, double* p) {
@@ -771,8 +791,15 @@ static int run_steps(hipStream_t s, const StgdArgs& args, const void* YA, const 
   P.YCt = YCt;
   P.ns_iters = ns_iters;
   auto launch = [&](int phase, int it) -> int {
-    hipLaunchKernelGGL((stgd_phase_kernel<DMODE, KMODE, TY, TPH>), dim3(phase_tiles(phase, kp, args.b)), dim3(256), 0, s, P,
-                       phase, it);
+    const dim3 grid(phase_tiles(phase, kp, args.b)), block(256);
+    switch (phase) {
+      case PH_SETUP: hipLaunchKernelGGL((stgd_phase_kernel<DMODE, KMODE, TY, TPH, PH_SETUP>), grid, block, 0, s, P, it); break;
+      case PH_P1: hipLaunchKernelGGL((stgd_phase_kernel<DMODE, KMODE, TY, TPH, PH_P1>), grid, block, 0, s, P, it); break;
+      case PH_P2: hipLaunchKernelGGL((stgd_phase_kernel<DMODE, KMODE, TY, TPH, PH_P2>), grid, block, 0, s, P, it); break;
+      case PH_GRAM: hipLaunchKernelGGL((stgd_phase_kernel<DMODE, KMODE, TY, TPH, PH_GRAM>), grid, block, 0, s, P, it); break;
+      case PH_UPDATE: hipLaunchKernelGGL((stgd_phase_kernel<DMODE, KMODE, TY, TPH, PH_UPDATE>), grid, block, 0, s, P, it); break;
+      default: hipLaunchKernelGGL((stgd_phase_kernel<DMODE, KMODE, TY, TPH, PH_COPY>), grid, block, 0, s, P, it); break;
+    }
     MMSB_LAUNCH_CHECK();
     return MMSB_OK;
   };
@@ -854,15 +881,25 @@ int stgd_run(hipStream_t s, int32_t precision, const StgdArgs& args, const void*
 struct StgdBatch {
   int32_t precision = MMSB_PREC_FP32;
   int32_t ninst = 0;
-  int32_t nsegs = 0;
-  int64_t ntickets = 0;
+  int32_t nslots = 0;
   int32_t grid = 0;
-  BatchInst* d_inst = nullptr;   // [ninst]
-  BatchSeg* d_segs = nullptr;    // [nsegs]
-  unsigned long long* d_ticket = nullptr;
-  uint32_t* d_done = nullptr;
-  int32_t* d_err = nullptr;
+  int32_t grid_max = 0;
+  BatchInst* d_inst = nullptr;     // [ninst]
+  BatchSeg* d_segs = nullptr;      // [segments]
+  int32_t* d_slot_seg = nullptr;   // [nslots + 1]: first segment of each slot
+  int64_t* d_slot_tiles = nullptr; // [nslots]
+  std::vector<int32_t> h_slot_seg;  // host copies for the per-slot launches
+  std::vector<int64_t> h_slot_tiles;
 };
+
+template <int DMODE, int KMODE, typename TY, typename TPH>
+static const void* batch_fn() {
+  return reinterpret_cast<const void*>(stgd_batch_kernel<DMODE, KMODE, TY, TPH>);
+}
+
+static const void* batch_kernel(int32_t precision) {
+  return precision == MMSB_PREC_BF16 ? batch_fn<MM_BF16, MM_BF16X3, __bf16, __bf16>() : batch_fn<MM_F32, MM_F32, float, float>();
+}
 
 int stgd_batch_plan(StgdBatch*& out, int32_t precision, const std::vector<StgdBatchMember>& members, int32_t ns_iters,
                     int32_t renorm_every) {
@@ -911,28 +948,31 @@ int stgd_batch_plan(StgdBatch*& out, int32_t precision, const std::vector<StgdBa
       }
     }
   }
-  // slot-major interleaving: the s-th segment of every instance, then the (s+1)-th
   std::vector<BatchSeg> segs;
-  std::vector<uint32_t> pre(members.size(), 0);
+  std::vector<int32_t> slot_seg;
+  std::vector<int64_t> slot_tiles;
   size_t maxlen = 0;
   for (auto& v : per) maxlen = std::max(maxlen, v.size());
-  int64_t t = 0;
-  for (size_t sl = 0; sl < maxlen; ++sl)
+  for (size_t sl = 0; sl < maxlen; ++sl) {
+    slot_seg.push_back((int32_t)segs.size());
+    int64_t t = 0;
     for (size_t i = 0; i < per.size(); ++i) {
       if (sl >= per[i].size()) continue;
       BatchSeg sg = per[i][sl];
-      sg.tstart = t;
-      sg.pre = pre[i];
-      pre[i] += (uint32_t)sg.ntiles;
+      sg.first = t;
       t += sg.ntiles;
       segs.push_back(sg);
     }
+    slot_tiles.push_back(t);
+  }
+  slot_seg.push_back((int32_t)segs.size());
   MMSB_CHECK_ARG(segs.size() < (size_t)INT32_MAX, "stgd batch: too many segments");
   auto* b = new StgdBatch();
   b->precision = precision;
   b->ninst = (int32_t)insts.size();
-  b->nsegs = (int32_t)segs.size();
-  b->ntickets = t;
+  b->nslots = (int32_t)slot_tiles.size();
+  b->h_slot_seg = slot_seg;
+  b->h_slot_tiles = slot_tiles;
   auto fail = [&](hipError_t e) {
     stgd_batch_free(b);
     return set_error(MMSB_ERR_HIP, "stgd batch: %s", hipGetErrorString(e));
@@ -940,51 +980,58 @@ int stgd_batch_plan(StgdBatch*& out, int32_t precision, const std::vector<StgdBa
   hipError_t e;
   if ((e = hipMalloc(&b->d_inst, sizeof(BatchInst) * insts.size())) != hipSuccess) return fail(e);
   if ((e = hipMalloc(&b->d_segs, sizeof(BatchSeg) * segs.size())) != hipSuccess) return fail(e);
-  if ((e = hipMalloc(&b->d_ticket, sizeof(unsigned long long))) != hipSuccess) return fail(e);
-  if ((e = hipMalloc(&b->d_done, sizeof(uint32_t) * insts.size())) != hipSuccess) return fail(e);
-  if ((e = hipMalloc(&b->d_err, sizeof(int32_t))) != hipSuccess) return fail(e);
+  if ((e = hipMalloc(&b->d_slot_seg, sizeof(int32_t) * slot_seg.size())) != hipSuccess) return fail(e);
+  if ((e = hipMalloc(&b->d_slot_tiles, sizeof(int64_t) * slot_tiles.size())) != hipSuccess) return fail(e);
   if ((e = hipMemcpy(b->d_inst, insts.data(), sizeof(BatchInst) * insts.size(), hipMemcpyHostToDevice)) != hipSuccess)
     return fail(e);
   if ((e = hipMemcpy(b->d_segs, segs.data(), sizeof(BatchSeg) * segs.size(), hipMemcpyHostToDevice)) != hipSuccess)
     return fail(e);
-  const void* fn = precision == MMSB_PREC_BF16
-                       ? reinterpret_cast<const void*>(stgd_batch_kernel<MM_BF16, MM_BF16X3, __bf16, __bf16>)
-                       : reinterpret_cast<const void*>(stgd_batch_kernel<MM_F32, MM_F32, float, float>);
+  if ((e = hipMemcpy(b->d_slot_seg, slot_seg.data(), sizeof(int32_t) * slot_seg.size(), hipMemcpyHostToDevice)) != hipSuccess)
+    return fail(e);
+  if ((e = hipMemcpy(b->d_slot_tiles, slot_tiles.data(), sizeof(int64_t) * slot_tiles.size(), hipMemcpyHostToDevice)) !=
+      hipSuccess)
+    return fail(e);
   int dev = 0, per_cu = 0, cus = 0;
   if ((e = hipGetDevice(&dev)) != hipSuccess) return fail(e);
-  if ((e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, 256, 0)) != hipSuccess) return fail(e);
+  if ((e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, batch_kernel(precision), 256, 0)) != hipSuccess) return fail(e);
   if ((e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev)) != hipSuccess) return fail(e);
-  b->grid = std::max(1, per_cu) * std::max(1, cus);
+  if (per_cu < 1) {
+    stgd_batch_free(b);
+    return set_error(MMSB_ERR_HIP, "stgd batch: the persistent kernel does not fit on a compute unit");
+  }
+  b->grid_max = per_cu * cus;   // cooperative launch: every workgroup resident
+  b->grid = cus;
   out = b;
   return MMSB_OK;
 }
 
 void stgd_batch_set_grid(StgdBatch* b, int32_t grid) {
-  if (b && grid > 0) b->grid = grid;
+  if (b && grid > 0) b->grid = std::min(grid, b->grid_max);
 }
 
 int32_t stgd_batch_grid(const StgdBatch* b) { return b ? b->grid : 0; }
+int32_t stgd_batch_grid_max(const StgdBatch* b) { return b ? b->grid_max : 0; }
 
 int stgd_batch_launch(hipStream_t s, StgdBatch* b) {
   MMSB_CHECK_ARG(b, "stgd batch: null plan");
-  hipLaunchKernelGGL(batch_reset_kernel, dim3((unsigned)ceil_div(b->ninst + 1, 256)), dim3(256), 0, s, b->ninst, b->d_ticket,
-                     b->d_done, b->d_err);
-  MMSB_LAUNCH_CHECK();
-  const int64_t spin_limit = 1ll << 22;
-  if (b->precision == MMSB_PREC_BF16)
-    hipLaunchKernelGGL((stgd_batch_kernel<MM_BF16, MM_BF16X3, __bf16, __bf16>), dim3(b->grid), dim3(256), 0, s, b->d_inst,
-                       b->d_segs, b->nsegs, b->ntickets, b->d_ticket, b->d_done, b->d_err, spin_limit);
-  else
-    hipLaunchKernelGGL((stgd_batch_kernel<MM_F32, MM_F32, float, float>), dim3(b->grid), dim3(256), 0, s, b->d_inst,
-                       b->d_segs, b->nsegs, b->ntickets, b->d_ticket, b->d_done, b->d_err, spin_limit);
-  MMSB_LAUNCH_CHECK();
+  void* args[] = {&b->d_inst, &b->d_segs, &b->d_slot_seg, &b->d_slot_tiles, &b->nslots};
+  MMSB_HIP(hipLaunchCooperativeKernel(batch_kernel(b->precision), dim3(b->grid), dim3(256), args, 0, s));
   return MMSB_OK;
 }
 
-int stgd_batch_error(hipStream_t s, StgdBatch* b, int32_t* err) {
-  MMSB_CHECK_ARG(b && err, "stgd batch: null argument");
-  MMSB_HIP(hipMemcpyAsync(err, b->d_err, sizeof(int32_t), hipMemcpyDeviceToHost, s));
-  MMSB_HIP(hipStreamSynchronize(s));
+int stgd_batch_launch_slots(hipStream_t s, StgdBatch* b) {
+  MMSB_CHECK_ARG(b, "stgd batch: null plan");
+  for (int32_t sl = 0; sl < b->nslots; ++sl) {
+    const dim3 grid((unsigned)b->h_slot_tiles[sl]);
+    const int32_t s0 = b->h_slot_seg[sl], s1 = b->h_slot_seg[sl + 1];
+    if (b->precision == MMSB_PREC_BF16)
+      hipLaunchKernelGGL((stgd_slot_kernel<MM_BF16, MM_BF16X3, __bf16, __bf16>), grid, dim3(256), 0, s, b->d_inst, b->d_segs,
+                         s0, s1);
+    else
+      hipLaunchKernelGGL((stgd_slot_kernel<MM_F32, MM_F32, float, float>), grid, dim3(256), 0, s, b->d_inst, b->d_segs, s0,
+                         s1);
+    MMSB_LAUNCH_CHECK();
+  }
   return MMSB_OK;
 }
 
@@ -992,9 +1039,8 @@ void stgd_batch_free(StgdBatch* b) {
   if (!b) return;
   (void)hipFree(b->d_inst);
   (void)hipFree(b->d_segs);
-  (void)hipFree(b->d_ticket);
-  (void)hipFree(b->d_done);
-  (void)hipFree(b->d_err);
+  (void)hipFree(b->d_slot_seg);
+  (void)hipFree(b->d_slot_tiles);
   delete b;
 }
 

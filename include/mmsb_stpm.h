@@ -259,15 +259,19 @@ int mmsb_set_learn_rate(mmsb_ctx* ctx, double learn_rate);
 int mmsb_stgd_timed(mmsb_ctx* ctx, int64_t steps, int32_t precision, double* phase_ms);
 /* Many independent instances (contexts with their neighbour vectors, on one device, each at an
  * epoch boundary, sharing ns_iters and renorm_every; n, k, minibatch, ALPHA_0 and LEARN_RATE may
- * differ) advanced together: every epoch of every member runs in ONE persistent launch, replayed
- * from a captured HIP graph. No counterpart in the reference, which runs one instance per process.
+ * differ) advanced together: every epoch of every member runs either in ONE persistent
+ * (cooperative) launch, or -- after mmsb_batch_autotune finds it faster -- as one grouped launch per
+ * phase slot replayed from a captured HIP graph. No counterpart in the reference, which runs one
+ * instance per process.
  * The members' Phi, frob logs and step counts advance exactly as with mmsb_stgd(ctx, epochs * spe,
  * precision); a member driven by other calls in between must be put in a new batch. */
 typedef struct mmsb_batch mmsb_batch;
 int mmsb_batch_create(mmsb_ctx* const* ctxs, int32_t count, int32_t precision, mmsb_batch** out);
 int mmsb_batch_run(mmsb_batch* batch, int32_t epochs);
-/* Times one epoch for each candidate persistent grid size (workgroups), restoring every member's
- * Phi afterwards, and keeps the fastest for later runs. */
+/* Times one epoch for each candidate launch shape -- the persistent kernel at several grid sizes
+ * (workgroups), and one grouped launch per phase slot replayed from a captured HIP graph -- restoring
+ * every member's Phi afterwards, and keeps the fastest for later runs. best_grid receives the
+ * persistent grid, or 0 when the grouped launches won. */
 int mmsb_batch_autotune(mmsb_batch* batch, int32_t* best_grid, double* best_ms);
 void mmsb_batch_destroy(mmsb_batch* batch);
 /* Loss and adjoint (mmsb_stgd_grad_f32) over X rows [row0, row0 + nrows) for a host Phi [k x k].
