@@ -315,7 +315,7 @@ def test_batch_matches_single_instances(mmsb, prec):
     batch = mmsb.Batch(members, prec)
     batch.run(2)
     grid, ms = batch.autotune()
-    assert grid > 0 and ms > 0
+    assert grid >= 0 and ms > 0          # 0: grouped per-slot launches won
     batch.run(1)
     for (n, k, b, a0, eta), p_single, p_batch in zip(specs, singles, members):
         spe = -(-n // b)
