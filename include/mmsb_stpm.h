@@ -255,7 +255,8 @@ int mmsb_set_learn_rate(mmsb_ctx* ctx, double learn_rate);
 /* Measurement entry (no counterpart in the reference): `steps` STGD steps as mmsb_stgd, launched
  * directly (never from a captured graph) with HIP events on the context stream around each phase;
  * phase_ms[0..2] receive the summed device time of the P1 kernel (Y Phi contractions and
- * (Phi^T Phi)^2), the P2 kernel (gradient step) and the renormalisation kernels. */
+ * (Phi^T Phi)^2), the P2 kernel (gradient step) and the renormalisation kernels, each interval
+ * less the time between two events recorded back to back (measured in the same call). */
 int mmsb_stgd_timed(mmsb_ctx* ctx, int64_t steps, int32_t precision, double* phase_ms);
 /* Many independent instances (contexts with their neighbour vectors, on one device, each at an
  * epoch boundary, sharing ns_iters and renorm_every; n, k, minibatch, ALPHA_0 and LEARN_RATE may
