@@ -1,0 +1,101 @@
+// CSR transpose on the device (G^T of each adjacency block, built once after upload).
+//
+// A stable LSD radix sort (hipCUB) of (column, nonzero index) pairs orders the nonzeros by column
+// and, within a column, by their position in G -- that is, by row, because G's nonzeros are
+// stored row by row. Row pointers of G^T are the lower bounds of each column in the sorted keys;
+// the rows themselves are recovered from the nonzero index by a binary search over G's row
+// pointers, and values (if any) gathered by the same index. The result equals the host
+// counting-sort transpose element for element.
+#include <hipcub/hipcub.hpp>
+
+#include "common.h"
+#include "dense.h"
+
+namespace mmsb {
+
+__global__ void iota_kernel(int64_t n, int32_t* __restrict__ out) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) out[t] = (int32_t)t;
+}
+
+// rpT[c] = number of sorted keys < c, for c in [0, nc]
+__global__ void lower_bounds_kernel(int32_t nc, int64_t nnz, const int32_t* __restrict__ keys, int32_t* __restrict__ rpT) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c > nc) return;
+  int64_t lo = 0, hi = nnz;
+  while (lo < hi) {
+    const int64_t mid = (lo + hi) >> 1;
+    if (keys[mid] < c) lo = mid + 1;
+    else hi = mid;
+  }
+  rpT[c] = (int32_t)lo;
+}
+
+// ciT[p] = row of nonzero perm[p] (last row r with rp[r] <= perm[p]); vT[p] = v[perm[p]]
+__global__ void gather_rows_kernel(int32_t nr, int64_t nnz, const int32_t* __restrict__ rp, const int32_t* __restrict__ perm,
+                                   const float* __restrict__ v, int32_t* __restrict__ ciT, float* __restrict__ vT) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= nnz) return;
+  const int32_t e = perm[p];
+  int32_t lo = 0, hi = nr - 1;  // rp[lo] <= e < rp[nr]
+  while (lo < hi) {
+    const int32_t mid = (lo + hi + 1) >> 1;
+    if (rp[mid] <= e) lo = mid;
+    else hi = mid - 1;
+  }
+  ciT[p] = lo;
+  if (v) vT[p] = v[e];
+}
+
+static int bits_for(int32_t n) {
+  int b = 1;
+  while (b < 31 && (1ll << b) < (int64_t)n) ++b;
+  return b;
+}
+
+int csr_transpose(hipStream_t s, int32_t nr, int32_t nc, int64_t nnz, const int32_t* rp, const int32_t* ci, const float* v,
+                  int32_t* rpT, int32_t* ciT, float* vT) {
+  MMSB_CHECK_ARG(nr >= 1 && nc >= 1 && nnz >= 0 && nnz < INT32_MAX, "csr_transpose: bad shape");
+  if (nnz == 0) {
+    hipLaunchKernelGGL(lower_bounds_kernel, dim3((unsigned)ceil_div((int64_t)nc + 1, 256)), dim3(256), 0, s, nc, (int64_t)0,
+                       ci, rpT);
+    MMSB_LAUNCH_CHECK();
+    return MMSB_OK;
+  }
+  int32_t *keys_out = nullptr, *idx = nullptr, *perm = nullptr;
+  void* temp = nullptr;
+  size_t temp_bytes = 0;
+  const int end_bit = bits_for(nc);
+  int st = MMSB_OK;
+  auto fail = [&](hipError_t e) { return set_error(MMSB_ERR_HIP, "csr_transpose: %s", hipGetErrorString(e)); };
+  hipError_t e = hipMalloc(&keys_out, sizeof(int32_t) * nnz);
+  if (e == hipSuccess) e = hipMalloc(&idx, sizeof(int32_t) * nnz);
+  if (e == hipSuccess) e = hipMalloc(&perm, sizeof(int32_t) * nnz);
+  if (e == hipSuccess)
+    e = hipcub::DeviceRadixSort::SortPairs(nullptr, temp_bytes, ci, keys_out, idx, perm, (int)nnz, 0, end_bit, s);
+  if (e == hipSuccess) e = hipMalloc(&temp, std::max<size_t>(temp_bytes, 1));
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(iota_kernel, dim3((unsigned)ceil_div(nnz, 256)), dim3(256), 0, s, nnz, idx);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess)
+    e = hipcub::DeviceRadixSort::SortPairs(temp, temp_bytes, ci, keys_out, idx, perm, (int)nnz, 0, end_bit, s);
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(lower_bounds_kernel, dim3((unsigned)ceil_div((int64_t)nc + 1, 256)), dim3(256), 0, s, nc, nnz,
+                       keys_out, rpT);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(gather_rows_kernel, dim3((unsigned)ceil_div(nnz, 256)), dim3(256), 0, s, nr, nnz, rp, perm, v, ciT, vT);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(s);  // the temporaries are freed below
+  if (e != hipSuccess) st = fail(e);
+  (void)hipFree(keys_out);
+  (void)hipFree(idx);
+  (void)hipFree(perm);
+  (void)hipFree(temp);
+  return st;
+}
+
+}  // namespace mmsb

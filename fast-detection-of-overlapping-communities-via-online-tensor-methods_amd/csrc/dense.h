@@ -34,6 +34,11 @@ int scale_cols(hipStream_t s, int64_t n, int32_t c, T* M, int32_t ld, const doub
 size_t jacobi_workspace_bytes(int32_t r);
 int jacobi_svd(hipStream_t s, int32_t r, double* A, double* U, double* sigma, double* V, void* work, int32_t* sweeps_out);
 
+// G^T in CSR (rpT [nc + 1], ciT [nnz], vT [nnz] if v) of the device CSR G [nr x nc]; returns after
+// the result is complete (csr_transpose.hip).
+int csr_transpose(hipStream_t s, int32_t nr, int32_t nc, int64_t nnz, const int32_t* rp, const int32_t* ci, const float* v,
+                  int32_t* rpT, int32_t* ciT, float* vT);
+
 int transpose_f32(hipStream_t s, int64_t R, int32_t C, const float* in, int64_t ldi, float* out, int64_t ldo);
 int pinv_scale(hipStream_t s, int32_t k, const double* v, double shift, double tol, int mode, double* out, double* vout);
 int sym_shift(hipStream_t s, int32_t r, double* S, double* shift_out);

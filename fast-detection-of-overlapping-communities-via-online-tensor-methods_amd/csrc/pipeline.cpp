@@ -35,6 +35,7 @@
 #include "mmsb_stpm.h"
 #include "post.h"
 #include "stgd.h"
+#include "xfer.h"
 
 namespace mmsb {
 
@@ -149,21 +150,6 @@ static double now_s() {
 }
 
 // ---- CSR upload (with the transpose built on the host by a counting sort) -------------------
-static void host_transpose(const mmsb_csr& a, std::vector<int32_t>& rp, std::vector<int32_t>& ci, std::vector<float>& v) {
-  rp.assign((size_t)a.n_cols + 1, 0);
-  for (int64_t e = 0; e < a.nnz; ++e) rp[a.colidx[e] + 1]++;
-  for (int32_t j = 0; j < a.n_cols; ++j) rp[j + 1] += rp[j];
-  ci.resize((size_t)a.nnz);
-  if (a.vals) v.resize((size_t)a.nnz);
-  std::vector<int32_t> fill(rp.begin(), rp.end() - 1);
-  for (int32_t i = 0; i < a.n_rows; ++i)
-    for (int32_t e = a.rowptr[i]; e < a.rowptr[i + 1]; ++e) {
-      const int32_t j = a.colidx[e];
-      ci[fill[j]] = i;
-      if (a.vals) v[fill[j]] = a.vals[e];
-      ++fill[j];
-    }
-}
 
 static int upload_csr(mmsb_ctx* c, DevCsr& d, int32_t nr, int32_t nc, int64_t nnz, const int32_t* rp, const int32_t* ci,
                       const float* v) {
@@ -172,14 +158,24 @@ static int upload_csr(mmsb_ctx* c, DevCsr& d, int32_t nr, int32_t nc, int64_t nn
   d.nnz = nnz;
   TRY(dalloc(c, d.rowptr, (size_t)nr + 1));
   TRY(dalloc(c, d.colidx, (size_t)std::max<int64_t>(nnz, 1)));
-  HCHK(hipMemcpyAsync(d.rowptr, rp, sizeof(int32_t) * ((size_t)nr + 1), hipMemcpyHostToDevice, c->s));
-  if (nnz) HCHK(hipMemcpyAsync(d.colidx, ci, sizeof(int32_t) * nnz, hipMemcpyHostToDevice, c->s));
+  TRY(h2d(c->s, d.rowptr, rp, sizeof(int32_t) * ((size_t)nr + 1)));
+  TRY(h2d(c->s, d.colidx, ci, sizeof(int32_t) * (size_t)nnz));
   if (v) {
     TRY(dalloc(c, d.vals, (size_t)std::max<int64_t>(nnz, 1)));
-    if (nnz) HCHK(hipMemcpyAsync(d.vals, v, sizeof(float) * nnz, hipMemcpyHostToDevice, c->s));
+    TRY(h2d(c->s, d.vals, v, sizeof(float) * (size_t)nnz));
   }
-  HCHK(hipStreamSynchronize(c->s));  // host arrays may be temporaries
   return MMSB_OK;
+}
+
+// G^T built on the device from the uploaded G.
+static int transpose_csr(mmsb_ctx* c, const DevCsr& g, DevCsr& t) {
+  t.n_rows = g.n_cols;
+  t.n_cols = g.n_rows;
+  t.nnz = g.nnz;
+  TRY(dalloc(c, t.rowptr, (size_t)t.n_rows + 1));
+  TRY(dalloc(c, t.colidx, (size_t)std::max<int64_t>(t.nnz, 1)));
+  if (g.vals) TRY(dalloc(c, t.vals, (size_t)std::max<int64_t>(t.nnz, 1)));
+  return csr_transpose(c->s, g.n_rows, g.n_cols, g.nnz, g.rowptr, g.colidx, g.vals, t.rowptr, t.colidx, t.vals);
 }
 
 static int spmm(mmsb_ctx* c, const DevCsr& g, float alpha, const float* B, int32_t ldb, int32_t r, float* Y, int32_t ldy) {
@@ -928,11 +924,9 @@ static int copy_raw(mmsb_ctx* c, const View& v, void* host) {
   void* staged = nullptr;
   size_t bytes = 0;
   TRY(stage_compact(c, v, &staged, &bytes));
-  hipError_t e = hipMemcpyAsync(host, staged ? staged : v.p, bytes, hipMemcpyDeviceToHost, c->s);
-  if (e == hipSuccess) e = hipStreamSynchronize(c->s);
+  const int st = d2h(c->s, host, staged ? staged : v.p, bytes);
   if (staged) (void)hipFree(staged);
-  if (e != hipSuccess) return set_error(MMSB_ERR_HIP, "copy_out: %s", hipGetErrorString(e));
-  return MMSB_OK;
+  return st;
 }
 
 static int copy_out(mmsb_ctx* c, const View& v, double* host) {
@@ -1083,17 +1077,27 @@ extern "C" int mmsb_set_block(mmsb_ctx* c, int32_t which, const mmsb_csr* b) {
                  which, b->n_rows, b->n_cols, rows_of[which], cols_of[which]);
   MMSB_CHECK_ARG(b->nnz >= 0 && b->nnz < INT32_MAX && b->rowptr && (b->nnz == 0 || b->colidx), "set_block: bad CSR");
   MMSB_CHECK_ARG(b->rowptr[0] == 0 && b->rowptr[b->n_rows] == b->nnz, "set_block: rowptr does not match nnz");
-  for (int32_t i = 0; i < b->n_rows; ++i)
-    MMSB_CHECK_ARG(b->rowptr[i + 1] >= b->rowptr[i], "set_block: rowptr not monotone at row %d", i);
-  for (int64_t e = 0; e < b->nnz; ++e)
-    MMSB_CHECK_ARG(b->colidx[e] >= 0 && b->colidx[e] < b->n_cols, "set_block: column index %d out of range", b->colidx[e]);
+  {  // every row pointer monotone, every column index in range (threads report the first bad index)
+    const int nt = default_threads();
+    std::vector<int64_t> bad_row(nt, -1), bad_nz(nt, -1);
+    parallel_for(b->n_rows, nt, [&](int64_t lo, int64_t hi) {
+      const int t = (int)(lo * nt / b->n_rows);
+      for (int64_t i = lo; i < hi && bad_row[t] < 0; ++i)
+        if (b->rowptr[i + 1] < b->rowptr[i]) bad_row[t] = i;
+    });
+    for (int64_t r : bad_row) MMSB_CHECK_ARG(r < 0, "set_block: rowptr not monotone at row %lld", (long long)r);
+    parallel_for(b->nnz, nt, [&](int64_t lo, int64_t hi) {
+      const int t = (int)(lo * nt / std::max<int64_t>(b->nnz, 1));
+      for (int64_t e = lo; e < hi && bad_nz[t] < 0; ++e)
+        if (b->colidx[e] < 0 || b->colidx[e] >= b->n_cols) bad_nz[t] = e;
+    });
+    for (int64_t e : bad_nz)
+      MMSB_CHECK_ARG(e < 0, "set_block: column index %d out of range", e < 0 ? 0 : b->colidx[e]);
+  }
   HCHK(hipSetDevice(c->cfg.device));
   if (c->blk[which].rowptr) return set_error(MMSB_ERR_STATE, "set_block: block %d already set", which);
   TRY(upload_csr(c, c->blk[which], b->n_rows, b->n_cols, b->nnz, b->rowptr, b->colidx, b->vals));
-  std::vector<int32_t> rp, ci;
-  std::vector<float> v;
-  host_transpose(*b, rp, ci, v);
-  TRY(upload_csr(c, c->blkT[which], b->n_cols, b->n_rows, b->nnz, rp.data(), ci.data(), b->vals ? v.data() : nullptr));
+  TRY(transpose_csr(c, c->blk[which], c->blkT[which]));
   c->have[which] = true;
   return MMSB_OK;
 }
@@ -1165,9 +1169,9 @@ extern "C" int mmsb_set_yvectors(mmsb_ctx* c, int64_t n_x, int32_t k, const doub
       b[(size_t)x * kp + i] = (float)yb[x * k + i];
       ct[(size_t)i * c->n_pad + x] = (float)yc[x * k + i];
     }
-  HCHK(hipMemcpyAsync(c->YA, a.data(), sizeof(float) * a.size(), hipMemcpyHostToDevice, c->s));
-  HCHK(hipMemcpyAsync(c->YB, b.data(), sizeof(float) * b.size(), hipMemcpyHostToDevice, c->s));
-  HCHK(hipMemcpyAsync(c->YCt, ct.data(), sizeof(float) * ct.size(), hipMemcpyHostToDevice, c->s));
+  TRY(h2d(c->s, c->YA, a.data(), sizeof(float) * a.size()));
+  TRY(h2d(c->s, c->YB, b.data(), sizeof(float) * b.size()));
+  TRY(h2d(c->s, c->YCt, ct.data(), sizeof(float) * ct.size()));
   const double* mus[3] = {mua, mub, muc};
   for (int i = 0; i < 3; ++i) HCHK(hipMemcpyAsync(c->muw[i], mus[i], sizeof(double) * k, hipMemcpyHostToDevice, c->s));
   HCHK(hipStreamSynchronize(c->s));

@@ -409,30 +409,89 @@ __device__ inline uint16_t* sp(uint16_t* p) {
   return SPLIT ? p : nullptr;
 }
 
-// P1 tiles [0, ntb * ntk): a 32-row slice of the minibatch x 32 columns of C_g; the rest: H tiles.
+// Sums of strided partials, one per output column: out[o] = sum_{t < nt} src[t * stride + off(o)]
+// for 64 outputs (P1: m_A, m_B of 32 columns) or 32 (P2: column sums). The loads of all threads
+// are issued by part_load before the MFMA loop and reduced by part_reduce after it, so their
+// latency overlaps the operand loads instead of forming a chain.
+constexpr int kPartRegs = 4;
+struct PartSum {
+  float v[kPartRegs];
+  int extra;  // partials beyond kPartRegs * lanes-per-output, summed late (large k or b)
+};
+
+// P1: tile (ti rows of Phi) partials of m_s[j0 + c]: thread -> output o = t / 4 (s = o / 32,
+// c = o % 32), lane q = t % 4 takes ti = q, q + 4, ...
+__device__ inline void mpart_load(PartSum& ps, const float* mpart, int kp, int ntk, int64_t j0) {
+  const int o = threadIdx.x >> 2, q = threadIdx.x & 3;
+  const int s = o >> 5, c = o & 31;
+#pragma unroll
+  for (int u = 0; u < kPartRegs; ++u) {
+    const int ti = q + 4 * u;
+    ps.v[u] = ti < ntk ? mpart[((int64_t)ti * 2 + s) * kp + j0 + c] : 0.f;
+  }
+  ps.extra = ntk > 4 * kPartRegs;
+}
+
+__device__ inline float mpart_reduce(const PartSum& ps, const float* mpart, int kp, int ntk, int64_t j0) {
+  const int o = threadIdx.x >> 2, q = threadIdx.x & 3;
+  const int s = o >> 5, c = o & 31;
+  float m = (ps.v[0] + ps.v[1]) + (ps.v[2] + ps.v[3]);
+  if (ps.extra)
+    for (int ti = q + 4 * kPartRegs; ti < ntk; ti += 4) m += mpart[((int64_t)ti * 2 + s) * kp + j0 + c];
+  m += __shfl_xor(m, 1);
+  m += __shfl_xor(m, 2);
+  return m;  // valid in every lane of the output's group of four
+}
+
+// P2: column sums of the b/32 row-tile partials cs_part[tb][j0 + c]: thread -> c = t / 8,
+// lane q = t % 8 takes tb = q, q + 8, ...
+__device__ inline void cspart_load(PartSum& ps, const float* cs_part, int kp, int ntb, int64_t j0) {
+  const int c = threadIdx.x >> 3, q = threadIdx.x & 7;
+#pragma unroll
+  for (int u = 0; u < kPartRegs; ++u) {
+    const int tb = q + 8 * u;
+    ps.v[u] = tb < ntb ? cs_part[(int64_t)tb * kp + j0 + c] : 0.f;
+  }
+  ps.extra = ntb > 8 * kPartRegs;
+}
+
+__device__ inline float cspart_reduce(const PartSum& ps, const float* cs_part, int kp, int ntb, int64_t j0) {
+  const int c = threadIdx.x >> 3, q = threadIdx.x & 7;
+  float m = (ps.v[0] + ps.v[1]) + (ps.v[2] + ps.v[3]);
+  if (ps.extra)
+    for (int tb = q + 8 * kPartRegs; tb < ntb; tb += 8) m += cs_part[(int64_t)tb * kp + j0 + c];
+  m += __shfl_xor(m, 1);
+  m += __shfl_xor(m, 2);
+  m += __shfl_xor(m, 4);
+  return m;
+}
+
+// P1 tiles [0, ntk * ntk): H tiles; the rest: a 32-row slice of the minibatch x 32 columns of C_g.
 template <int DMODE, int KMODE, typename TY, typename TPH>
 __device__ void p1_tile(const StepCtx& P, int tile, StgdSmem& sm) {
   constexpr bool SPLIT = KMODE == MM_BF16X3;
   const StgdArgs& a = P.a;
   const int kp = a.kp;
-  const int ntk = kp / 32, ntb = a.b / 32;
+  const int ntk = kp / 32;
   const int64_t nel = (int64_t)kp * kp;
   const int w = threadIdx.x >> 6;
   Red4& red = sm.p1.red;
   floatx16 acc = {};
   int64_t k0, kl;
-  if (tile < ntb * ntk) {
+  if (tile >= ntk * ntk) {  // data tiles follow the H tiles, which then start in the first wave
+    tile -= ntk * ntk;
     const int tb = tile / ntk, tj = tile % ntk;
     const int64_t x0 = (int64_t)tb * 32, j0 = (int64_t)tj * 32;
+    PartSum ps;
+    mpart_load(ps, P.B.mpart, kp, ntk, j0);  // m_A, m_B of the block's columns
     // waves 0, 1: P_A over two halves of the k reduction; waves 2, 3: P_B
     const TY* Yb = static_cast<const TY*>(w < 2 ? P.YA : P.YB) + P.row0 * (int64_t)kp;
     kslice<DMODE>(kp, 2, w & 1, k0, kl);
     wave_mm<DMODE, true>(acc, opnd<TY>(Yb, nullptr, kp), pick<DMODE>(P.B.phiT, P.B.phiT16, nel, kp), x0, j0, k0, kl);
-    if (threadIdx.x < 64) {  // m_A, m_B of the block's columns from the per-row-block partials
-      const int s = threadIdx.x >> 5, c = threadIdx.x & 31;
-      float m = 0.f;
-      for (int ti = 0; ti < ntk; ++ti) m += P.B.mpart[((int64_t)ti * 2 + s) * kp + j0 + c];
-      sm.p1.mvec[s][c] = m;
+    {
+      const float m = mpart_reduce(ps, P.B.mpart, kp, ntk, j0);
+      const int o = threadIdx.x >> 2;
+      if ((threadIdx.x & 3) == 0) sm.p1.mvec[o >> 5][o & 31] = m;
     }
     red_put(red, acc);
     __syncthreads();
@@ -460,7 +519,6 @@ __device__ void p1_tile(const StepCtx& P, int tile, StgdSmem& sm) {
       }
     }
   } else {
-    tile -= ntb * ntk;
     const int ti = tile / ntk, tj = tile % ntk;
     const int64_t i0 = (int64_t)ti * 32, j0 = (int64_t)tj * 32;
     const Opnd<op_t<KMODE>> phiT_k = pick<KMODE>(P.B.phiT, P.B.phiT16, nel, kp);
@@ -492,6 +550,9 @@ __device__ void p2_tile(const StepCtx& P, int tile, StgdSmem& sm) {
   Red4& red = sm.p2.red;
   floatx16 acc = {};
   int64_t k0, kl;
+  PartSum ps;  // column sums of P_A o P_B and m_A, m_B for c_mu, gathered during the MFMA loop
+  cspart_load(ps, a.cs_part, kp, a.b / 32, j0);
+  const float mab_a = a.mab[j0 + (threadIdx.x >> 3)], mab_b = a.mab[kp + j0 + (threadIdx.x >> 3)];
   if (w < 2) {  // Phi H over two halves of k (H symmetric: row j of H is column j)
     kslice<KMODE>(kp, 2, w, k0, kl);
     wave_mm<KMODE, true>(acc, pick<KMODE>(a.phi, P.B.phi16[P.cur], nel, kp), pick<KMODE>(P.B.h, P.B.h16, nel, kp), i0, j0,
@@ -505,11 +566,10 @@ __device__ void p2_tile(const StepCtx& P, int tile, StgdSmem& sm) {
     else
       wave_mm<DMODE, false>(acc, yc, cg, i0, j0, k0, kl);
   }
-  if (threadIdx.x < 32) {
-    const int c = threadIdx.x;
-    float s = 0.f;
-    for (int tb = 0; tb < a.b / 32; ++tb) s += a.cs_part[(int64_t)tb * kp + j0 + c];
-    sm.p2.cmu[c] = a.c0 * a.mab[j0 + c] * a.mab[kp + j0 + c] + a.c1 * s / (float)P.nb;
+  {
+    const int c = threadIdx.x >> 3;
+    const float ssum = cspart_reduce(ps, a.cs_part, kp, a.b / 32, j0);
+    if ((threadIdx.x & 7) == 0) sm.p2.cmu[c] = a.c0 * mab_a * mab_b + a.c1 * ssum / (float)P.nb;
   }
   if (tile == 0)
     for (int t = threadIdx.x; t < kp; t += blockDim.x) P.B.rowsum[t] = 0.f;
@@ -581,11 +641,18 @@ __device__ void update_tile(const StepCtx& P, int it, int tile, StgdSmem& sm) {
   floatx16 acc = {};
   int64_t k0, kl;
   kslice<KMODE>(kp, 4, w, k0, kl);
+  float rs[8];  // wave 3: ||G||_inf from the row sums of |G|, loads issued before the MFMA loop
+  if (w == 3) {
+#pragma unroll
+    for (int u = 0; u < 8; ++u) rs[u] = lane + 64 * u < kp ? rowsum[lane + 64 * u] : 0.f;
+  }
   wave_mm<KMODE, true>(acc, pick<KMODE>(nb.x, nb.x16, nel, kp), pick<KMODE>(P.B.gt, P.B.gt16, nel, kp), i0, j0, k0,
                        kl);  // X G (G symmetric)
-  if (w == 3) {  // ||G||_inf from the row sums of |G|
+  if (w == 3) {
     float mx = 0.f;
-    for (int t = lane; t < kp; t += 64) mx = fmaxf(mx, rowsum[t]);
+#pragma unroll
+    for (int u = 0; u < 8; ++u) mx = fmaxf(mx, rs[u]);
+    for (int t = lane + 512; t < kp; t += 64) mx = fmaxf(mx, rowsum[t]);
     for (int off = 32; off > 0; off >>= 1) mx = fmaxf(mx, __shfl_xor(mx, off));
     if (lane == 0) sm.p2.cscale = mx > 2.f ? sqrtf(2.f / mx) : 1.f;
   }
