@@ -363,7 +363,7 @@ __global__ __launch_bounds__(256) void nn_kernel(int64_t n, int32_t p, int32_t q
 
 // Y tile [128 x 128] = alpha * X[rows, :] C[:, cols] (+ beta * Y) on FP64 MFMA; CT: C stored transposed.
 template <typename TX, typename TY, bool CT>
-__global__ __launch_bounds__(256) void nn_mfma_kernel(int64_t n, int32_t p, int32_t q, double alpha,
+__global__ __launch_bounds__(256, 2) void nn_mfma_kernel(int64_t n, int32_t p, int32_t q, double alpha,
                                                       const TX* __restrict__ X, int32_t ldx, const double* __restrict__ C,
                                                       int32_t ldc, double beta, TY* __restrict__ Y, int32_t ldy) {
   __shared__ double Xs[kMK][kMPad];  // k-major: Xs[k][row]
