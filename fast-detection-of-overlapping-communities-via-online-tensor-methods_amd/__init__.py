@@ -71,7 +71,8 @@ class StpmConfigC(ctypes.Structure):
                 ("folder_name", ctypes.c_char * 512), ("file_prefix", ctypes.c_char * 64),
                 ("alpha_0_str", ctypes.c_char * 32), ("learn_rate_str", ctypes.c_char * 32),
                 ("minibatch", ctypes.c_int32), ("epochs", ctypes.c_int32), ("theta", ctypes.c_double),
-                ("renorm_every", ctypes.c_int32), ("ns_iters", ctypes.c_int32), ("precision", ctypes.c_int32),
+                ("renorm_every", ctypes.c_int32), ("ns_iters", ctypes.c_int32), ("ns_degree", ctypes.c_int32),
+                ("precision", ctypes.c_int32),
                 ("polish_epochs", ctypes.c_int32), ("oversample", ctypes.c_int32), ("power_iters", ctypes.c_int32),
                 ("m2_scaling", ctypes.c_int32), ("member_tau", ctypes.c_double), ("seed", ctypes.c_uint64),
                 ("ba_first_is_row", ctypes.c_int32), ("feof_repeat", ctypes.c_int32), ("device", ctypes.c_int32),

@@ -16,8 +16,8 @@ static void usage() {
   std::printf("usage: ./algo <option> [KEY=VALUE ...]\n");
   std::printf("choices for <option>:\n\t1) GPU_A\n\t2) GPU_B\n");
   std::printf("keys: NX NA NB NC NCOM ALPHA_0 LEARN_RATE THRESH PINV_TOL SGD_ITER_MIN SGD_ITER_MAX BINARY\n"
-              "      SYNTHETIC FOLDER_NAME FILE_PREFIX MINIBATCH EPOCHS THETA RENORM_EVERY NS_ITERS PRECISION\n"
-              "      POLISH_EPOCHS OVERSAMPLE POWER_ITERS M2_SCALING MEMBER_TAU SEED BA_FIRST_IS_ROW\n"
+              "      SYNTHETIC FOLDER_NAME FILE_PREFIX MINIBATCH EPOCHS THETA RENORM_EVERY NS_ITERS NS_DEGREE\n"
+              "      PRECISION POLISH_EPOCHS OVERSAMPLE POWER_ITERS M2_SCALING MEMBER_TAU SEED BA_FIRST_IS_ROW\n"
               "      FEOF_REPEAT VERBOSE USE_GRAPHS\n");
 }
 
@@ -47,6 +47,7 @@ static int set_field(stpm_config& c, const std::string& key, const std::string& 
   else if (key == "THETA") D(c.theta);
   else if (key == "RENORM_EVERY") I(c.renorm_every);
   else if (key == "NS_ITERS") I(c.ns_iters);
+  else if (key == "NS_DEGREE") I(c.ns_degree);
   else if (key == "PRECISION") c.precision = (val == "bf16" || val == "1") ? MMSB_PREC_BF16 : MMSB_PREC_FP32;
   else if (key == "POLISH_EPOCHS") I(c.polish_epochs);
   else if (key == "OVERSAMPLE") I(c.oversample);

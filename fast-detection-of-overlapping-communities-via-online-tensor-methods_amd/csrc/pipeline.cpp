@@ -99,9 +99,9 @@ struct mmsb_ctx {
   float* phi[2] = {nullptr, nullptr};
   int cur = 0;
   float *phit = nullptr, *gt = nullptr, *phia = nullptr, *h = nullptr, *cgt = nullptr, *cs_part = nullptr, *mab = nullptr;
-  float *ns_rowsum = nullptr, *mpart = nullptr;
+  float *ns_rowsum = nullptr, *mpart = nullptr, *nsq = nullptr;
   bf16_bits *phi16[2] = {nullptr, nullptr}, *phit16 = nullptr, *phia16 = nullptr, *phiT16 = nullptr, *phitT16 = nullptr,
-            *phiaT16 = nullptr, *gt16 = nullptr, *h16 = nullptr;
+            *phiaT16 = nullptr, *gt16 = nullptr, *h16 = nullptr, *nsq16 = nullptr;
   double* frob = nullptr;
   int64_t frob_cap = 0, steps_done = 0;
   bf16_bits *YA16 = nullptr, *YB16 = nullptr, *YCt16 = nullptr;
@@ -395,6 +395,7 @@ static int alloc_stgd(mmsb_ctx* c) {
   TRY(dalloc(c, c->phiT, kk));
   TRY(dalloc(c, c->phitT, kk));
   TRY(dalloc(c, c->phiaT, kk));
+  TRY(dalloc(c, c->nsq, kk));
   return MMSB_OK;
 }
 
@@ -416,7 +417,7 @@ static int prepare_bf16(mmsb_ctx* c) {
   TRY(dalloc(c, c->YCt16, ny));
   const size_t kk2 = (size_t)2 * c->kp * c->kp;
   for (bf16_bits** p : {&c->phi16[0], &c->phi16[1], &c->phit16, &c->phia16, &c->phiT16, &c->phitT16, &c->phiaT16, &c->gt16,
-                        &c->h16})
+                        &c->h16, &c->nsq16})
     TRY(dalloc(c, *p, kk2));
   if (!c->y16_valid) {
     TRY(f32_to_bf16(c->s, (int64_t)ny, c->YA, c->YA16));
@@ -480,6 +481,7 @@ static StgdBuffers stgd_bufs(const mmsb_ctx* c) {
   bufs.phitT = c->phitT;
   bufs.phiaT = c->phiaT;
   bufs.h = c->h;
+  bufs.q = c->nsq;
   bufs.phi16[0] = c->phi16[0];
   bufs.phi16[1] = c->phi16[1];
   bufs.phit16 = c->phit16;
@@ -489,6 +491,7 @@ static StgdBuffers stgd_bufs(const mmsb_ctx* c) {
   bufs.phiaT16 = c->phiaT16;
   bufs.gt16 = c->gt16;
   bufs.h16 = c->h16;
+  bufs.q16 = c->nsq16;
   bufs.cgt = c->cgt;
   bufs.rowsum = c->ns_rowsum;
   bufs.mpart = c->mpart;
@@ -514,7 +517,8 @@ static int run_stgd_steps(mmsb_ctx* c, int64_t nsteps, int32_t precision, StgdTi
   y_ptrs(c, precision, ya, yb, yc);
   const int64_t spe = (c->n[0] + c->bl - 1) / c->bl;
   auto launch = [&](int64_t step0, int64_t ns, double* frob_out, int& cur, StgdTiming* tm) -> int {
-    return stgd_run(c->s, precision, a, ya, yb, yc, step0, ns, c->cfg.renorm_every, c->cfg.ns_iters, bufs, cur, frob_out, tm);
+    return stgd_run(c->s, precision, a, ya, yb, yc, step0, ns, c->cfg.renorm_every, c->cfg.ns_iters, c->cfg.ns_degree, bufs,
+                    cur, frob_out, tm);
   };
   // Whole epochs that start at an epoch boundary replay a captured HIP graph of the epoch's
   // launches (one graph per precision and Phi ping-pong parity); anything else launches directly.
@@ -628,8 +632,9 @@ extern "C" int mmsb_batch_create(mmsb_ctx* const* ctxs, int32_t count, int32_t p
     MMSB_CHECK_ARG(c, "batch_create: null context %d", i);
     if (!c->have_y) return set_error(MMSB_ERR_STATE, "batch_create: context %d has no neighbour vectors", i);
     MMSB_CHECK_ARG(c->cfg.device == c0->cfg.device, "batch_create: contexts on different devices");
-    MMSB_CHECK_ARG(c->cfg.ns_iters == c0->cfg.ns_iters && c->cfg.renorm_every == c0->cfg.renorm_every,
-                   "batch_create: contexts disagree on ns_iters / renorm_every");
+    MMSB_CHECK_ARG(c->cfg.ns_iters == c0->cfg.ns_iters && c->cfg.ns_degree == c0->cfg.ns_degree &&
+                       c->cfg.renorm_every == c0->cfg.renorm_every,
+                   "batch_create: contexts disagree on ns_iters / ns_degree / renorm_every");
     const int64_t spe = steps_per_epoch(c);
     MMSB_CHECK_ARG(c->steps_done % spe == 0, "batch_create: context %d is not at an epoch boundary", i);
     MMSB_CHECK_ARG(c->cfg.renorm_every <= 1 || spe % c->cfg.renorm_every == 0,
@@ -671,7 +676,7 @@ extern "C" int mmsb_batch_create(mmsb_ctx* const* ctxs, int32_t count, int32_t p
     }
   }
   for (int p = 0; p < 2; ++p) {
-    const int st = stgd_batch_plan(b->plan[p], precision, mem[p], c0->cfg.ns_iters, c0->cfg.renorm_every);
+    const int st = stgd_batch_plan(b->plan[p], precision, mem[p], c0->cfg.ns_iters, c0->cfg.ns_degree, c0->cfg.renorm_every);
     if (st != MMSB_OK) return fail(st);
   }
   *out = b;
@@ -997,6 +1002,7 @@ extern "C" void stpm_config_default(stpm_config* c) {
   c->theta = 1.0;
   c->renorm_every = 1;
   c->ns_iters = 2;
+  c->ns_degree = 3;
   c->precision = MMSB_PREC_FP32;
   c->polish_epochs = 1;
   c->oversample = 10;
@@ -1020,6 +1026,7 @@ extern "C" int mmsb_create(const stpm_config* cfg, mmsb_ctx** out) {
   MMSB_CHECK_ARG(cfg->ncom + cfg->oversample <= std::min(std::min(cfg->na, cfg->nb), cfg->nc),
                  "create: k + oversample must not exceed the partition sizes");
   MMSB_CHECK_ARG(cfg->minibatch >= 1 && cfg->power_iters >= 0 && cfg->ns_iters >= 0, "create: bad STGD settings");
+  MMSB_CHECK_ARG(cfg->ns_degree == 3 || cfg->ns_degree == 5, "create: ns_degree must be 3 or 5 (got %d)", cfg->ns_degree);
   mmsb_ctx* c = new (std::nothrow) mmsb_ctx();
   if (!c) return set_error(MMSB_ERR_NOMEM, "create: out of host memory");
   c->cfg = *cfg;

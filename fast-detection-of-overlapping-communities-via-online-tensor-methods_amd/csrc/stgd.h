@@ -39,6 +39,7 @@ struct StgdBuffers {
   float* phitT;         // phit^T
   float* phiaT;         // phia^T
   float* h;             // (Phi^T Phi) o (Phi^T Phi)
+  float* q;             // degree-5 Newton-Schulz: the polynomial Q of the iteration's Gram matrix
   uint16_t* phi16[2];
   uint16_t* phit16;
   uint16_t* phia16;
@@ -47,6 +48,7 @@ struct StgdBuffers {
   uint16_t* phiaT16;
   uint16_t* gt16;
   uint16_t* h16;
+  uint16_t* q16;
   void* cgt;            // [kp x b] C_g^T: FP32, or BF16 in BF16 mode
   float* rowsum;        // [2 x kp] row sums of |X^T X| (Newton-Schulz scale guard)
   float* mpart;         // [kp/32 x 2 x kp] per-row-block partial sums of mu_A^T Phi, mu_B^T Phi
@@ -62,13 +64,15 @@ struct StgdTiming {
 };
 
 // Runs nsteps minibatch steps starting at global step step0 (batch = step mod steps-per-epoch).
+// Every renorm_every-th step renormalises Phi with ns_iters Newton-Schulz iterations of degree
+// ns_degree (3: X <- c X (3I - c^2 G)/2; 5: X <- c X (15I - 10 c^2 G + 3 c^4 G^2)/8).
 // bufs.phi[cur] holds Phi on entry; cur is flipped once per step and bufs.phi[cur] holds the
 // result on exit. frob_log[st] receives ||Phi_{t+1} - Phi_t||_F^2 of the st-th step of this call.
 // precision is MMSB_PREC_FP32 (Y as float) or MMSB_PREC_BF16 (Y as BF16). Without timing every
 // launch is capturable in a HIP graph (no memset nodes, no host synchronisation).
 int stgd_run(hipStream_t s, int32_t precision, const StgdArgs& args, const void* YA, const void* YB, const void* YCt,
-             int64_t step0, int64_t nsteps, int32_t renorm_every, int32_t ns_iters, const StgdBuffers& bufs, int& cur,
-             double* frob_log, StgdTiming* timing = nullptr);
+             int64_t step0, int64_t nsteps, int32_t renorm_every, int32_t ns_iters, int32_t ns_degree,
+             const StgdBuffers& bufs, int& cur, double* frob_log, StgdTiming* timing = nullptr);
 
 // ---- many instances in one persistent launch ----------------------------------------------------
 // One member of a batch: its step arguments, buffers and neighbour vectors (FP32 or BF16 as the
@@ -89,7 +93,7 @@ struct StgdBatch;
 // Lays out the tiles of every member's steps slot by slot and uploads the plan. The plan is fixed:
 // every launch replays the same steps (the members' cur must match at each launch).
 int stgd_batch_plan(StgdBatch*& out, int32_t precision, const std::vector<StgdBatchMember>& members, int32_t ns_iters,
-                    int32_t renorm_every);
+                    int32_t ns_degree, int32_t renorm_every);
 // One cooperative launch of the persistent kernel over the whole plan.
 int stgd_batch_launch(hipStream_t s, StgdBatch* b);
 // The same plan as one grouped launch per phase slot (capturable in a HIP graph).

@@ -6,11 +6,12 @@
 //       per-tile column sums of P_A o P_B;  and  H = (Phi^T Phi) o (Phi^T Phi)
 //   P2  Phi~ = Phi - eta theta Phi H + eta [Y_C[X_b]^T C_g / nb + mu_C c_mu^T],
 //       c_mu = c0 m_A o m_B + c1 sum_x(P_A o P_B) / nb                       (stpm.c:532-691)
-//   NS  G = X^T X; X <- c X (3I - c^2 G) / 2 with c = min(1, sqrt(2 / ||G||_inf)), ns_iters times
+//   NS  G = X^T X; X <- c X (3I - c^2 G) / 2 (degree 3), or Q = -(5/4) c^2 G + (3/8) c^4 G^2 and
+//       X <- c X (15/8 I + Q) (degree 5), with c = min(1, sqrt(2 / ||G||_inf)), ns_iters times
 //       (polar renormalisation in place of the QR of stpm.c:774-786). The polar factor is
 //       scale-invariant; c only acts when sigma_max(X)^2 may exceed 2, where the unscaled
-//       iteration (convergent for sigma < sqrt 3) would diverge -- e.g. after a minibatch that
-//       holds very high-degree nodes.
+//       iterations (convergent for sigma < sqrt 3, resp. sqrt(7/3)) would diverge -- e.g. after a
+//       minibatch that holds very high-degree nodes.
 // (c3, c1, c0) are the entries of the 4x4 matrices of init_buff (stpm.c:1026-1038).
 //
 // Work decomposition: every GEMM output is cut into 32 x 32 blocks, one per workgroup of four
@@ -145,8 +146,10 @@ struct Frag<MM_BF16X3> {
   }
 };
 
-// Ring depth per mode (fragment registers: 32 VGPRs per set for FP32 and BF16, 64 for split BF16);
-// deeper rings hide more load latency per wave at the cost of waves per SIMD.
+// Ring depth per mode (fragment registers: 32 VGPRs per set for FP32 and BF16, 64 for split BF16).
+// Two sets keep 16 (FP32, BF16) or 32 (split BF16) loads per wave in flight; deeper rings run into
+// the limit of 63 outstanding vector loads per wave and measured slower (P2: 9.0 -> 10.1 us in BF16
+// at depth 8).
 #ifndef MMSB_RING_F32
 #define MMSB_RING_F32 2
 #endif
@@ -246,6 +249,30 @@ __device__ inline void put4(float* f32, uint16_t* s16, int64_t nel, int64_t off,
   }
 }
 
+// Inputs of finish_tile that do not depend on the tile's products, loaded before the MFMA loop so
+// that their latency overlaps the operand loads: the previous iterate's block values (for the frob
+// log) and mu_A, mu_B of the thread's row (for the m partial sums).
+struct TilePre {
+  float pv[4];
+  float ua, ub;
+};
+
+__device__ inline void tile_pre(TilePre& t, int kp, int64_t i0, int64_t j0, const float* prev, const double* mu_a,
+                                const double* mu_b) {
+  const int r = threadIdx.x >> 3, c4 = 4 * (threadIdx.x & 7);
+  if (prev) {
+    const float4 p = *reinterpret_cast<const float4*>(prev + (i0 + r) * kp + j0 + c4);
+    t.pv[0] = p.x;
+    t.pv[1] = p.y;
+    t.pv[2] = p.z;
+    t.pv[3] = p.w;
+  }
+  if (mu_a) {
+    t.ua = (float)mu_a[i0 + r];
+    t.ub = (float)mu_b[i0 + r];
+  }
+}
+
 // Shared memory of the tile epilogue.
 struct TileShm {
   float t[32][33];    // the block transposed
@@ -255,13 +282,13 @@ struct TileShm {
 
 // Writes block (i0, j0) of a new kp x kp iterate held as v[e] = element (i0 + r, j0 + c4 + e), with
 // r = thread / 8 and c4 = 4 (thread % 8): Out row-major, OutT transposed (FP32 and, with SPLIT, BF16
-// pairs), frob += ||Out - prev||_F^2 over the block when prev is given, and, when mpart is given,
-// mpart[(ti * 2 + s) * kp + j] = sum over the block's rows i of mu_s[i] Out[i][j] (s = A, B).
-// Every thread of the block calls it.
+// pairs), frob += ||Out - prev||_F^2 over the block when frob is given (prev in pre.pv), and, when
+// mpart is given, mpart[(ti * 2 + s) * kp + j] = sum over the block's rows i of mu_s[i] Out[i][j]
+// (s = A, B; mu in pre.ua, pre.ub). Every thread of the block calls it.
 template <bool SPLIT>
 __device__ inline void finish_tile(TileShm& S, int kp, int ti, int64_t i0, int64_t j0, const float v[4], float* Out,
-                                   uint16_t* Out16, float* OutT, uint16_t* OutT16, const float* prev, double* frob,
-                                   const double* mu_a, const double* mu_b, float* mpart) {
+                                   uint16_t* Out16, float* OutT, uint16_t* OutT16, const TilePre& pre, double* frob,
+                                   float* mpart) {
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int r = threadIdx.x >> 3, c4 = 4 * (threadIdx.x & 7);
   const int64_t nel = (int64_t)kp * kp;
@@ -269,25 +296,22 @@ __device__ inline void finish_tile(TileShm& S, int kp, int ti, int64_t i0, int64
   put4<SPLIT>(Out, Out16, nel, off, v);
 #pragma unroll
   for (int e = 0; e < 4; ++e) S.t[c4 + e][r] = v[e];
-  if (prev) {
-    const float4 p = *reinterpret_cast<const float4*>(prev + off);
-    const float pv[4] = {p.x, p.y, p.z, p.w};
+  if (frob) {
     double d = 0.0;
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
-      const double dd = (double)v[e] - (double)pv[e];
+      const double dd = (double)v[e] - (double)pre.pv[e];
       d += dd * dd;
     }
     for (int o = 32; o > 0; o >>= 1) d += __shfl_xor(d, o);
     if (lane == 0) S.d[w] = d;
   }
   if (mpart) {
-    const float ua = (float)mu_a[i0 + r], ub = (float)mu_b[i0 + r];
     float sa[4], sb[4];
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
-      sa[e] = ua * v[e];
-      sb[e] = ub * v[e];
+      sa[e] = pre.ua * v[e];
+      sb[e] = pre.ub * v[e];
       for (int o = 8; o < 64; o <<= 1) {  // lanes with equal lane % 8 hold the same columns
         sa[e] += __shfl_xor(sa[e], o);
         sb[e] += __shfl_xor(sb[e], o);
@@ -307,7 +331,7 @@ __device__ inline void finish_tile(TileShm& S, int kp, int ti, int64_t i0, int64
     const float t[4] = {S.t[c][r4], S.t[c][r4 + 1], S.t[c][r4 + 2], S.t[c][r4 + 3]};
     put4<SPLIT>(OutT, OutT16, nel, (j0 + c) * kp + i0 + r4, t);
   }
-  if (prev && threadIdx.x == 0) atomicAdd(frob, (S.d[0] + S.d[1]) + (S.d[2] + S.d[3]));
+  if (frob && threadIdx.x == 0) atomicAdd(frob, (S.d[0] + S.d[1]) + (S.d[2] + S.d[3]));
   if (mpart && threadIdx.x < 64) {
     const int s = threadIdx.x >> 5, c = threadIdx.x & 31;
     mpart[((int64_t)ti * 2 + s) * kp + j0 + c] = (S.m[0][s][c] + S.m[1][s][c]) + (S.m[2][s][c] + S.m[3][s][c]);
@@ -327,11 +351,12 @@ __device__ inline void store4(T* p, const float v[4]) {
 }
 
 // ---- the phases of one step, as per-tile device functions ----------------------------------------
-// A step is the phase sequence P1, P2, then ns_iters x (GRAM, UPDATE), or COPY when the step does not
-// renormalise; SETUP precedes the first step of a run. The same tile functions serve the per-phase
+// A step is the phase sequence P1, P2, then ns_iters x (GRAM, UPDATE) -- (GRAM, QPOLY, UPDATE) for
+// degree-5 iterations -- or COPY when the step does not renormalise; SETUP precedes the first step of
+// a run. The same tile functions serve the per-phase
 // launches of one instance (stgd_phase_kernel, blockIdx.x = tile) and the persistent launch over many
 // instances (stgd_batch_kernel, tiles from a work queue).
-enum StgdPhase { PH_SETUP = 0, PH_P1 = 1, PH_P2 = 2, PH_GRAM = 3, PH_UPDATE = 4, PH_COPY = 5 };
+enum StgdPhase { PH_SETUP = 0, PH_P1 = 1, PH_P2 = 2, PH_GRAM = 3, PH_UPDATE = 4, PH_COPY = 5, PH_QPOLY = 6 };
 
 // Everything the phases of one step of one instance read or write.
 struct StepCtx {
@@ -342,6 +367,7 @@ struct StepCtx {
   const void* YCt;
   int32_t cur;         // B.phi[cur] holds the step's input Phi
   int32_t ns_iters;
+  int32_t ns_degree;   // 3 or 5
   int64_t row0;        // first X row of the minibatch
   int32_t nb;          // rows in the minibatch
   double* frob;        // this step's ||Phi_{t+1} - Phi_t||_F^2
@@ -553,6 +579,9 @@ __device__ void p2_tile(const StepCtx& P, int tile, StgdSmem& sm) {
   PartSum ps;  // column sums of P_A o P_B and m_A, m_B for c_mu, gathered during the MFMA loop
   cspart_load(ps, a.cs_part, kp, a.b / 32, j0);
   const float mab_a = a.mab[j0 + (threadIdx.x >> 3)], mab_b = a.mab[kp + j0 + (threadIdx.x >> 3)];
+  const int r = threadIdx.x >> 3, c4 = 4 * (threadIdx.x & 7);
+  const float muc = (float)a.mu_c[i0 + r];  // epilogue inputs, loaded ahead of the MFMA loop
+  const float4 ph = *reinterpret_cast<const float4*>(a.phi + (i0 + r) * kp + j0 + c4);
   if (w < 2) {  // Phi H over two halves of k (H symmetric: row j of H is column j)
     kslice<KMODE>(kp, 2, w, k0, kl);
     wave_mm<KMODE, true>(acc, pick<KMODE>(a.phi, P.B.phi16[P.cur], nel, kp), pick<KMODE>(P.B.h, P.B.h16, nel, kp), i0, j0,
@@ -576,9 +605,6 @@ __device__ void p2_tile(const StepCtx& P, int tile, StgdSmem& sm) {
   red_put(red, acc);
   __syncthreads();
   const float eta = a.eta, inv_b = 1.f / (float)P.nb;
-  const int r = threadIdx.x >> 3, c4 = 4 * (threadIdx.x & 7);
-  const float muc = (float)a.mu_c[i0 + r];
-  const float4 ph = *reinterpret_cast<const float4*>(a.phi + (i0 + r) * kp + j0 + c4);
   const float p0[4] = {ph.x, ph.y, ph.z, ph.w};
   float v[4];
 #pragma unroll
@@ -588,8 +614,27 @@ __device__ void p2_tile(const StepCtx& P, int tile, StgdSmem& sm) {
     const float t = red_get(red, 2, r, c) + red_get(red, 3, r, c);
     v[e] = p0[e] - eta * a.theta * phh + eta * (t * inv_b + muc * sm.p2.cmu[c]);
   }
-  finish_tile<SPLIT>(sm.p2.S, kp, ti, i0, j0, v, a.phit, sp<SPLIT>(P.B.phit16), P.B.phitT, sp<SPLIT>(P.B.phitT16), nullptr,
-                     nullptr, nullptr, nullptr, nullptr);
+  finish_tile<SPLIT>(sm.p2.S, kp, ti, i0, j0, v, a.phit, sp<SPLIT>(P.B.phit16), P.B.phitT, sp<SPLIT>(P.B.phitT16),
+                     TilePre{}, nullptr, nullptr);
+}
+
+// The Newton-Schulz scale guard c = min(1, sqrt(2 / ||G||_inf)) from the row sums of |G|, computed
+// by wave 3 of a block: guard_load issues the loads before the block's MFMA loop, guard_scale
+// reduces them after it and leaves c in *cscale (shared memory; read after the next barrier).
+__device__ inline void guard_load(float rs[8], const float* rowsum, int kp) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int u = 0; u < 8; ++u) rs[u] = lane + 64 * u < kp ? rowsum[lane + 64 * u] : 0.f;
+}
+
+__device__ inline void guard_scale(const float rs[8], const float* rowsum, int kp, float* cscale) {
+  const int lane = threadIdx.x & 63;
+  float mx = 0.f;
+#pragma unroll
+  for (int u = 0; u < 8; ++u) mx = fmaxf(mx, rs[u]);
+  for (int t = lane + 512; t < kp; t += 64) mx = fmaxf(mx, rowsum[t]);
+  for (int off = 32; off > 0; off >>= 1) mx = fmaxf(mx, __shfl_xor(mx, off));
+  if (lane == 0) *cscale = mx > 2.f ? sqrtf(2.f / mx) : 1.f;
 }
 
 // G = X^T X of Newton-Schulz iteration `it`, and rowsum[i] += sum over the tile's columns of |G_ij|.
@@ -623,14 +668,47 @@ __device__ void gram_tile(const StepCtx& P, int it, int tile, StgdSmem& sm) {
   if ((threadIdx.x & 7) == 0) atomicAdd(rowsum + i0 + r, s);
 }
 
-// Xout = c (1.5 X - 0.5 c^2 X G); the outputs are those of finish_tile (the last iteration also
-// logs ||Phi_next - Phi||^2 and the partial sums of m_A, m_B).
+// Degree-5 iteration, second stage: Q = -(5/4) c^2 G + (3/8) c^4 G^2 (symmetric, as G), so that the
+// update is Xout = c X (15/8 I + Q).
+template <int KMODE>
+__device__ void qpoly_tile(const StepCtx& P, int it, int tile, StgdSmem& sm) {
+  constexpr bool SPLIT = KMODE == MM_BF16X3;
+  const int kp = P.a.kp, ntk = kp / 32;
+  const int64_t nel = (int64_t)kp * kp;
+  const int w = threadIdx.x >> 6;
+  const int ti = tile / ntk, tj = tile % ntk;
+  const int64_t i0 = (int64_t)ti * 32, j0 = (int64_t)tj * 32;
+  const float* rowsum = P.B.rowsum + (it & 1) * kp;
+  Red4& red = sm.p2.red;
+  floatx16 acc = {};
+  int64_t k0, kl;
+  kslice<KMODE>(kp, 4, w, k0, kl);
+  float rs[8];
+  if (w == 3) guard_load(rs, rowsum, kp);
+  const int r = threadIdx.x >> 3, c4 = 4 * (threadIdx.x & 7);
+  const float4 gv = *reinterpret_cast<const float4*>(P.B.gt + (i0 + r) * kp + j0 + c4);
+  const Opnd<op_t<KMODE>> g = pick<KMODE>(P.B.gt, P.B.gt16, nel, kp);
+  wave_mm<KMODE, true>(acc, g, g, i0, j0, k0, kl);  // (G G)_ij = G[i,:] . G[j,:] (G symmetric)
+  if (w == 3) guard_scale(rs, rowsum, kp, &sm.p2.cscale);
+  red_put(red, acc);
+  __syncthreads();
+  const float c2 = sm.p2.cscale * sm.p2.cscale;
+  const float a1 = -1.25f * c2, a2 = 0.375f * c2 * c2;
+  const float gg[4] = {gv.x, gv.y, gv.z, gv.w};
+  float v[4];
+#pragma unroll
+  for (int e = 0; e < 4; ++e) v[e] = a1 * gg[e] + a2 * red_sum4(red, r, c4 + e);
+  put4<SPLIT>(P.B.q, sp<SPLIT>(P.B.q16), nel, (i0 + r) * kp + j0 + c4, v);
+}
+
+// Degree 3: Xout = c (1.5 X - 0.5 c^2 X G); degree 5: Xout = c (1.875 X + X Q). The outputs are those
+// of finish_tile (the last iteration also logs ||Phi_next - Phi||^2 and the partial sums of m_A, m_B).
 template <int KMODE>
 __device__ void update_tile(const StepCtx& P, int it, int tile, StgdSmem& sm) {
   constexpr bool SPLIT = KMODE == MM_BF16X3;
   const int kp = P.a.kp, ntk = kp / 32;
   const int64_t nel = (int64_t)kp * kp;
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int w = threadIdx.x >> 6;
   const int ti = tile / ntk, tj = tile % ntk;
   const int64_t i0 = (int64_t)ti * 32, j0 = (int64_t)tj * 32;
   const NsBufs nb = ns_bufs(P.B, P.cur, it, P.ns_iters);
@@ -641,35 +719,29 @@ __device__ void update_tile(const StepCtx& P, int it, int tile, StgdSmem& sm) {
   floatx16 acc = {};
   int64_t k0, kl;
   kslice<KMODE>(kp, 4, w, k0, kl);
-  float rs[8];  // wave 3: ||G||_inf from the row sums of |G|, loads issued before the MFMA loop
-  if (w == 3) {
-#pragma unroll
-    for (int u = 0; u < 8; ++u) rs[u] = lane + 64 * u < kp ? rowsum[lane + 64 * u] : 0.f;
-  }
-  wave_mm<KMODE, true>(acc, pick<KMODE>(nb.x, nb.x16, nel, kp), pick<KMODE>(P.B.gt, P.B.gt16, nel, kp), i0, j0, k0,
-                       kl);  // X G (G symmetric)
-  if (w == 3) {
-    float mx = 0.f;
-#pragma unroll
-    for (int u = 0; u < 8; ++u) mx = fmaxf(mx, rs[u]);
-    for (int t = lane + 512; t < kp; t += 64) mx = fmaxf(mx, rowsum[t]);
-    for (int off = 32; off > 0; off >>= 1) mx = fmaxf(mx, __shfl_xor(mx, off));
-    if (lane == 0) sm.p2.cscale = mx > 2.f ? sqrtf(2.f / mx) : 1.f;
-  }
+  float rs[8];
+  if (w == 3) guard_load(rs, rowsum, kp);
+  const bool quintic = P.ns_degree == 5;
+  const int r = threadIdx.x >> 3, c4 = 4 * (threadIdx.x & 7);
+  const float4 xv = *reinterpret_cast<const float4*>(nb.x + (i0 + r) * kp + j0 + c4);
+  TilePre pre;
+  tile_pre(pre, kp, i0, j0, last ? P.a.phi : nullptr, last ? P.a.mu_a : nullptr, P.a.mu_b);
+  wave_mm<KMODE, true>(acc, pick<KMODE>(nb.x, nb.x16, nel, kp),
+                       quintic ? pick<KMODE>(P.B.q, P.B.q16, nel, kp) : pick<KMODE>(P.B.gt, P.B.gt16, nel, kp), i0, j0,
+                       k0, kl);  // X G or X Q (both symmetric)
+  if (w == 3) guard_scale(rs, rowsum, kp, &sm.p2.cscale);
   if (tile == 0)
     for (int t = threadIdx.x; t < kp; t += blockDim.x) rowsum_next[t] = 0.f;
   red_put(red, acc);
   __syncthreads();
   const float cs = sm.p2.cscale;
-  const float c1 = 1.5f * cs, c3 = 0.5f * cs * cs * cs;
-  const int r = threadIdx.x >> 3, c4 = 4 * (threadIdx.x & 7);
-  const float4 xv = *reinterpret_cast<const float4*>(nb.x + (i0 + r) * kp + j0 + c4);
+  const float c1 = (quintic ? 1.875f : 1.5f) * cs, c3 = quintic ? cs : -0.5f * cs * cs * cs;
   const float x[4] = {xv.x, xv.y, xv.z, xv.w};
   float v[4];
 #pragma unroll
-  for (int e = 0; e < 4; ++e) v[e] = c1 * x[e] - c3 * red_sum4(red, r, c4 + e);
-  finish_tile<SPLIT>(sm.p2.S, kp, ti, i0, j0, v, nb.dst, sp<SPLIT>(nb.dst16), nb.dstT, sp<SPLIT>(nb.dstT16),
-                     last ? P.a.phi : nullptr, P.frob, P.a.mu_a, P.a.mu_b, last ? P.B.mpart : nullptr);
+  for (int e = 0; e < 4; ++e) v[e] = c1 * x[e] + c3 * red_sum4(red, r, c4 + e);
+  finish_tile<SPLIT>(sm.p2.S, kp, ti, i0, j0, v, nb.dst, sp<SPLIT>(nb.dst16), nb.dstT, sp<SPLIT>(nb.dstT16), pre,
+                     last ? P.frob : nullptr, last ? P.B.mpart : nullptr);
 }
 
 // COPY: Phi_next = Phi~ with the outputs of a last update. SETUP: the derived copies of the
@@ -685,12 +757,14 @@ __device__ void copy_tile(const StepCtx& P, bool setup, int tile, StgdSmem& sm) 
   const float v[4] = {xv.x, xv.y, xv.z, xv.w};
   if (setup && tile == 0)
     for (int64_t t = threadIdx.x; t < P.nfrob; t += blockDim.x) P.frob_zero[t] = 0.0;
+  TilePre pre;
+  tile_pre(pre, kp, i0, j0, setup ? nullptr : P.a.phi, P.a.mu_a, P.a.mu_b);
   if (setup)
     finish_tile<SPLIT>(sm.S, kp, ti, i0, j0, v, nullptr, sp<SPLIT>(P.B.phi16[P.cur]), P.B.phiT, sp<SPLIT>(P.B.phiT16),
-                       nullptr, nullptr, P.a.mu_a, P.a.mu_b, P.B.mpart);
+                       pre, nullptr, P.B.mpart);
   else
     finish_tile<SPLIT>(sm.S, kp, ti, i0, j0, v, P.B.phi[P.cur ^ 1], sp<SPLIT>(P.B.phi16[P.cur ^ 1]), P.B.phiT,
-                       sp<SPLIT>(P.B.phiT16), P.a.phi, P.frob, P.a.mu_a, P.a.mu_b, P.B.mpart);
+                       sp<SPLIT>(P.B.phiT16), pre, P.frob, P.B.mpart);
 }
 
 template <int DMODE, int KMODE, typename TY, typename TPH>
@@ -702,12 +776,15 @@ __device__ void run_tile(const StepCtx& P, int phase, int it, int tile, StgdSmem
     case PH_P2: p2_tile<DMODE, KMODE, TY, TPH>(P, tile, sm); break;
     case PH_GRAM: gram_tile<KMODE>(P, it, tile, sm); break;
     case PH_UPDATE: update_tile<KMODE>(P, it, tile, sm); break;
+    case PH_QPOLY: qpoly_tile<KMODE>(P, it, tile, sm); break;
     default: copy_tile<SPLIT>(P, false, tile, sm); break;
   }
 }
 
 // One phase of one step, one kernel per phase (each with its own register allocation):
-// blockIdx.x = tile.
+// blockIdx.x = tile. (An XCD-aware order that gave each XCD a compact group of tiles cut the HBM
+// fetches of P1 from 24 to 9 MiB per launch, but the step took 2.6 % longer: the waves of a group
+// then request the same lines at the same moment.)
 template <int DMODE, int KMODE, typename TY, typename TPH, int PHASE>
 __global__ __launch_bounds__(256) void stgd_phase_kernel(StepCtx P, int32_t it) {
   __shared__ StgdSmem sm;
@@ -717,6 +794,7 @@ __global__ __launch_bounds__(256) void stgd_phase_kernel(StepCtx P, int32_t it) 
   else if constexpr (PHASE == PH_P2) p2_tile<DMODE, KMODE, TY, TPH>(P, blockIdx.x, sm);
   else if constexpr (PHASE == PH_GRAM) gram_tile<KMODE>(P, it, blockIdx.x, sm);
   else if constexpr (PHASE == PH_UPDATE) update_tile<KMODE>(P, it, blockIdx.x, sm);
+  else if constexpr (PHASE == PH_QPOLY) qpoly_tile<KMODE>(P, it, blockIdx.x, sm);
   else copy_tile<SPLIT>(P, false, blockIdx.x, sm);
 }
 
@@ -743,6 +821,7 @@ struct BatchInst {
   const void* YCt;
   int32_t cur0;       // phi[cur0] holds Phi at the start of the launch
   int32_t ns_iters;
+  int32_t ns_degree;
   int64_t step0;      // global index of the launch's first step
   int64_t spe;        // steps per epoch
   double* frob_log;   // [steps of the launch]
@@ -757,6 +836,7 @@ __device__ inline void make_step(const BatchInst& I, int64_t st, StepCtx& P) {
   P.YCt = I.YCt;
   P.cur = I.cur0 ^ (int)(st & 1);
   P.ns_iters = I.ns_iters;
+  P.ns_degree = I.ns_degree;
   const int64_t g = I.step0 + st;
   P.row0 = (g % I.spe) * I.a.bl;
   P.nb = (int32_t)min((int64_t)I.a.bl, I.a.n_x - P.row0);
@@ -846,8 +926,8 @@ static bool step_renorms(int64_t gstep, int32_t renorm_every, int32_t ns_iters) 
 
 template <int DMODE, int KMODE, typename TY, typename TPH>
 static int run_steps(hipStream_t s, const StgdArgs& args, const void* YA, const void* YB, const void* YCt, int64_t step0,
-                     int64_t nsteps, int32_t renorm_every, int32_t ns_iters, const StgdBuffers& B, int& cur,
-                     double* frob_log, StgdTiming* timing) {
+                     int64_t nsteps, int32_t renorm_every, int32_t ns_iters, int32_t ns_degree, const StgdBuffers& B,
+                     int& cur, double* frob_log, StgdTiming* timing) {
   const int kp = args.kp;
   const int64_t spe = ceil_div(args.n_x, args.bl);
   StepCtx P{};
@@ -857,6 +937,7 @@ static int run_steps(hipStream_t s, const StgdArgs& args, const void* YA, const 
   P.YB = YB;
   P.YCt = YCt;
   P.ns_iters = ns_iters;
+  P.ns_degree = ns_degree;
   auto launch = [&](int phase, int it) -> int {
     const dim3 grid(phase_tiles(phase, kp, args.b)), block(256);
     switch (phase) {
@@ -865,6 +946,7 @@ static int run_steps(hipStream_t s, const StgdArgs& args, const void* YA, const 
       case PH_P2: hipLaunchKernelGGL((stgd_phase_kernel<DMODE, KMODE, TY, TPH, PH_P2>), grid, block, 0, s, P, it); break;
       case PH_GRAM: hipLaunchKernelGGL((stgd_phase_kernel<DMODE, KMODE, TY, TPH, PH_GRAM>), grid, block, 0, s, P, it); break;
       case PH_UPDATE: hipLaunchKernelGGL((stgd_phase_kernel<DMODE, KMODE, TY, TPH, PH_UPDATE>), grid, block, 0, s, P, it); break;
+      case PH_QPOLY: hipLaunchKernelGGL((stgd_phase_kernel<DMODE, KMODE, TY, TPH, PH_QPOLY>), grid, block, 0, s, P, it); break;
       default: hipLaunchKernelGGL((stgd_phase_kernel<DMODE, KMODE, TY, TPH, PH_COPY>), grid, block, 0, s, P, it); break;
     }
     MMSB_LAUNCH_CHECK();
@@ -891,6 +973,7 @@ static int run_steps(hipStream_t s, const StgdArgs& args, const void* YA, const 
     if (step_renorms(gstep, renorm_every, ns_iters)) {
       for (int it = 0; it < ns_iters; ++it) {
         MMSB_TRY(launch(PH_GRAM, it));
+        if (ns_degree == 5) MMSB_TRY(launch(PH_QPOLY, it));
         MMSB_TRY(launch(PH_UPDATE, it));
       }
     } else {
@@ -910,7 +993,7 @@ static int run_steps(hipStream_t s, const StgdArgs& args, const void* YA, const 
   return MMSB_OK;
 }
 
-static int check_args(const StgdArgs& args, int32_t ns_iters) {
+static int check_args(const StgdArgs& args, int32_t ns_iters, int32_t ns_degree) {
   MMSB_CHECK_ARG(args.kp % 64 == 0 && args.b % 64 == 0 && args.kp >= 64 && args.b >= 64,
                  "stgd: kp and b must be positive multiples of 64 (got %d, %d)", args.kp, args.b);
   MMSB_CHECK_ARG(args.bl >= 1 && args.bl <= args.b, "stgd: logical minibatch %d outside [1, %d]", args.bl, args.b);
@@ -918,30 +1001,31 @@ static int check_args(const StgdArgs& args, int32_t ns_iters) {
   const int64_t spe = ceil_div(args.n_x, args.bl);
   MMSB_CHECK_ARG(args.n_pad >= (spe - 1) * args.bl + args.b, "stgd: n_pad too small");
   MMSB_CHECK_ARG(ns_iters >= 0, "stgd: negative iteration count");
+  MMSB_CHECK_ARG(ns_degree == 3 || ns_degree == 5, "stgd: Newton-Schulz degree must be 3 or 5 (got %d)", ns_degree);
   return MMSB_OK;
 }
 
 static int check_split(const StgdBuffers& bufs) {
   MMSB_CHECK_ARG(bufs.phi16[0] && bufs.phi16[1] && bufs.phit16 && bufs.phia16 && bufs.phiT16 && bufs.phitT16 &&
-                     bufs.phiaT16 && bufs.gt16 && bufs.h16,
+                     bufs.phiaT16 && bufs.gt16 && bufs.h16 && bufs.q16,
                  "stgd: BF16 mode needs the split operand buffers");
   return MMSB_OK;
 }
 
 int stgd_run(hipStream_t s, int32_t precision, const StgdArgs& args, const void* YA, const void* YB, const void* YCt,
-             int64_t step0, int64_t nsteps, int32_t renorm_every, int32_t ns_iters, const StgdBuffers& bufs, int& cur,
-             double* frob_log, StgdTiming* timing) {
-  MMSB_TRY(check_args(args, ns_iters));
+             int64_t step0, int64_t nsteps, int32_t renorm_every, int32_t ns_iters, int32_t ns_degree,
+             const StgdBuffers& bufs, int& cur, double* frob_log, StgdTiming* timing) {
+  MMSB_TRY(check_args(args, ns_iters, ns_degree));
   MMSB_CHECK_ARG(nsteps >= 0, "stgd: negative step count");
   if (nsteps == 0) return MMSB_OK;
   if (precision == MMSB_PREC_BF16) {
     MMSB_TRY(check_split(bufs));
-    return run_steps<MM_BF16, MM_BF16X3, __bf16, __bf16>(s, args, YA, YB, YCt, step0, nsteps, renorm_every, ns_iters, bufs,
-                                                         cur, frob_log, timing);
+    return run_steps<MM_BF16, MM_BF16X3, __bf16, __bf16>(s, args, YA, YB, YCt, step0, nsteps, renorm_every, ns_iters,
+                                                         ns_degree, bufs, cur, frob_log, timing);
   }
   MMSB_CHECK_ARG(precision == MMSB_PREC_FP32, "stgd: unknown precision %d", precision);
-  return run_steps<MM_F32, MM_F32, float, float>(s, args, YA, YB, YCt, step0, nsteps, renorm_every, ns_iters, bufs, cur,
-                                                 frob_log, timing);
+  return run_steps<MM_F32, MM_F32, float, float>(s, args, YA, YB, YCt, step0, nsteps, renorm_every, ns_iters, ns_degree,
+                                                 bufs, cur, frob_log, timing);
 }
 
 // ---- batch ------------------------------------------------------------------------------------------
@@ -969,7 +1053,7 @@ static const void* batch_kernel(int32_t precision) {
 }
 
 int stgd_batch_plan(StgdBatch*& out, int32_t precision, const std::vector<StgdBatchMember>& members, int32_t ns_iters,
-                    int32_t renorm_every) {
+                    int32_t ns_degree, int32_t renorm_every) {
   out = nullptr;
   MMSB_CHECK_ARG(!members.empty(), "stgd batch: no instances");
   MMSB_CHECK_ARG(precision == MMSB_PREC_FP32 || precision == MMSB_PREC_BF16, "stgd batch: unknown precision %d", precision);
@@ -977,7 +1061,7 @@ int stgd_batch_plan(StgdBatch*& out, int32_t precision, const std::vector<StgdBa
   std::vector<std::vector<BatchSeg>> per(members.size());
   for (size_t i = 0; i < members.size(); ++i) {
     const StgdBatchMember& m = members[i];
-    MMSB_TRY(check_args(m.args, ns_iters));
+    MMSB_TRY(check_args(m.args, ns_iters, ns_degree));
     if (precision == MMSB_PREC_BF16) MMSB_TRY(check_split(m.bufs));
     MMSB_CHECK_ARG(m.nsteps >= 1, "stgd batch: instance %zu has no steps", i);
     BatchInst& I = insts[i];
@@ -988,6 +1072,7 @@ int stgd_batch_plan(StgdBatch*& out, int32_t precision, const std::vector<StgdBa
     I.YCt = m.YCt;
     I.cur0 = m.cur;
     I.ns_iters = ns_iters;
+    I.ns_degree = ns_degree;
     I.step0 = m.step0;
     I.spe = ceil_div(m.args.n_x, m.args.bl);
     I.frob_log = m.frob_log;
@@ -1008,6 +1093,7 @@ int stgd_batch_plan(StgdBatch*& out, int32_t precision, const std::vector<StgdBa
       if (step_renorms(m.step0 + st, renorm_every, ns_iters)) {
         for (int it = 0; it < ns_iters; ++it) {
           add(PH_GRAM, it, st);
+          if (ns_degree == 5) add(PH_QPOLY, it, st);
           add(PH_UPDATE, it, st);
         }
       } else {

@@ -187,6 +187,7 @@ typedef struct stpm_config {
   double theta;               /* weight of the orthogonality term (reference GPU: 1)            */
   int32_t renorm_every;       /* orthonormal renormalisation every n steps (reference: 1), 0 = never */
   int32_t ns_iters;           /* Newton-Schulz iterations per renormalisation                    */
+  int32_t ns_degree;          /* 3: X <- X (3I - G)/2; 5: X <- X (15I - 10G + 3G^2)/8 (G = X^T X)  */
   int32_t precision;          /* MMSB_PREC_*: STGD contraction precision                         */
   int32_t polish_epochs;      /* FP32 epochs that follow BF16 epochs                             */
   int32_t oversample;         /* r - k of the randomized range finders                           */
@@ -203,7 +204,7 @@ typedef struct stpm_config {
 
 /* Fills the reference defaults of stpm.h:47-106 (DBLP sizes, NCOM 100, ALPHA_0 1, LEARN_RATE 1e-3,
  * THRESH 1, PINV_TOL 1e-6, BINARY, SYNTHETIC) plus: minibatch 1024, 1 epoch, theta 1, renorm every
- * step with 2 Newton-Schulz iterations, FP32, oversampling 10, 1 power iteration, paper M2
+ * step with 2 degree-3 Newton-Schulz iterations, FP32, oversampling 10, 1 power iteration, paper M2
  * scaling, tau 0, seed 0, CPU file orientation. */
 void stpm_config_default(stpm_config* cfg);
 

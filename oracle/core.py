@@ -260,36 +260,46 @@ def t_hat_c(phi, YA, YB, YC, muA, muB, muC, coeffs):
     return YC.T @ cg / YA.shape[0] + np.outer(muC, cmu)
 
 
-def polar_renorm(phi, ns_iters=2):
-    """Orthonormal renormalisation by Newton-Schulz iterations X <- c X (3I - c^2 X^T X)/2, which
-    converge quadratically to the polar factor X (X^T X)^-1/2. The scale c = min(1, sqrt(2/||G||_inf))
-    (G = X^T X, infinity norm = max row sum of |G|, an upper bound of sigma_max(X)^2) keeps every
-    singular value of cX below sqrt(2) < sqrt(3), inside the region where the iteration converges;
-    the polar factor is scale-invariant, so c changes nothing when it is 1 and only rescues
-    iterates that would otherwise diverge. The reference re-orthonormalises every step with a QR
+def polar_renorm(phi, ns_iters=2, degree=3):
+    """Orthonormal renormalisation by Newton-Schulz iterations towards the polar factor
+    X (X^T X)^-1/2, with G = X^T X:
+      degree 3: X <- c X (3I - c^2 G)/2                         (quadratic: 1 + d -> 1 - 1.5 d^2)
+      degree 5: X <- c X (15I - 10 c^2 G + 3 c^4 G^2)/8          (cubic: 1 + d -> 1 + 2.5 d^3)
+    Both map each singular value s of cX through a polynomial (s (3 - s^2)/2, s (15 - 10 s^2 +
+    3 s^4)/8) whose fixed point 1 attracts (0, sqrt(3)) and (0, sqrt(7/3)) respectively. The scale
+    c = min(1, sqrt(2/||G||_inf)) (infinity norm = max row sum of |G|, an upper bound of
+    sigma_max(X)^2) keeps every singular value of cX at or below sqrt(2), inside both regions; the
+    polar factor is scale-invariant, so c changes nothing when it is 1 and only rescues iterates
+    that would otherwise diverge. The reference re-orthonormalises every step with a QR
     factorisation instead (stpm.c:774-786)."""
+    if degree not in (3, 5):
+        raise ValueError("Newton-Schulz degree must be 3 or 5")
     I = np.eye(phi.shape[1])
     for _ in range(ns_iters):
         G = phi.T @ phi
         ginf = np.abs(G).sum(1).max()
         c = np.sqrt(2.0 / ginf) if ginf > 2.0 else 1.0
-        phi = c * phi @ (1.5 * I - 0.5 * c * c * G)
+        if degree == 3:
+            phi = c * phi @ (1.5 * I - 0.5 * c * c * G)
+        else:
+            Q = -1.25 * c * c * G + 0.375 * c ** 4 * (G @ G)
+            phi = c * phi @ (1.875 * I + Q)
     return phi
 
 
-def stgd_step(phi, YA, YB, YC, muA, muB, muC, alpha0, eta, theta=1.0, renorm=True, ns_iters=2):
+def stgd_step(phi, YA, YB, YC, muA, muB, muC, alpha0, eta, theta=1.0, renorm=True, ns_iters=2, ns_degree=3):
     """Phi <- Phi - eta [theta Phi (Phi^T Phi)^o2 - T(Phi, Phi, I)] (stpm.c:649-691), then the
     orthonormal renormalisation."""
     G = phi.T @ phi
     T = t_hat_c(phi, YA, YB, YC, muA, muB, muC, gpu_coeffs(alpha0))
     new = phi - eta * (theta * phi @ (G * G) - T)
     if renorm:
-        new = polar_renorm(new, ns_iters)
+        new = polar_renorm(new, ns_iters, ns_degree)
     return new
 
 
 def stgd(Y_A, Y_B, Y_C, mu_A, mu_B, mu_C, phi0, alpha0, eta, batch, epochs, theta=1.0,
-         renorm_every=1, steps=None, ns_iters=2):
+         renorm_every=1, steps=None, ns_iters=2, ns_degree=3):
     """Sequential minibatches over X (stpm.c:499-502,768); returns Phi and ||dPhi||_F^2 per step."""
     phi = phi0.copy()
     n = Y_A.shape[0]
@@ -300,7 +310,8 @@ def stgd(Y_A, Y_B, Y_C, mu_A, mu_B, mu_C, phi0, alpha0, eta, batch, epochs, thet
         lo = (s % ((n + batch - 1) // batch)) * batch
         hi = min(lo + batch, n)
         renorm = renorm_every > 0 and ((step + 1) % renorm_every == 0)
-        new = stgd_step(phi, Y_A[lo:hi], Y_B[lo:hi], Y_C[lo:hi], mu_A, mu_B, mu_C, alpha0, eta, theta, renorm, ns_iters)
+        new = stgd_step(phi, Y_A[lo:hi], Y_B[lo:hi], Y_C[lo:hi], mu_A, mu_B, mu_C, alpha0, eta, theta, renorm, ns_iters,
+                        ns_degree)
         log.append(float(((new - phi) ** 2).sum()))
         phi = new
         step += 1

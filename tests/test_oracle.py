@@ -379,3 +379,32 @@ def test_polar_renorm_converges_to_polar_factor():
         for _ in range(40):
             x = x @ (1.5 * I - 0.5 * x.T @ x)
     assert not np.isfinite(x).all() or np.abs(x).max() > 1e6    # the unguarded iteration blows up
+
+
+def test_quintic_newton_schulz_maps_singular_values():
+    """One degree-5 step acts on the singular values alone: X = U diag(s) (guard inactive, since
+    ||X^T X||_inf = max s^2 < 2) goes to U diag(s (15 - 10 s^2 + 3 s^4) / 8); iterated it reaches the
+    polar factor from near-orthonormal and from badly scaled starts; from 1 + d it leaves a
+    residual of order d^3 where one cubic step leaves order d^2."""
+    rng = np.random.default_rng(8)
+    k = 10
+    U, _ = np.linalg.qr(rng.standard_normal((k, k)))
+    s = np.linspace(0.5, 1.2, k)
+    got = core.polar_renorm(U * s, 1, degree=5)
+    np.testing.assert_allclose(got, U * (s * (15 - 10 * s ** 2 + 3 * s ** 4) / 8), rtol=0, atol=1e-14)
+    near = U + 1e-3 * rng.standard_normal((k, k))
+    far = U @ np.diag(np.geomspace(0.3, 80.0, k)) @ np.linalg.qr(rng.standard_normal((k, k)))[0]
+    for X, iters in ((near, 3), (far, 30)):
+        Uu, _, Vt = np.linalg.svd(X)
+        assert np.abs(core.polar_renorm(X, iters, degree=5) - Uu @ Vt).max() < 1e-10
+    D = rng.standard_normal((k, k))
+
+    def residual(eps, degree):
+        x = core.polar_renorm(U + eps * D, 1, degree=degree)
+        return np.abs(x.T @ x - np.eye(k)).max()
+    # halving the perturbation divides the residual by ~8 (degree 5) and ~4 (degree 3)
+    assert 6.0 < residual(2e-3, 5) / residual(1e-3, 5) < 10.0
+    assert 3.0 < residual(2e-3, 3) / residual(1e-3, 3) < 5.0
+    assert residual(1e-3, 5) < 0.05 * residual(1e-3, 3)
+    with pytest.raises(ValueError):
+        core.polar_renorm(near, 1, degree=4)

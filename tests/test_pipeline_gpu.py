@@ -81,6 +81,56 @@ def test_stgd_steps_match_oracle(mmsb, k, b, renorm):
     p.close()
 
 
+@pytest.mark.parametrize("k,b,iters", [(10, 100, 1), (70, 200, 1), (20, 256, 2)])
+def test_stgd_quintic_matches_oracle(mmsb, k, b, iters):
+    """Degree-5 Newton-Schulz renormalisation (Gram, polynomial and update kernels) step for step
+    against the oracle's degree-5 iteration."""
+    n = 1000
+    rng = np.random.default_rng(k * b + iters)
+    ya, yb, yc = (rng.standard_normal((n, k)).astype(np.float32).astype(np.float64) for _ in range(3))
+    mua, mub, muc = (0.3 * rng.standard_normal(k) for _ in range(3))
+    phi0 = core.init_phi(9, k)
+    cfg = mmsb.config(nx=n, na=n, nb=n, nc=n, ncom=k, alpha_0=1.1, learn_rate=2e-3, minibatch=b, ns_iters=iters,
+                      ns_degree=5, oversample=0)
+    print(f"case k={k} b={b} iters={iters}", flush=True)
+    p = mmsb.Pipeline(cfg)
+    p.set_yvectors(ya, yb, yc, mua, mub, muc)
+    p.set_phi(phi0)
+    steps = 25
+    p.stgd(steps, "fp32")
+    phi_ref, log_ref = core.stgd(ya, yb, yc, mua, mub, muc, phi0, alpha0=1.1, eta=2e-3, batch=b, epochs=1,
+                                 steps=steps, ns_iters=iters, ns_degree=5)
+    assert rel_err(p.get("phi"), phi_ref) < 1e-4
+    np.testing.assert_allclose(p.get("frob"), log_ref, rtol=2e-3, atol=1e-12)
+    p.close()
+
+
+@pytest.mark.parametrize("degree,iters", [(3, 2), (5, 1)])
+def test_stgd_guarded_renorm_matches_oracle(mmsb, degree, iters):
+    """Three rows of 12x norm (hub nodes) push ||Phi~^T Phi~||_inf to 3-6, so the scale guard of the
+    Newton-Schulz iteration fires in most of the steps; the kernels follow the oracle's guarded
+    iteration (the unguarded one diverges from such starts)."""
+    n, k, b = 600, 16, 100
+    rng = np.random.default_rng(42)
+    ys = [rng.standard_normal((n, k)).astype(np.float32).astype(np.float64) for _ in range(3)]
+    for y in ys:
+        y[[50, 170, 333]] *= 12.0
+    mus = [0.2 * rng.standard_normal(k) for _ in range(3)]
+    phi0 = core.init_phi(5, k)
+    cfg = mmsb.config(nx=n, na=n, nb=n, nc=n, ncom=k, alpha_0=1.0, learn_rate=5e-3, minibatch=b, ns_iters=iters,
+                      ns_degree=degree, oversample=0)
+    print(f"case degree={degree} iters={iters}", flush=True)
+    p = mmsb.Pipeline(cfg)
+    p.set_yvectors(*ys, *mus)
+    p.set_phi(phi0)
+    p.stgd(12, "fp32")
+    phi_ref, log_ref = core.stgd(*ys, *mus, phi0, alpha0=1.0, eta=5e-3, batch=b, epochs=1, steps=12, ns_iters=iters,
+                                 ns_degree=degree)
+    assert rel_err(p.get("phi"), phi_ref) < 1e-4
+    np.testing.assert_allclose(p.get("frob"), log_ref, rtol=2e-3, atol=1e-12)
+    p.close()
+
+
 def test_stgd_full_epochs_and_wraparound(mmsb):
     """Several epochs over a batch size that does not divide n_X (last batch partial)."""
     n, k, b = 1037, 8, 100
@@ -168,8 +218,8 @@ def test_read_inputs_and_write_outputs(mmsb, tmp_path):
     p.close()
 
 
-@pytest.mark.parametrize("k,b", [(10, 128), (70, 200), (24, 64)])
-def test_stgd_bf16_tracks_oracle(mmsb, k, b):
+@pytest.mark.parametrize("k,b,degree", [(10, 128, 3), (70, 200, 3), (24, 64, 3), (70, 200, 5)])
+def test_stgd_bf16_tracks_oracle(mmsb, k, b, degree):
     """BF16 contractions (FP32 accumulation, split-BF16 k^3 products): Phi within 5e-3 of the FP64
     oracle after 25 steps, and the FP32 polish epoch brings it back to the FP32 trajectory's level."""
     n = 1000
@@ -177,13 +227,16 @@ def test_stgd_bf16_tracks_oracle(mmsb, k, b):
     ya, yb, yc = (rng.standard_normal((n, k)).astype(np.float32).astype(np.float64) for _ in range(3))
     mus = [0.3 * rng.standard_normal(k) for _ in range(3)]
     phi0 = core.init_phi(3, k)
-    cfg = mmsb.config(nx=n, na=n, nb=n, nc=n, ncom=k, alpha_0=1.3, learn_rate=2e-3, minibatch=b, oversample=0)
-    print(f"case k={k} b={b}", flush=True)
+    iters = 2 if degree == 3 else 1
+    cfg = mmsb.config(nx=n, na=n, nb=n, nc=n, ncom=k, alpha_0=1.3, learn_rate=2e-3, minibatch=b, oversample=0,
+                      ns_iters=iters, ns_degree=degree)
+    print(f"case k={k} b={b} degree={degree}", flush=True)
     p = mmsb.Pipeline(cfg)
     p.set_yvectors(ya, yb, yc, *mus)
     p.set_phi(phi0)
     p.stgd(25, "bf16")
-    phi_ref, log_ref = core.stgd(ya, yb, yc, *mus, phi0, alpha0=1.3, eta=2e-3, batch=b, epochs=1, steps=25)
+    phi_ref, log_ref = core.stgd(ya, yb, yc, *mus, phi0, alpha0=1.3, eta=2e-3, batch=b, epochs=1, steps=25,
+                                 ns_iters=iters, ns_degree=degree)
     got = p.get("phi")
     assert rel_err(got, phi_ref) < 5e-3
     assert np.abs(got.T @ got - np.eye(k)).max() < 1e-4      # split-BF16 renormalisation keeps Phi orthonormal
@@ -293,8 +346,8 @@ def test_timed_steps_match_untimed(mmsb, prec):
         p.close()
 
 
-@pytest.mark.parametrize("prec", ["fp32", "bf16"])
-def test_batch_matches_single_instances(mmsb, prec):
+@pytest.mark.parametrize("prec,degree", [("fp32", 3), ("bf16", 3), ("bf16", 5)])
+def test_batch_matches_single_instances(mmsb, prec, degree):
     """Uneven instances (n, k, minibatch, alpha0, eta all differ; odd and even steps per epoch)
     advanced together by the persistent batch launch give the same Phi, bit for bit, as each
     instance run alone, and the same ||dPhi||^2 log; also after autotuning the grid."""
@@ -306,12 +359,13 @@ def test_batch_matches_single_instances(mmsb, prec):
         mus = [0.2 * rng.standard_normal(k) for _ in range(3)]
         phi0 = core.init_phi(idx + 1, k)
         for lst in (singles, members):
-            cfg = mmsb.config(nx=n, na=n, nb=n, nc=n, ncom=k, alpha_0=a0, oversample=0, minibatch=b, learn_rate=eta)
+            cfg = mmsb.config(nx=n, na=n, nb=n, nc=n, ncom=k, alpha_0=a0, oversample=0, minibatch=b, learn_rate=eta,
+                              ns_iters=2 if degree == 3 else 1, ns_degree=degree)
             p = mmsb.Pipeline(cfg)
             p.set_yvectors(*ys, *mus)
             p.set_phi(phi0)
             lst.append(p)
-    print(f"case prec={prec} instances={len(specs)}", flush=True)
+    print(f"case prec={prec} degree={degree} instances={len(specs)}", flush=True)
     batch = mmsb.Batch(members, prec)
     batch.run(2)
     grid, ms = batch.autotune()
