@@ -32,14 +32,13 @@ PEAK = {"bf16": 2.5e15, "fp32": 157.3e12, "hbm": 8.0e12}
 # The degree-corrected graph's whitened vectors are heavy-tailed (row norms: median ~4, 99.9th
 # percentile ~120, max ~600 at 1M nodes), and the cubic step at the reference's eta = 1e-3 moves Phi
 # by O(1) per minibatch (median ||dPhi||_F^2 ~ 7): a chaotic regime in which no two precisions, or
-# two summation orders, can agree. eta = 1e-5 keeps the steps small (median ||dPhi||^2 ~ 7e-5). In
-# that regime one degree-5 Newton-Schulz iteration per step is as orthonormal as two degree-3 ones at
-# FP32 resolution (1 + d -> 1 + 2.5 d^3 against 1 - 3.4 d^4) and one kernel shorter; the accuracy
-# check still compares against the oracle's two degree-3 iterations.
+# two summation orders, can agree. eta = 1e-5 keeps the steps small (median ||dPhi||^2 ~ 7e-5). The
+# renormalisation is the library default (two degree-3 Newton-Schulz iterations per step): a single
+# degree-5 iteration is one kernel shorter but, on this graph, ends two epochs 8.5e-3 away from it.
 WORKLOADS = {
     "c1": dict(n=1000, k=10, alpha0=1.0, dc=False, p_in=0.6, p_out=0.02, b=100, eta=1e-3),
     "c2": dict(n=100000, k=100, alpha0=1.0, dc=False, p_in=0.01, p_out=2e-4, b=1024, eta=1e-3),
-    "c3": dict(n=1000000, k=500, alpha0=2.0, dc=True, mean_degree=80.0, b=1024, eta=1e-5, ns_degree=5, ns_iters=1),
+    "c3": dict(n=1000000, k=500, alpha0=2.0, dc=True, mean_degree=80.0, b=1024, eta=1e-5),
 }
 
 
@@ -167,8 +166,9 @@ def main():
     ph_ms = p.stgd_timed(spe, prec_used)
     phases_us = {nm: 1e3 * t / spe for nm, t in zip(("p1", "p2", "renorm"), ph_ms)}
     p1_flops = (4.0 * k * k * n + 2.0 * k ** 3 * spe) / spe       # per launch: 2 b k^2 (P_A) + 2 b k^2 (P_B) + 2 k^3
-    p1_achieved = p1_flops / (phases_us["p1"] * 1e-6)
-    log(f"phases (us/step): {phases_us}; P1 {p1_achieved / 1e12:.1f} TFLOP/s")
+    # (under a tracing profiler the event pairs can cost more than the phases; no rate then)
+    p1_achieved = p1_flops / (phases_us["p1"] * 1e-6) if phases_us["p1"] > 0 else None
+    log(f"phases (us/step): {phases_us}; P1 {(p1_achieved or 0) / 1e12:.1f} TFLOP/s")
 
     # ---- accuracy against the FP64 oracle, and the CPU baseline ----------------------------------
     accuracy, cpu = None, None
@@ -205,8 +205,9 @@ def main():
         "stgd_nodes_per_s": {pr: v[0] for pr, v in rates.items()},
         "bf16_speedup_vs_fp32": rates["bf16"][0] / rates["fp32"][0],
         "accuracy": accuracy,
-        "roofline": {"bound": "mfma", "achieved": p1_achieved / 1e12, "peak": peak / 1e12, "unit": "TFLOP/s",
-                     "frac": p1_achieved / peak, "traffic": measured_traffic(args.workload, prec_used),
+        "roofline": {"bound": "mfma", "achieved": p1_achieved and p1_achieved / 1e12, "peak": peak / 1e12,
+                     "unit": "TFLOP/s", "frac": p1_achieved and p1_achieved / peak,
+                     "traffic": measured_traffic(args.workload, prec_used),
                      "kernel": f"stgd_phase_kernel P1 ({prec_used}): Y_A Phi, Y_B Phi + C_g epilogue, (Phi^T Phi)^o2",
                      "flops_per_launch": p1_flops, "avg_us": phases_us["p1"], "phases_us_per_step": phases_us,
                      "epoch_tflops": epoch_achieved / 1e12, "epoch_frac": epoch_achieved / peak,

@@ -28,17 +28,24 @@ inline int default_threads() {
 }
 
 // Calls fn(lo, hi) on `threads` contiguous sub-ranges of [0, n).
-inline void parallel_for(int64_t n, int threads, const std::function<void(int64_t, int64_t)>& fn) {
+// Runs fn(chunk, lo, hi) for the min(threads, n) static chunks [n c / t, n (c + 1) / t) of [0, n),
+// one thread each. Per-thread state must be indexed by the chunk argument: recovering it from lo
+// (as lo * t / n) is off by one whenever n * c / t is not an integer.
+inline void parallel_chunks(int64_t n, int threads, const std::function<void(int, int64_t, int64_t)>& fn) {
   if (n <= 0) return;
   const int64_t t = std::max<int64_t>(1, std::min<int64_t>(threads, n));
   if (t == 1) {
-    fn(0, n);
+    fn(0, 0, n);
     return;
   }
   std::vector<std::thread> pool;
   pool.reserve(t);
-  for (int64_t i = 0; i < t; ++i) pool.emplace_back(fn, n * i / t, n * (i + 1) / t);
+  for (int64_t i = 0; i < t; ++i) pool.emplace_back(fn, (int)i, n * i / t, n * (i + 1) / t);
   for (auto& th : pool) th.join();
+}
+
+inline void parallel_for(int64_t n, int threads, const std::function<void(int64_t, int64_t)>& fn) {
+  parallel_chunks(n, threads, [&](int, int64_t lo, int64_t hi) { fn(lo, hi); });
 }
 
 // Sorts keys ascending and removes duplicates.
@@ -53,8 +60,7 @@ inline void parallel_sort_unique(std::vector<uint64_t>& keys, int threads) {
   const int nb = 8 * threads;
   const uint64_t width = mx / (uint64_t)nb + 1;
   std::vector<std::vector<int64_t>> hist(threads, std::vector<int64_t>(nb, 0));
-  parallel_for(n, threads, [&](int64_t lo, int64_t hi) {
-    const int t = (int)((lo * threads) / n);
+  parallel_chunks(n, threads, [&](int t, int64_t lo, int64_t hi) {
     for (int64_t i = lo; i < hi; ++i) hist[t][keys[i] / width]++;
   });
   // offsets per (bucket, thread) in bucket-major order
@@ -70,8 +76,7 @@ inline void parallel_sort_unique(std::vector<uint64_t>& keys, int threads) {
   }
   bstart[nb] = pos;
   std::vector<uint64_t> tmp(n);
-  parallel_for(n, threads, [&](int64_t lo, int64_t hi) {
-    const int t = (int)((lo * threads) / n);
+  parallel_chunks(n, threads, [&](int t, int64_t lo, int64_t hi) {
     for (int64_t i = lo; i < hi; ++i) tmp[off[t][keys[i] / width]++] = keys[i];
   });
   std::vector<int64_t> uniq(nb, 0);

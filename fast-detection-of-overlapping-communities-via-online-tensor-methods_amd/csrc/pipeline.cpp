@@ -1087,14 +1087,12 @@ extern "C" int mmsb_set_block(mmsb_ctx* c, int32_t which, const mmsb_csr* b) {
   {  // every row pointer monotone, every column index in range (threads report the first bad index)
     const int nt = default_threads();
     std::vector<int64_t> bad_row(nt, -1), bad_nz(nt, -1);
-    parallel_for(b->n_rows, nt, [&](int64_t lo, int64_t hi) {
-      const int t = (int)(lo * nt / b->n_rows);
+    parallel_chunks(b->n_rows, nt, [&](int t, int64_t lo, int64_t hi) {
       for (int64_t i = lo; i < hi && bad_row[t] < 0; ++i)
         if (b->rowptr[i + 1] < b->rowptr[i]) bad_row[t] = i;
     });
     for (int64_t r : bad_row) MMSB_CHECK_ARG(r < 0, "set_block: rowptr not monotone at row %lld", (long long)r);
-    parallel_for(b->nnz, nt, [&](int64_t lo, int64_t hi) {
-      const int t = (int)(lo * nt / std::max<int64_t>(b->nnz, 1));
+    parallel_chunks(b->nnz, nt, [&](int t, int64_t lo, int64_t hi) {
       for (int64_t e = lo; e < hi && bad_nz[t] < 0; ++e)
         if (b->colidx[e] < 0 || b->colidx[e] >= b->n_cols) bad_nz[t] = e;
     });

@@ -208,6 +208,75 @@ __device__ inline void wave_mm(floatx16& acc, const Opnd<TA>& A, const Opnd<TB>&
   }
 }
 
+// Two products that share their column operand, acc_a += A1 B^T and acc_b += A2 B^T, for one wave
+// over k in [k0, k0 + kl) (FP32 or BF16 modes): the fragment set holds A1, A2 and B, so B is loaded
+// once for both.
+template <int MODE>
+struct FragPair {
+  using V = typename std::conditional<MODE == MM_F32, float4, bf16x8>::type;
+  using T = op_t<MODE>;
+  static constexpr int W = MODE == MM_F32 ? 4 : 8;  // elements per lane and chunk
+  V a1[4], a2[4], b[4];
+  __device__ inline void load(const T* p1, const T* p2, const T* pb, int h) {
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      if constexpr (MODE == MM_F32) {
+        a1[c] = ld4<true>(p1 + 8 * c + 4 * h);
+        a2[c] = ld4<true>(p2 + 8 * c + 4 * h);
+        b[c] = ld4<true>(pb + 8 * c + 4 * h);
+      } else {
+        a1[c] = ld8<true>(p1 + 16 * c + 8 * h);
+        a2[c] = ld8<true>(p2 + 16 * c + 8 * h);
+        b[c] = ld8<true>(pb + 16 * c + 8 * h);
+      }
+    }
+  }
+  __device__ inline void mma(floatx16& acc_a, floatx16& acc_b) const {
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      if constexpr (MODE == MM_F32) {
+        acc_a = __builtin_amdgcn_mfma_f32_32x32x2f32(a1[c].x, b[c].x, acc_a, 0, 0, 0);
+        acc_b = __builtin_amdgcn_mfma_f32_32x32x2f32(a2[c].x, b[c].x, acc_b, 0, 0, 0);
+        acc_a = __builtin_amdgcn_mfma_f32_32x32x2f32(a1[c].y, b[c].y, acc_a, 0, 0, 0);
+        acc_b = __builtin_amdgcn_mfma_f32_32x32x2f32(a2[c].y, b[c].y, acc_b, 0, 0, 0);
+        acc_a = __builtin_amdgcn_mfma_f32_32x32x2f32(a1[c].z, b[c].z, acc_a, 0, 0, 0);
+        acc_b = __builtin_amdgcn_mfma_f32_32x32x2f32(a2[c].z, b[c].z, acc_b, 0, 0, 0);
+        acc_a = __builtin_amdgcn_mfma_f32_32x32x2f32(a1[c].w, b[c].w, acc_a, 0, 0, 0);
+        acc_b = __builtin_amdgcn_mfma_f32_32x32x2f32(a2[c].w, b[c].w, acc_b, 0, 0, 0);
+      } else {
+        acc_a = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1[c], b[c], acc_a, 0, 0, 0);
+        acc_b = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a2[c], b[c], acc_b, 0, 0, 0);
+      }
+    }
+  }
+};
+
+template <int MODE>
+__device__ inline void wave_mm_pair(floatx16& acc_a, floatx16& acc_b, const op_t<MODE>* A1, const op_t<MODE>* A2, int64_t lda,
+                                    const op_t<MODE>* B, int64_t ldb, int64_t r0, int64_t c0, int64_t k0, int64_t kl) {
+  if (kl <= 0) return;
+  constexpr int G = kgroup<MODE>();
+  constexpr int D = 2;
+  const int lane = threadIdx.x & 63;
+  const int rr = lane & 31, h = lane >> 5;
+  const op_t<MODE>* a1 = A1 + (r0 + rr) * lda + k0;
+  const op_t<MODE>* a2 = A2 + (r0 + rr) * lda + k0;
+  const op_t<MODE>* b = B + (c0 + rr) * ldb + k0;
+  FragPair<MODE> f[D];
+  f[0].load(a1, a2, b, h);
+  for (int64_t g = 0; g < kl; g += D * G) {
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      const int64_t kk = g + d * G;
+      if (kk < kl) {
+        const int64_t kn = kk + (D - 1) * G;
+        if (kn < kl) f[(d + D - 1) % D].load(a1 + kn, a2 + kn, b + kn, h);
+        f[d].mma(acc_a, acc_b);
+      }
+    }
+  }
+}
+
 // The four waves' 32 x 32 partial accumulators. Register i of lane l holds element
 // (8 (i / 4) + 4 (l / 32) + i % 4, l % 32) of the block; rows of 66 words keep the column-wise
 // read-back of the epilogues to two-way bank conflicts.
@@ -377,7 +446,7 @@ struct StepCtx {
 
 union StgdSmem {
   struct {
-    Red4 red;
+    Red4 red[2];  // data tiles: the four waves' partials of P_A (red[0]) and P_B (red[1])
     float mvec[2][32];
   } p1;
   struct {
@@ -501,7 +570,6 @@ __device__ void p1_tile(const StepCtx& P, int tile, StgdSmem& sm) {
   const int ntk = kp / 32;
   const int64_t nel = (int64_t)kp * kp;
   const int w = threadIdx.x >> 6;
-  Red4& red = sm.p1.red;
   floatx16 acc = {};
   int64_t k0, kl;
   if (tile >= ntk * ntk) {  // data tiles follow the H tiles, which then start in the first wave
@@ -510,16 +578,19 @@ __device__ void p1_tile(const StepCtx& P, int tile, StgdSmem& sm) {
     const int64_t x0 = (int64_t)tb * 32, j0 = (int64_t)tj * 32;
     PartSum ps;
     mpart_load(ps, P.B.mpart, kp, ntk, j0);  // m_A, m_B of the block's columns
-    // waves 0, 1: P_A over two halves of the k reduction; waves 2, 3: P_B
-    const TY* Yb = static_cast<const TY*>(w < 2 ? P.YA : P.YB) + P.row0 * (int64_t)kp;
-    kslice<DMODE>(kp, 2, w & 1, k0, kl);
-    wave_mm<DMODE, true>(acc, opnd<TY>(Yb, nullptr, kp), pick<DMODE>(P.B.phiT, P.B.phiT16, nel, kp), x0, j0, k0, kl);
+    // every wave: P_A and P_B over its quarter of the k reduction, sharing the Phi^T fragments
+    floatx16 acc_b = {};
+    kslice<DMODE>(kp, 4, w, k0, kl);
+    wave_mm_pair<DMODE>(acc, acc_b, static_cast<const TY*>(P.YA) + P.row0 * (int64_t)kp,
+                        static_cast<const TY*>(P.YB) + P.row0 * (int64_t)kp, kp, pick<DMODE>(P.B.phiT, P.B.phiT16, nel, kp).hi,
+                        kp, x0, j0, k0, kl);
     {
       const float m = mpart_reduce(ps, P.B.mpart, kp, ntk, j0);
       const int o = threadIdx.x >> 2;
       if ((threadIdx.x & 3) == 0) sm.p1.mvec[o >> 5][o & 31] = m;
     }
-    red_put(red, acc);
+    red_put(sm.p1.red[0], acc);
+    red_put(sm.p1.red[1], acc_b);
     __syncthreads();
     // C_g^T rows are columns j of the block: thread -> column c, rows r4 .. r4 + 3
     const int c = threadIdx.x >> 3, r4 = 4 * (threadIdx.x & 7);
@@ -528,8 +599,8 @@ __device__ void p1_tile(const StepCtx& P, int tile, StgdSmem& sm) {
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
       const int r = r4 + e;
-      const float p = red_get(red, 0, r, c) + red_get(red, 1, r, c);
-      const float q = red_get(red, 2, r, c) + red_get(red, 3, r, c);
+      const float p = red_sum4(sm.p1.red[0], r, c);
+      const float q = red_sum4(sm.p1.red[1], r, c);
       const bool valid = (x0 + r) < P.nb;
       const float ab = p * q;
       cs += valid ? ab : 0.f;
@@ -550,6 +621,7 @@ __device__ void p1_tile(const StepCtx& P, int tile, StgdSmem& sm) {
     const Opnd<op_t<KMODE>> phiT_k = pick<KMODE>(P.B.phiT, P.B.phiT16, nel, kp);
     kslice<KMODE>(kp, 4, w, k0, kl);
     wave_mm<KMODE, true>(acc, phiT_k, phiT_k, i0, j0, k0, kl);  // G = Phi^T Phi
+    Red4& red = sm.p1.red[0];
     red_put(red, acc);
     __syncthreads();
     const int r = threadIdx.x >> 3, c4 = 4 * (threadIdx.x & 7);

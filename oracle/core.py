@@ -23,19 +23,18 @@ from .philox import STREAM_OMEGA, draw, u53, u53_open0
 
 # ---- sparse helpers ------------------------------------------------------------------------
 def csr_spmm(rowptr, colidx, vals, B, m, alpha=1.0):
-    """Y = alpha * G @ B for CSR G with m rows (float64 accumulation)."""
+    """Y = alpha * G @ B for CSR G with m rows (float64; each output row accumulates its nonzeros
+    in CSR order, as scipy's CSR-times-dense loop does)."""
+    import scipy.sparse as sp
     B = np.asarray(B, np.float64)
     rowptr = np.asarray(rowptr, np.int64)
-    colidx = np.asarray(colidx, np.int64)
-    nnz = int(rowptr[m]) - int(rowptr[0])
-    if nnz == 0:
+    r0, r1 = int(rowptr[0]), int(rowptr[m])
+    if r1 == r0:
         return np.zeros((m, B.shape[1]))
-    rows = np.repeat(np.arange(m), np.diff(rowptr[: m + 1]))
-    cols = colidx[rowptr[0]: rowptr[m]]
-    w = np.ones(nnz) if vals is None else np.asarray(vals, np.float64)[rowptr[0]: rowptr[m]]
-    Y = np.zeros((m, B.shape[1]))
-    np.add.at(Y, rows, w[:, None] * B[cols])
-    return alpha * Y
+    cols = np.asarray(colidx, np.int64)[r0:r1]
+    w = np.ones(r1 - r0) if vals is None else np.asarray(vals, np.float64)[r0:r1]
+    G = sp.csr_matrix((w, cols, rowptr[: m + 1] - r0), shape=(m, B.shape[0]))
+    return alpha * (G @ B)
 
 
 def csr_transpose(rowptr, colidx, n_rows, n_cols, vals=None):

@@ -38,6 +38,22 @@ def test_generator_independent_of_thread_count(mmsb):
         assert np.array_equal(a[name][0], b[name][0]) and np.array_equal(a[name][1], b[name][1])
 
 
+def test_generator_large_graph_matches_oracle_at_any_thread_count(mmsb):
+    """1.65M edges: above the 2^20 keys where the edge list is sorted and deduplicated in parallel
+    buckets. 7 threads split the keys into chunks whose bounds n c / 7 are not integers, which once
+    made two chunks share one histogram (edges were lost, and the count varied run to run)."""
+    case = dict(n_x=12000, n_a=12000, n_b=12000, n_c=12000, k=30, alpha0=1.5, p_in=0.02, p_out=0.0008, seed=4,
+                degree_corrected=True, mode="balls")
+    a = mmsb.generate(**case, threads=1)
+    b = mmsb.generate(**case, threads=7)
+    g = gen.generate(gen.GenParams(**case))
+    assert a["n_edges"] == b["n_edges"] == len(g.edges) > (1 << 20)
+    for name in ("XA", "XB", "XC", "BA", "CA"):
+        orp, oci, _, _ = g.blocks[name]
+        for got in (a, b):
+            assert np.array_equal(got[name][0], orp) and np.array_equal(got[name][1], oci), name
+
+
 def test_generator_rejects_bad_params(mmsb):
     with pytest.raises(mmsb.MMSBError):
         mmsb.generate(0, 10, 10, 10, 3, 1.0, 0.1, 0.01)
