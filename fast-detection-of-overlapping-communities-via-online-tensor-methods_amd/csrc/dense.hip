@@ -148,10 +148,13 @@ static int tn_mfma_tiles(int32_t p, int32_t q, bool sym) {
   return sym ? tp * (tp + 1) / 2 : tp * tq;
 }
 
+// Slices of the reduction: about 1024 workgroups in all, i.e. two full rounds of the 512 that fit at
+// once (256 CUs x 2); rounding the count down keeps the last round full (10 symmetric tiles of a
+// 510 x 510 Gram: 102 slices, where 103 capped to 64 left the second round a quarter full).
 static int tn_mfma_slices(int64_t n, int32_t p, int32_t q, bool sym) {
   const int64_t tiles = tn_mfma_tiles(p, q, sym);
-  int64_t s = ceil_div(1024, tiles);
-  const int64_t cap = std::max<int64_t>(1, (int64_t)(128ll << 20) / ((int64_t)p * q * 8));  // partials <= 128 MiB
+  int64_t s = std::max<int64_t>(1, 1024 / tiles);
+  const int64_t cap = std::max<int64_t>(1, (int64_t)(256ll << 20) / ((int64_t)p * q * 8));  // partials <= 256 MiB
   s = std::min<int64_t>(std::min<int64_t>(s, cap), std::max<int64_t>(1, n / 1024));
   return (int)std::max<int64_t>(1, s);
 }
@@ -473,9 +476,10 @@ template int gemm_nn<double, float>(hipStream_t, int64_t, int32_t, int32_t, doub
 // chol_kernel: right-looking blocked Cholesky of the lower triangle of G + shift trace(G) I in
 // 16-column panels: the diagonal block is factored in LDS, the rows below it solved one row per
 // thread, and the trailing lower triangle updated once per panel in 4 x 4 register tiles.
-// trinv_kernel: X = L^{-1} by forward substitution, PI rows at a time; each thread owns one column
-// c of X (a column of L^{-1} depends only on itself), so the only barriers are between row panels,
-// which make the panel's rows of L visible in LDS. R = L^T and R^{-1} = X^T are written out.
+// R = L^T and R^{-1} = X^T with X = L^{-1}: for r <= 1024, chol_transpose_kernel writes R through
+// LDS tiles and trinv_cols_kernel solves blocks of kTrinvCols columns of X in parallel workgroups
+// (one thread per row of X); larger r falls back to trinv_kernel, one workgroup that walks PI rows
+// at a time with one thread per column of X.
 constexpr int kCholP = 16;
 constexpr int kCholThreads = 512;
 
@@ -639,13 +643,87 @@ __global__ __launch_bounds__(1024) void trinv_kernel(int32_t r, const double* __
   }
 }
 
+// R = L^T (upper triangle, zeros below) from the lower triangle of L, through a 32 x 32 LDS tile so
+// that reads and writes are both along rows.
+__global__ __launch_bounds__(256) void chol_transpose_kernel(int32_t r, const double* __restrict__ L,
+                                                             double* __restrict__ R) {
+  __shared__ double t[32][33];
+  const int bi = blockIdx.y * 32, bj = blockIdx.x * 32;  // R rows bi.., R columns bj..
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+  for (int y = ty; y < 32; y += 8) {
+    const int l = bj + y, i = bi + tx;  // L[l][i] = R[i][l]
+    t[y][tx] = (l < r && i < r) ? L[(int64_t)l * r + i] : 0.0;
+  }
+  __syncthreads();
+  for (int y = ty; y < 32; y += 8) {
+    const int i = bi + y, l = bj + tx;
+    if (i < r && l < r) R[(int64_t)i * r + l] = (l >= i) ? t[tx][y] : 0.0;
+  }
+}
+
+// Columns [c0, c0 + kTrinvCols) of X = L^{-1} by forward substitution, one thread per row k of X
+// (blockDim >= r): b_k starts as the unit vector entries; at step i the owner of row i finalises
+// x_i = b_i / L_ii and broadcasts it through LDS (double-buffered by step parity, one barrier per
+// step), and every row k > i subtracts L_ki x_i. L_ki = R[i][k] is read along row i of R (coalesced),
+// kTrinvRing steps ahead of its use. Rows of X above a column's diagonal stay exactly zero. Writes
+// row c of R^{-1} (= column c of X) for each of the block's columns.
+constexpr int kTrinvCols = 16;
+constexpr int kTrinvRing = 16;
+
+__global__ __launch_bounds__(1024) void trinv_cols_kernel(int32_t r, const double* __restrict__ R,
+                                                          double* __restrict__ Rinv) {
+  constexpr int C = kTrinvCols, D = kTrinvRing;
+  __shared__ double xb[2][C];
+  const int k = threadIdx.x;
+  const int c0 = blockIdx.x * C;
+  double b[C];
+#pragma unroll
+  for (int c = 0; c < C; ++c) b[c] = (k == c0 + c) ? 1.0 : 0.0;
+  const double inv_kk = k < r ? 1.0 / R[(int64_t)k * r + k] : 0.0;
+  auto lval = [&](int i) -> double { return (i < r && k > i && k < r) ? R[(int64_t)i * r + k] : 0.0; };
+  double ring[D];
+#pragma unroll
+  for (int d = 0; d < D; ++d) ring[d] = lval(c0 + d);
+  for (int ib = c0; ib < r; ib += D) {
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      const int i = ib + d;
+      if (i < r) {  // uniform over the block
+        const double lki = ring[d];
+        ring[d] = lval(i + D);
+        if (k == i) {
+#pragma unroll
+          for (int c = 0; c < C; ++c) {
+            b[c] *= inv_kk;
+            xb[i & 1][c] = b[c];
+          }
+        }
+        __syncthreads();
+        if (k > i) {
+#pragma unroll
+          for (int c = 0; c < C; ++c) b[c] = fma(-lki, xb[i & 1][c], b[c]);
+        }
+      }
+    }
+  }
+  if (k < r) {
+#pragma unroll
+    for (int c = 0; c < C; ++c)
+      if (c0 + c < r) Rinv[(int64_t)(c0 + c) * r + k] = (k >= c0 + c) ? b[c] : 0.0;
+  }
+}
+
 int chol_inv(hipStream_t s, int32_t r, double* G, double shift, double* R, double* Rinv, int32_t* status) {
   MMSB_CHECK_ARG(r >= 1 && r <= 4096, "chol_inv: r=%d out of range", r);
   hipLaunchKernelGGL(chol_kernel, dim3(1), dim3(kCholThreads), 0, s, r, G, shift, status);
   MMSB_LAUNCH_CHECK();
-  if (r <= 1024)
-    hipLaunchKernelGGL(trinv_kernel<16>, dim3(1), dim3(1024), 0, s, r, G, R, Rinv);
-  else if (r <= 2048)
+  if (r <= 1024) {
+    const int nt = (r + 31) / 32;
+    hipLaunchKernelGGL(chol_transpose_kernel, dim3(nt, nt), dim3(256), 0, s, r, G, R);
+    MMSB_LAUNCH_CHECK();
+    const int threads = (r + 63) / 64 * 64;
+    hipLaunchKernelGGL(trinv_cols_kernel, dim3((r + kTrinvCols - 1) / kTrinvCols), dim3(threads), 0, s, r, R, Rinv);
+  } else if (r <= 2048)
     hipLaunchKernelGGL(trinv_kernel<8>, dim3(1), dim3(1024), 0, s, r, G, R, Rinv);
   else
     hipLaunchKernelGGL(trinv_kernel<4>, dim3(1), dim3(1024), 0, s, r, G, R, Rinv);
