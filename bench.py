@@ -68,6 +68,9 @@ def main():
     ap.add_argument("--precision", default="bf16", choices=["fp32", "bf16"])
     ap.add_argument("--no-oracle", action="store_true", help="skip the FP64 oracle epochs (accuracy, cpu_baseline)")
     ap.add_argument("--accuracy-epochs", type=int, default=2)
+    ap.add_argument("--cpu-e2e-sample", type=int, default=50000,
+                    help="nodes per partition of the sample on which the oracle pipeline is timed for the "
+                         "end-to-end CPU estimate (0: skip)")
     ap.add_argument("--ns-degree", type=int, default=None, choices=[3, 5],
                     help="Newton-Schulz degree of the renormalisation (default: the workload's, else the library's)")
     ap.add_argument("--ns-iters", type=int, default=None,
@@ -134,7 +137,8 @@ def main():
             "nonfinite_W_rows0_1000": int((~np.isfinite(p.get_rows("W", 0, 1000))).sum()),
             "lam_m2_top": [float(x) for x in p.get("lam_m2")[:3]],
             "lam_m2_last": [float(x) for x in p.get("lam_m2")[-3:]],
-            "frob_last": float(p.get("frob")[-1])}
+            "frob_last": float(p.get("frob")[-1]),
+            "jacobi_sweeps": [int(x) for x in p.get("sweeps")]}
     log(f"diagnostics {diag}")
     del pin
 
@@ -176,6 +180,11 @@ def main():
         accuracy, cpu = accuracy_vs_oracle(m, p, phi0, n, k, b, w, spe, args.accuracy_epochs)
         log(f"accuracy {accuracy}")
 
+    cpu_e2e = None
+    if not args.no_oracle and args.cpu_e2e_sample > 0 and args.cpu_e2e_sample < n:
+        cpu_e2e = cpu_e2e_estimate(m, w, n, args.cpu_e2e_sample, t_e2e)
+        log(f"cpu e2e {cpu_e2e}")
+
     # ---- SpMM roofline on G_XA (n x n, nnz) times a tall n x r FP32 matrix ----------------------
     spmm = spmm_roofline(m, g["XA"], r=k + 10)
 
@@ -216,6 +225,8 @@ def main():
     }
     if cpu is not None:
         out["cpu_baseline"] = cpu
+    if cpu_e2e is not None:
+        out["cpu_e2e"] = cpu_e2e
     p.close()
     print(json.dumps(out), flush=True)
 
@@ -298,6 +309,45 @@ def accuracy_vs_oracle(m, p, phi0, n, k, b, w, spe, epochs):
            "sample": f"{epochs} full STGD epochs (n={n}, b={b}, k={k}) of the oracle's FP64 numpy step on the "
                      f"whitened vectors of this run, {t_oracle:.1f}s"}
     return acc, cpu
+
+
+def cpu_e2e_estimate(m, w, n, ns, gpu_e2e_s):
+    """The oracle's whole pipeline (FP64 numpy / scipy: whitening, y vectors, one STGD epoch,
+    eigenvalues, Pi and its normalisation) timed on a sample of the workload -- ns nodes per partition,
+    same k, alpha0, degree distribution and mean degree -- and scaled by n / ns. Every stage is linear in
+    n at fixed k and degree (SpMMs in nnz * r, tall products in n r^2; the r x r factorisations are a
+    fixed cost), and the full size only adds cache misses, so the scaled figure underestimates the
+    CPU's time at n. The graph generation is not timed."""
+    import numpy as np
+    from oracle import core
+    p_in, p_out = c3_probs(ns, w["k"], w["mean_degree"]) if "mean_degree" in w else (w["p_in"], w["p_out"])
+    g = m.generate(ns, ns, ns, ns, w["k"], w["alpha0"], p_in, p_out, seed=0, degree_corrected=w["dc"], threads=16)
+    bl = {nm: core.Csr(g[nm][0], g[nm][1], g[nm][3], g[nm][4]) for nm in ("XA", "XB", "XC", "BA", "CA")}
+    del g
+    wcfg = core.WhitenConfig(k=w["k"], alpha0=w["alpha0"], oversample=10, power_iters=1, seed=1)
+    t = {}
+    t0 = time.time()
+    wh = core.whiten(bl["XA"], bl["XB"], bl["XC"], wcfg)
+    t["whiten"] = time.time() - t0
+    t0 = time.time()
+    ys = core.yvectors(bl["XA"], bl["XB"], bl["XC"], wh)
+    t["yvectors"] = time.time() - t0
+    t0 = time.time()
+    phi, _ = core.stgd(ys["Y_A"], ys["Y_B"], ys["Y_C"], ys["mu_A"], ys["mu_B"], ys["mu_C"], core.init_phi(1, w["k"]),
+                       alpha0=w["alpha0"], eta=w["eta"], batch=w["b"], epochs=1)
+    t["stgd_1_epoch"] = time.time() - t0
+    t0 = time.time()
+    _, lam_p, _ = core.eigenvalues(phi)
+    pis = core.post_proc(wh, phi, lam_p, bl["XA"], bl["BA"], bl["CA"])
+    _ = [core.norm_prob_rows(P) for P in pis]
+    t["eigen_post"] = time.time() - t0
+    scale = n / ns
+    est = scale * sum(t.values())
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return {"estimate_s": est, "gpu_end_to_end_s": gpu_e2e_s, "speedup": est / gpu_e2e_s, "cores": threads,
+            "kind": "port", "sample_nodes_per_partition": ns, "scale": scale,
+            "sample_s": {key: round(val, 3) for key, val in t.items()},
+            "sample": f"oracle pipeline on {ns} nodes/partition of the same workload family, scaled by {scale:g}"}
 
 
 def spmm_roofline(m, blk, r):

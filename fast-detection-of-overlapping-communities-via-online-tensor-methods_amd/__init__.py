@@ -85,7 +85,7 @@ BLOCKS = {"XA": 0, "XB": 1, "XC": 2, "BA": 3, "CA": 4}
 QUANTITIES = ["mu_a", "mu_b", "mu_c", "W", "U", "V", "sigma", "LB", "LC", "lam_m2", "YA", "YB", "YC",
               "muw_a", "muw_b", "muw_c", "phi", "frob", "lambda", "lambda_p", "alpha_hat",
               "pi_x", "pi_a", "pi_b", "pi_c", "pin_x", "pin_a", "pin_b", "pin_c",
-              "count_x", "count_a", "count_b", "count_c", "times"]
+              "count_x", "count_a", "count_b", "count_c", "times", "sweeps"]
 Q = {name: i for i, name in enumerate(QUANTITIES)}
 
 # name -> (restype, argtypes); every symbol declared in include/mmsb_stpm.h appears here.

@@ -1396,6 +1396,7 @@ extern "C" int mmsb_post(mmsb_ctx* c) {
 extern "C" int64_t mmsb_get_size(mmsb_ctx* c, int32_t what) {
   if (!c) return 0;
   if (what == MMSB_Q_TIMES) return 4;
+  if (what == MMSB_Q_SWEEPS) return c->whitened ? 2 : 0;
   const View v = view_of(c, what);
   return v.p ? v.rows * v.cols : 0;
 }
@@ -1406,6 +1407,17 @@ extern "C" int mmsb_get(mmsb_ctx* c, int32_t what, double* host, int64_t count) 
   if (what == MMSB_Q_TIMES) {
     MMSB_CHECK_ARG(count >= 4, "get: need 4 doubles for the timings");
     std::memcpy(host, c->times, sizeof(c->times));
+    return MMSB_OK;
+  }
+  if (what == MMSB_Q_SWEEPS) {
+    MMSB_CHECK_ARG(c->whitened, "get: no whitening has run");
+    MMSB_CHECK_ARG(count >= 2, "get: need 2 doubles for the sweep counts");
+    int32_t st[8];
+    HCHK(hipSetDevice(c->cfg.device));
+    HCHK(hipMemcpyAsync(st, c->dstat, sizeof(st), hipMemcpyDeviceToHost, c->s));
+    HCHK(hipStreamSynchronize(c->s));
+    host[0] = st[6];
+    host[1] = st[7];
     return MMSB_OK;
   }
   const View v = view_of(c, what);

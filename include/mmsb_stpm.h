@@ -228,6 +228,8 @@ enum {
   MMSB_Q_PIN_X, MMSB_Q_PIN_A, MMSB_Q_PIN_B, MMSB_Q_PIN_C, /* [n_p x k] normalised Pi            */
   MMSB_Q_COUNT_X, MMSB_Q_COUNT_A, MMSB_Q_COUNT_B, MMSB_Q_COUNT_C, /* [n_p] memberships per node */
   MMSB_Q_TIMES,                                     /* [4] seconds: whiten, stgd, post, total   */
+  MMSB_Q_SWEEPS,                                    /* [2] Jacobi sweeps of the last whitening's
+                                                       r x r SVD and eigensolve                  */
   MMSB_Q__COUNT
 };
 

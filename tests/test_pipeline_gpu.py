@@ -56,6 +56,8 @@ def test_whitening_matches_oracle(mmsb, k, n, scaling):
     we = core.whiten_exact(bl["XA"], bl["XB"], bl["XC"], wcfg)
     W = p.get("W")
     assert np.abs(W.T @ we["M2"] @ W - np.eye(k)).max() < 1e-3
+    sweeps = p.get("sweeps")              # both r x r Jacobi factorisations converged before the cap
+    assert sweeps.shape == (2,) and (sweeps >= 1).all() and (sweeps < 40).all(), sweeps
     p.close()
 
 
