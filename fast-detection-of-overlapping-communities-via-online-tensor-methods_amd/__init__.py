@@ -94,6 +94,10 @@ SIGNATURES: dict = {
     "mmsb_version": (ctypes.c_char_p, []),
     "mmsb_spmm_f32": (ctypes.c_int, [_vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int64, _vp, _vp, _vp,
                                      ctypes.c_float, _vp, ctypes.c_int32, ctypes.c_int32, _vp, ctypes.c_int32]),
+    "mmsb_spmm_f32_ordered": (ctypes.c_int, [_vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int64, _vp, _vp, _vp,
+                                             ctypes.c_float, _vp, ctypes.c_int32, ctypes.c_int32, _vp, ctypes.c_int32,
+                                             _vp]),
+    "mmsb_csr_row_order": (ctypes.c_int, [_vp, ctypes.c_int32, _vp, _vp]),
     "mmsb_gen_params_default": (None, [ctypes.POINTER(GenParamsC)]),
     "mmsb_generate": (ctypes.c_int, [ctypes.POINTER(GenParamsC), ctypes.c_int32, ctypes.POINTER(ctypes.POINTER(GraphC))]),
     "mmsb_graph_free": (None, [ctypes.POINTER(GraphC)]),
@@ -183,12 +187,23 @@ def stream_handle(stream=None) -> int | None:
     return s.cuda_stream
 
 
-def spmm_f32(rowptr, colidx, vals, B, Y, r: int, n: int, alpha: float = 1.0, stream=None):
-    """Y[:m, :r] = alpha * G @ B[:, :r] through mmsb_spmm_f32 (all device tensors)."""
+def spmm_f32(rowptr, colidx, vals, B, Y, r: int, n: int, alpha: float = 1.0, stream=None, order=None):
+    """Y[:m, :r] = alpha * G @ B[:, :r] through mmsb_spmm_f32, or through mmsb_spmm_f32_ordered with the
+    rows visited in `order` (all device tensors)."""
     m = rowptr.numel() - 1
-    check(lib.mmsb_spmm_f32(stream_handle(stream), m, n, colidx.numel(), ptr(rowptr), ptr(colidx), ptr(vals),
-                            float(alpha), ptr(B), B.stride(0), r, ptr(Y), Y.stride(0)))
+    args = (stream_handle(stream), m, n, colidx.numel(), ptr(rowptr), ptr(colidx), ptr(vals), float(alpha), ptr(B),
+            B.stride(0), r, ptr(Y), Y.stride(0))
+    check(lib.mmsb_spmm_f32(*args) if order is None else lib.mmsb_spmm_f32_ordered(*args, ptr(order)))
     return Y
+
+
+def csr_row_order(rowptr, stream=None):
+    """The rows of a device CSR matrix by descending length (mmsb_csr_row_order), as an int32 tensor."""
+    import torch
+    m = rowptr.numel() - 1
+    order = torch.empty(m, dtype=torch.int32, device=rowptr.device)
+    check(lib.mmsb_csr_row_order(stream_handle(stream), m, ptr(rowptr), ptr(order)))
+    return order
 
 
 def _csr_to_numpy(c: CsrC):

@@ -1,4 +1,5 @@
-// CSR transpose on the device (G^T of each adjacency block, built once after upload).
+// CSR transpose on the device (G^T of each adjacency block, built once after upload), and the
+// rows of a CSR matrix ordered by descending length (the SpMM's row schedule).
 //
 // A stable LSD radix sort (hipCUB) of (column, nonzero index) pairs orders the nonzeros by column
 // and, within a column, by their position in G -- that is, by row, because G's nonzeros are
@@ -98,4 +99,42 @@ int csr_transpose(hipStream_t s, int32_t nr, int32_t nc, int64_t nnz, const int3
   return st;
 }
 
+__global__ void row_lengths_kernel(int32_t nr, const int32_t* __restrict__ rp, int32_t* __restrict__ len) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nr) len[i] = rp[i + 1] - rp[i];
+}
+
+int csr_row_order(hipStream_t s, int32_t nr, const int32_t* rp, int32_t* order) {
+  MMSB_CHECK_ARG(nr >= 1 && rp && order, "csr_row_order: bad arguments");
+  int32_t *len = nullptr, *len_out = nullptr, *idx = nullptr;
+  void* temp = nullptr;
+  size_t temp_bytes = 0;
+  int st = MMSB_OK;
+  hipError_t e = hipMalloc(&len, sizeof(int32_t) * nr);
+  if (e == hipSuccess) e = hipMalloc(&len_out, sizeof(int32_t) * nr);
+  if (e == hipSuccess) e = hipMalloc(&idx, sizeof(int32_t) * nr);
+  if (e == hipSuccess)
+    e = hipcub::DeviceRadixSort::SortPairsDescending(nullptr, temp_bytes, len, len_out, idx, order, (int)nr, 0, 31, s);
+  if (e == hipSuccess) e = hipMalloc(&temp, std::max<size_t>(temp_bytes, 1));
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(row_lengths_kernel, dim3((unsigned)ceil_div((int64_t)nr, 256)), dim3(256), 0, s, nr, rp, len);
+    hipLaunchKernelGGL(iota_kernel, dim3((unsigned)ceil_div((int64_t)nr, 256)), dim3(256), 0, s, (int64_t)nr, idx);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess)
+    e = hipcub::DeviceRadixSort::SortPairsDescending(temp, temp_bytes, len, len_out, idx, order, (int)nr, 0, 31, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);  // the temporaries are freed below
+  if (e != hipSuccess) st = set_error(MMSB_ERR_HIP, "csr_row_order: %s", hipGetErrorString(e));
+  (void)hipFree(len);
+  (void)hipFree(len_out);
+  (void)hipFree(idx);
+  (void)hipFree(temp);
+  return st;
+}
+
 }  // namespace mmsb
+
+extern "C" int mmsb_csr_row_order(void* stream, int32_t m, const int32_t* rowptr, int32_t* order) {
+  mmsb::clear_error();
+  return mmsb::csr_row_order(static_cast<hipStream_t>(stream), m, rowptr, order);
+}

@@ -29,13 +29,18 @@ template <typename T>
 int scale_cols(hipStream_t s, int64_t n, int32_t c, T* M, int32_t ld, const double* sc);
 
 // One-sided Jacobi SVD of a square r x r FP64 matrix given column-major in A (overwritten):
-// A = U diag(sigma) V^T with sigma sorted descending; U, V column-major r x r.
-// work: jacobi_workspace_bytes(r). sweeps_out[0] = sweeps used (device int).
+// A = U diag(sigma) V^T with sigma sorted descending; U, V column-major r x r. V may be null when only
+// U and sigma are wanted (e.g. the eigenvectors of a symmetric positive definite A): the rotations are
+// then not accumulated. work: jacobi_workspace_bytes(r). sweeps_out[0] = sweeps used (device int).
 size_t jacobi_workspace_bytes(int32_t r);
 int jacobi_svd(hipStream_t s, int32_t r, double* A, double* U, double* sigma, double* V, void* work, int32_t* sweeps_out);
 
 // G^T in CSR (rpT [nc + 1], ciT [nnz], vT [nnz] if v) of the device CSR G [nr x nc]; returns after
 // the result is complete (csr_transpose.hip).
+// order[0..nr) = the rows of a CSR matrix sorted by descending length (ties in ascending row order):
+// a stable radix sort of (length, row) pairs; temporaries are allocated and freed inside, and the
+// call returns once the order is complete (csr_transpose.hip).
+int csr_row_order(hipStream_t s, int32_t nr, const int32_t* rp, int32_t* order);
 int csr_transpose(hipStream_t s, int32_t nr, int32_t nc, int64_t nnz, const int32_t* rp, const int32_t* ci, const float* v,
                   int32_t* rpT, int32_t* ciT, float* vT);
 
@@ -47,7 +52,7 @@ int diag_axpy(hipStream_t s, int64_t n, int32_t c, const double* mu, double alph
               int32_t ldo);
 int copy2d_f32(hipStream_t s, int32_t rows, int32_t cols, const float* in, int32_t ldi, float* out, int32_t ldo);
 int spmm_f32(hipStream_t s, int32_t m, int32_t n, int64_t nnz, const int32_t* rowptr, const int32_t* colidx, const float* vals,
-             float alpha, const float* B, int32_t ldb, int32_t r, float* Y, int32_t ldy);
+             float alpha, const float* B, int32_t ldb, int32_t r, float* Y, int32_t ldy, const int32_t* order = nullptr);
 
 size_t grad_workspace_bytes(int32_t k, int32_t nb);
 template <typename TY>

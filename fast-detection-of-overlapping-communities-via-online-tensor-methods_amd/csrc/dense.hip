@@ -472,10 +472,10 @@ template int gemm_nn<double, float>(hipStream_t, int64_t, int32_t, int32_t, doub
 // Right-looking Crout on the lower triangle, then R = L^T and R^{-1} by column-parallel back
 // substitution. status[0] = 0 on success, j+1 if the pivot of column j is not positive.
 // ---- Cholesky factor and its inverse (CholeskyQR of the tall whitening sketches) ----------------
-// One workgroup per factorisation; the r x r FP64 matrices stay in L2.
-// chol_kernel: right-looking blocked Cholesky of the lower triangle of G + shift trace(G) I in
-// 16-column panels: the diagonal block is factored in LDS, the rows below it solved one row per
-// thread, and the trailing lower triangle updated once per panel in 4 x 4 register tiles.
+// The r x r FP64 matrices stay in L2. Right-looking blocked Cholesky of the lower triangle of
+// G + shift trace(G) I in 16-column panels, two launches per panel: chol_panel_kernel (one
+// workgroup) factors the diagonal block in LDS and solves the rows below it, one row per thread;
+// chol_trail_kernel updates the trailing lower triangle in 64 x 64 tiles spread over workgroups.
 // R = L^T and R^{-1} = X^T with X = L^{-1}: for r <= 1024, chol_transpose_kernel writes R through
 // LDS tiles and trinv_cols_kernel solves blocks of kTrinvCols columns of X in parallel workgroups
 // (one thread per row of X); larger r falls back to trinv_kernel, one workgroup that walks PI rows
@@ -483,16 +483,14 @@ template int gemm_nn<double, float>(hipStream_t, int64_t, int32_t, int32_t, doub
 constexpr int kCholP = 16;
 constexpr int kCholThreads = 512;
 
-__global__ __launch_bounds__(kCholThreads) void chol_kernel(int32_t r, double* __restrict__ G, double shift,
-                                                            int32_t* __restrict__ status) {
+// Diagonal shift G += max(shift trace(G), 1e-300) I; clears the status word.
+__global__ __launch_bounds__(kCholThreads) void chol_shift_kernel(int32_t r, double* __restrict__ G, double shift,
+                                                                  int32_t* __restrict__ status) {
   __shared__ double red[kCholThreads];
-  __shared__ double D[kCholP][kCholP + 1];
-  __shared__ int bad;
   const int t = threadIdx.x;
   double tr = 0.0;
   for (int i = t; i < r; i += blockDim.x) tr += G[(int64_t)i * r + i];
   red[t] = tr;
-  if (t == 0) bad = 0;
   __syncthreads();
   for (int off = blockDim.x / 2; off > 0; off >>= 1) {
     if (t < off) red[t] += red[t + off];
@@ -501,97 +499,112 @@ __global__ __launch_bounds__(kCholThreads) void chol_kernel(int32_t r, double* _
   double sh = shift * red[0];
   if (!(sh > 0.0)) sh = 1e-300;
   for (int i = t; i < r; i += blockDim.x) G[(int64_t)i * r + i] += sh;
+  if (t == 0) status[0] = 0;
+}
+
+// Panel j0 (columns j0 .. j0 + pb): the diagonal block factored in LDS, then the rows below it solved
+// against it, L21 = A21 L11^{-T}, one row per thread. A non-positive pivot is replaced by 1 and the
+// first such column (1-based) is recorded in status[0].
+__global__ __launch_bounds__(kCholThreads) void chol_panel_kernel(int32_t r, double* __restrict__ G, int32_t j0,
+                                                                  int32_t* __restrict__ status) {
+  __shared__ double D[kCholP][kCholP + 1];
+  __shared__ int bad;
+  const int t = threadIdx.x;
+  const int pb = min(kCholP, r - j0);
+  if (t == 0) bad = 0;
+  if (t < kCholP * kCholP) {
+    const int a = t / kCholP, b = t % kCholP;
+    if (a < pb && b <= a) D[a][b] = G[(int64_t)(j0 + a) * r + j0 + b];
+  }
   __syncthreads();
-  for (int j0 = 0; j0 < r; j0 += kCholP) {
-    const int pb = min(kCholP, r - j0);
-    // 1. diagonal block
-    if (t < kCholP * kCholP) {
-      const int a = t / kCholP, b = t % kCholP;
-      if (a < pb && b <= a) D[a][b] = G[(int64_t)(j0 + a) * r + j0 + b];
-    }
-    __syncthreads();
-    for (int c = 0; c < pb; ++c) {
-      if (t == 0) {
-        const double d = D[c][c];
-        if (!(d > 0.0)) {
-          if (!bad) bad = j0 + c + 1;
-          D[c][c] = 1.0;
-        } else {
-          D[c][c] = sqrt(d);
-        }
-      }
-      __syncthreads();
-      if (t > c && t < pb) D[t][c] /= D[c][c];
-      __syncthreads();
-      if (t < kCholP * kCholP) {
-        const int a = t / kCholP, b = t % kCholP;
-        if (a < pb && b > c && b <= a) D[a][b] -= D[a][c] * D[b][c];
-      }
-      __syncthreads();
-    }
-    if (t < kCholP * kCholP) {
-      const int a = t / kCholP, b = t % kCholP;
-      if (a < pb && b <= a) G[(int64_t)(j0 + a) * r + j0 + b] = D[a][b];
-    }
-    // 2. rows below the diagonal block: L21 = A21 L11^{-T}
-    for (int i = j0 + pb + t; i < r; i += blockDim.x) {
-      __asm__ volatile("" ::: "memory");  // re-read D per row instead of holding 136 values in registers
-      double* row = G + (int64_t)i * r + j0;
-      double x[kCholP];
-#pragma unroll
-      for (int c = 0; c < kCholP; ++c) x[c] = c < pb ? row[c] : 0.0;
-#pragma unroll
-      for (int c = 0; c < kCholP; ++c) {
-        if (c < pb) {
-          double v = x[c];
-#pragma unroll
-          for (int cc = 0; cc < c; ++cc) v -= x[cc] * D[c][cc];
-          x[c] = v / D[c][c];
-          row[c] = x[c];
-        }
+  for (int c = 0; c < pb; ++c) {
+    if (t == 0) {
+      const double d = D[c][c];
+      if (!(d > 0.0)) {
+        if (!bad) bad = j0 + c + 1;
+        D[c][c] = 1.0;
+      } else {
+        D[c][c] = sqrt(d);
       }
     }
     __syncthreads();
-    // 3. trailing lower triangle: G[i][l] -= sum_c L[i][c] L[l][c], j0 + pb <= l <= i < r
-    const int base = j0 + pb;
-    const int m = r - base;
-    const int nb4 = (m + 3) / 4;
-    const int ty = t >> 5, tx = t & 31;
-    for (int bi = ty; bi < nb4; bi += kCholThreads / 32) {
-      for (int bl = tx; bl <= bi; bl += 32) {
-        const double* pi[4];
-        const double* pl[4];
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {  // rows past r read row r - 1 and are not stored
-          pi[u] = G + (int64_t)min(base + 4 * bi + u, r - 1) * r + j0;
-          pl[u] = G + (int64_t)min(base + 4 * bl + u, r - 1) * r + j0;
-        }
-        double acc[4][4] = {};
-#pragma unroll 4
-        for (int c = 0; c < pb; ++c) {
-          double a[4], b[4];
-#pragma unroll
-          for (int u = 0; u < 4; ++u) {
-            a[u] = pi[u][c];
-            b[u] = pl[u][c];
-          }
-#pragma unroll
-          for (int u = 0; u < 4; ++u)
-#pragma unroll
-            for (int v = 0; v < 4; ++v) acc[u][v] = fma(a[u], b[v], acc[u][v]);
-        }
-#pragma unroll
-        for (int u = 0; u < 4; ++u)
-#pragma unroll
-          for (int v = 0; v < 4; ++v) {
-            const int i = base + 4 * bi + u, l = base + 4 * bl + v;
-            if (i < r && l <= i) G[(int64_t)i * r + l] -= acc[u][v];
-          }
-      }
+    if (t > c && t < pb) D[t][c] /= D[c][c];
+    __syncthreads();
+    if (t < kCholP * kCholP) {
+      const int a = t / kCholP, b = t % kCholP;
+      if (a < pb && b > c && b <= a) D[a][b] -= D[a][c] * D[b][c];
     }
     __syncthreads();
   }
-  if (t == 0) status[0] = bad;
+  if (t < kCholP * kCholP) {
+    const int a = t / kCholP, b = t % kCholP;
+    if (a < pb && b <= a) G[(int64_t)(j0 + a) * r + j0 + b] = D[a][b];
+  }
+  for (int i = j0 + pb + t; i < r; i += blockDim.x) {
+    __asm__ volatile("" ::: "memory");  // re-read D per row instead of holding 136 values in registers
+    double* row = G + (int64_t)i * r + j0;
+    double x[kCholP];
+#pragma unroll
+    for (int c = 0; c < kCholP; ++c) x[c] = c < pb ? row[c] : 0.0;
+#pragma unroll
+    for (int c = 0; c < kCholP; ++c) {
+      if (c < pb) {
+        double v = x[c];
+#pragma unroll
+        for (int cc = 0; cc < c; ++cc) v -= x[cc] * D[c][cc];
+        x[c] = v / D[c][c];
+        row[c] = x[c];
+      }
+    }
+  }
+  if (t == 0 && bad && status[0] == 0) status[0] = bad;
+}
+
+// Trailing update of panel j0: G[i][l] -= sum_c L[i][c] L[l][c] over the lower triangle
+// j0 + pb <= l <= i < r, one 64 x 64 tile pair (ti >= tj) per workgroup, the panel rows of both tiles
+// staged in LDS, 4 x 4 outputs per thread (the same summation order as a single-workgroup update).
+constexpr int kTrailT = 64;
+
+__global__ __launch_bounds__(256) void chol_trail_kernel(int32_t r, double* __restrict__ G, int32_t j0) {
+  __shared__ double Ls[2][kTrailT][kCholP + 1];
+  const int pb = min(kCholP, r - j0);
+  const int base = j0 + pb;
+  int rem = blockIdx.x, ti = 0;
+  while (rem > ti) {
+    rem -= ti + 1;
+    ++ti;
+  }
+  const int tj = rem;
+  const int i0 = base + ti * kTrailT, l0 = base + tj * kTrailT;
+  const int t = threadIdx.x;
+  for (int e = t; e < kTrailT * kCholP; e += blockDim.x) {
+    const int rr = e / kCholP, c = e % kCholP;
+    const int i = i0 + rr, l = l0 + rr;
+    Ls[0][rr][c] = (i < r && c < pb) ? G[(int64_t)i * r + j0 + c] : 0.0;
+    Ls[1][rr][c] = (l < r && c < pb) ? G[(int64_t)l * r + j0 + c] : 0.0;
+  }
+  __syncthreads();
+  const int bu = 4 * (t >> 4), bv = 4 * (t & 15);
+  double acc[4][4] = {};
+  for (int c = 0; c < pb; ++c) {
+    double a[4], b[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      a[u] = Ls[0][bu + u][c];
+      b[u] = Ls[1][bv + u][c];
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) acc[u][v] = fma(a[u], b[v], acc[u][v]);
+  }
+#pragma unroll
+  for (int u = 0; u < 4; ++u)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int i = i0 + bu + u, l = l0 + bv + v;
+      if (i < r && l <= i) G[(int64_t)i * r + l] -= acc[u][v];
+    }
 }
 
 template <int PI>
@@ -715,8 +728,18 @@ __global__ __launch_bounds__(1024) void trinv_cols_kernel(int32_t r, const doubl
 
 int chol_inv(hipStream_t s, int32_t r, double* G, double shift, double* R, double* Rinv, int32_t* status) {
   MMSB_CHECK_ARG(r >= 1 && r <= 4096, "chol_inv: r=%d out of range", r);
-  hipLaunchKernelGGL(chol_kernel, dim3(1), dim3(kCholThreads), 0, s, r, G, shift, status);
+  hipLaunchKernelGGL(chol_shift_kernel, dim3(1), dim3(kCholThreads), 0, s, r, G, shift, status);
   MMSB_LAUNCH_CHECK();
+  for (int j0 = 0; j0 < r; j0 += kCholP) {
+    hipLaunchKernelGGL(chol_panel_kernel, dim3(1), dim3(kCholThreads), 0, s, r, G, j0, status);
+    MMSB_LAUNCH_CHECK();
+    const int m = r - std::min(r, j0 + kCholP);
+    if (m > 0) {
+      const int tt = (m + kTrailT - 1) / kTrailT;
+      hipLaunchKernelGGL(chol_trail_kernel, dim3(tt * (tt + 1) / 2), dim3(256), 0, s, r, G, j0);
+      MMSB_LAUNCH_CHECK();
+    }
+  }
   if (r <= 1024) {
     const int nt = (r + 31) / 32;
     hipLaunchKernelGGL(chol_transpose_kernel, dim3(nt, nt), dim3(256), 0, s, r, G, R);

@@ -143,7 +143,7 @@ __global__ __launch_bounds__(kJacobiThreads) void jacobi_svd_kernel(int32_t r, d
     const double* vj = Vw + (int64_t)j * r;
     for (int i = threadIdx.x; i < r; i += blockDim.x) {
       U[(int64_t)rank * r + i] = aj[i] * inv;
-      V[(int64_t)rank * r + i] = vj[i];
+      if (V) V[(int64_t)rank * r + i] = vj[i];
     }
     if (threadIdx.x == 0) sigma[rank] = sj;
     __syncthreads();
@@ -156,14 +156,14 @@ __global__ __launch_bounds__(kJacobiThreads) void jacobi_svd_kernel(int32_t r, d
 // orthogonalises every pair among them (one inner sweep, 2B - 1 inner rounds of B disjoint pairs,
 // 256 / B threads per pair), and writes them back. A sweep needs nblk - 1 grid barriers instead
 // of r - 1. Convergence test, ordering of the output and exit protocol as above.
-template <int B>
+template <int B, bool WV>
 __global__ __launch_bounds__(kJacobiThreads) void jacobi_block_kernel(int32_t r, double* __restrict__ A,
                                                                        double* __restrict__ U, double* __restrict__ sigma,
                                                                        double* __restrict__ V, double* __restrict__ Vw,
                                                                        double* __restrict__ sw, int32_t* __restrict__ counters,
                                                                        int32_t* __restrict__ sweeps_out) {
   constexpr int TPG = kJacobiThreads / B;  // threads per pair (<= 64: one wave or half)
-  extern __shared__ double lds[];          // [2B][r] columns of A, then [2B][r] columns of V
+  extern __shared__ double lds[];          // [2B][r] columns of A, then (WV) [2B][r] columns of V
   cg::grid_group grid = cg::this_grid();
   __shared__ double red[kJacobiThreads / 64];
   __shared__ int go_on, nrot;
@@ -173,7 +173,8 @@ __global__ __launch_bounds__(kJacobiThreads) void jacobi_block_kernel(int32_t r,
   const int64_t rr = (int64_t)r * r;
   const int64_t gtid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t gsz = (int64_t)nwg * blockDim.x;
-  for (int64_t e = gtid; e < rr; e += gsz) Vw[e] = ((e / r) == (e % r)) ? 1.0 : 0.0;
+  if constexpr (WV)
+    for (int64_t e = gtid; e < rr; e += gsz) Vw[e] = ((e / r) == (e % r)) ? 1.0 : 0.0;
   for (int64_t e = gtid; e < kMaxSweeps + 2; e += gsz) counters[e] = 0;
   grid.sync();
 
@@ -197,7 +198,7 @@ __global__ __launch_bounds__(kJacobiThreads) void jacobi_block_kernel(int32_t r,
           if (c < 0) continue;
           for (int i = threadIdx.x; i < r; i += blockDim.x) {
             la[(int64_t)lc * r + i] = A[(int64_t)c * r + i];
-            lv[(int64_t)lc * r + i] = Vw[(int64_t)c * r + i];
+            if constexpr (WV) lv[(int64_t)lc * r + i] = Vw[(int64_t)c * r + i];
           }
         }
         if (threadIdx.x == 0) nrot = 0;
@@ -231,12 +232,14 @@ __global__ __launch_bounds__(kJacobiThreads) void jacobi_block_kernel(int32_t r,
                 ap[i] = c * x - sn * y;
                 aq[i] = sn * x + c * y;
               }
-              double* vp = lv + (int64_t)lp * r;
-              double* vq = lv + (int64_t)lq * r;
-              for (int i = tg; i < r; i += TPG) {
-                const double x = vp[i], y = vq[i];
-                vp[i] = c * x - sn * y;
-                vq[i] = sn * x + c * y;
+              if constexpr (WV) {
+                double* vp = lv + (int64_t)lp * r;
+                double* vq = lv + (int64_t)lq * r;
+                for (int i = tg; i < r; i += TPG) {
+                  const double x = vp[i], y = vq[i];
+                  vp[i] = c * x - sn * y;
+                  vq[i] = sn * x + c * y;
+                }
               }
               if (tg == 0) ++myrot;
             }
@@ -249,7 +252,7 @@ __global__ __launch_bounds__(kJacobiThreads) void jacobi_block_kernel(int32_t r,
           if (c < 0) continue;
           for (int i = threadIdx.x; i < r; i += blockDim.x) {
             A[(int64_t)c * r + i] = la[(int64_t)lc * r + i];
-            Vw[(int64_t)c * r + i] = lv[(int64_t)lc * r + i];
+            if constexpr (WV) Vw[(int64_t)c * r + i] = lv[(int64_t)lc * r + i];
           }
         }
         __syncthreads();
@@ -281,10 +284,9 @@ __global__ __launch_bounds__(kJacobiThreads) void jacobi_block_kernel(int32_t r,
     const int rank = (int)block_sum((double)cnt, red);
     const double inv = sj > 0.0 ? 1.0 / sj : 0.0;
     const double* aj = A + (int64_t)j * r;
-    const double* vj = Vw + (int64_t)j * r;
     for (int i = threadIdx.x; i < r; i += blockDim.x) {
       U[(int64_t)rank * r + i] = aj[i] * inv;
-      V[(int64_t)rank * r + i] = vj[i];
+      if constexpr (WV) V[(int64_t)rank * r + i] = Vw[(int64_t)j * r + i];
     }
     if (threadIdx.x == 0) sigma[rank] = sj;
     __syncthreads();
@@ -292,11 +294,11 @@ __global__ __launch_bounds__(kJacobiThreads) void jacobi_block_kernel(int32_t r,
   if (blockIdx.x == 0 && threadIdx.x == 0) sweeps_out[0] = sweep + (sweep < kMaxSweeps ? 1 : 0);
 }
 
-template <int B>
+template <int B, bool WV>
 static int launch_block(hipStream_t s, int32_t r, double* A, double* U, double* sigma, double* V, double* Vw, double* sw,
                         int32_t* counters, int32_t* sweeps_out) {
-  const void* fn = reinterpret_cast<const void*>(jacobi_block_kernel<B>);
-  const size_t shm = sizeof(double) * (size_t)4 * B * r;
+  const void* fn = reinterpret_cast<const void*>(jacobi_block_kernel<B, WV>);
+  const size_t shm = sizeof(double) * (size_t)(WV ? 4 : 2) * B * r;
   MMSB_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
   int dev = 0, per_cu = 0, cus = 0;
   MMSB_HIP(hipGetDevice(&dev));
@@ -316,10 +318,15 @@ int jacobi_svd(hipStream_t s, int32_t r, double* A, double* U, double* sigma, do
   double* Vw = static_cast<double*>(work);
   double* sw = Vw + (size_t)r * r;
   int32_t* counters = reinterpret_cast<int32_t*>(sw + r);
-  // block variant while 2B columns of A and V (4 B r doubles) fit in the 160 KiB of LDS
-  auto fits = [&](int b) { return sizeof(double) * (size_t)4 * b * r + 256 <= (size_t)160 * 1024; };
-  if (r >= 64 && fits(8)) return launch_block<8>(s, r, A, U, sigma, V, Vw, sw, counters, sweeps_out);
-  if (r >= 64 && fits(4)) return launch_block<4>(s, r, A, U, sigma, V, Vw, sw, counters, sweeps_out);
+  // block variant while 2B columns of A (and of V when it is wanted) fit in the 160 KiB of LDS; without
+  // V the blocks can be twice as wide, which halves the rounds (and grid barriers) per sweep
+  auto fits = [&](int b, int arrays) { return sizeof(double) * (size_t)2 * arrays * b * r + 256 <= (size_t)160 * 1024; };
+  if (!V) {
+    if (r >= 64 && fits(16, 1)) return launch_block<16, false>(s, r, A, U, sigma, V, Vw, sw, counters, sweeps_out);
+    if (r >= 64 && fits(8, 1)) return launch_block<8, false>(s, r, A, U, sigma, V, Vw, sw, counters, sweeps_out);
+  }
+  if (r >= 64 && fits(8, 2)) return launch_block<8, true>(s, r, A, U, sigma, V, Vw, sw, counters, sweeps_out);
+  if (r >= 64 && fits(4, 2)) return launch_block<4, true>(s, r, A, U, sigma, V, Vw, sw, counters, sweeps_out);
   int dev = 0;
   MMSB_HIP(hipGetDevice(&dev));
   int per_cu = 0, cus = 0;

@@ -59,6 +59,7 @@ struct DevCsr {
   int32_t* rowptr = nullptr;
   int32_t* colidx = nullptr;
   float* vals = nullptr;
+  int32_t* order = nullptr;  // rows by descending length: the SpMM's schedule
 };
 
 static int64_t r64(int64_t a, int64_t b) { return (a + b - 1) / b * b; }
@@ -85,7 +86,7 @@ struct mmsb_ctx {
   int32_t* dstat = nullptr;  // [8] chol status, jacobi sweeps
   double *Ut = nullptr, *Vt = nullptr, *sig = nullptr, *sinv = nullptr, *sgn = nullptr, *shift = nullptr;
   unsigned long long* best = nullptr;
-  double *P1 = nullptr, *P2 = nullptr, *mrow = nullptr, *S = nullptr, *E = nullptr, *E2 = nullptr, *lamv = nullptr,
+  double *P1 = nullptr, *P2 = nullptr, *mrow = nullptr, *S = nullptr, *E = nullptr, *lamv = nullptr,
          *lamis = nullptr;
   double* mu[3] = {nullptr, nullptr, nullptr};
   float *W = nullptr, *U = nullptr, *V = nullptr, *LB = nullptr, *LC = nullptr, *MB = nullptr, *MC = nullptr;
@@ -164,7 +165,8 @@ static int upload_csr(mmsb_ctx* c, DevCsr& d, int32_t nr, int32_t nc, int64_t nn
     TRY(dalloc(c, d.vals, (size_t)std::max<int64_t>(nnz, 1)));
     TRY(h2d(c->s, d.vals, v, sizeof(float) * (size_t)nnz));
   }
-  return MMSB_OK;
+  TRY(dalloc(c, d.order, (size_t)nr));
+  return csr_row_order(c->s, nr, d.rowptr, d.order);
 }
 
 // G^T built on the device from the uploaded G.
@@ -175,11 +177,13 @@ static int transpose_csr(mmsb_ctx* c, const DevCsr& g, DevCsr& t) {
   TRY(dalloc(c, t.rowptr, (size_t)t.n_rows + 1));
   TRY(dalloc(c, t.colidx, (size_t)std::max<int64_t>(t.nnz, 1)));
   if (g.vals) TRY(dalloc(c, t.vals, (size_t)std::max<int64_t>(t.nnz, 1)));
-  return csr_transpose(c->s, g.n_rows, g.n_cols, g.nnz, g.rowptr, g.colidx, g.vals, t.rowptr, t.colidx, t.vals);
+  TRY(csr_transpose(c->s, g.n_rows, g.n_cols, g.nnz, g.rowptr, g.colidx, g.vals, t.rowptr, t.colidx, t.vals));
+  TRY(dalloc(c, t.order, (size_t)t.n_rows));
+  return csr_row_order(c->s, t.n_rows, t.rowptr, t.order);
 }
 
 static int spmm(mmsb_ctx* c, const DevCsr& g, float alpha, const float* B, int32_t ldb, int32_t r, float* Y, int32_t ldy) {
-  return spmm_f32(c->s, g.n_rows, g.n_cols, g.nnz, g.rowptr, g.colidx, g.vals, alpha, B, ldb, r, Y, ldy);
+  return spmm_f32(c->s, g.n_rows, g.n_cols, g.nnz, g.rowptr, g.colidx, g.vals, alpha, B, ldb, r, Y, ldy, g.order);
 }
 
 // ---- CholeskyQR2: Q = X R^{-1} (Q may alias X); optionally R = R2 R1 -------------------------
@@ -255,7 +259,6 @@ static int alloc_small(mmsb_ctx* c) {
   TRY(dalloc(c, c->mrow, rm));
   TRY(dalloc(c, c->S, (size_t)rm * rm));
   TRY(dalloc(c, c->E, (size_t)rm * rm));
-  TRY(dalloc(c, c->E2, (size_t)rm * rm));
   TRY(dalloc(c, c->lamv, rm));
   TRY(dalloc(c, c->lamis, rm));
   return MMSB_OK;
@@ -349,7 +352,7 @@ static int whiten_impl(mmsb_ctx* c) {
   TRY(m2_apply(c, T[qbuf], T[1]));
   TRY((gemm_tn<float, float>(s, na, r, r, 1.0, T[qbuf], ld, T[1], ld, 0.0, c->S, c->work)));
   TRY(sym_shift(s, r, c->S, c->shift));
-  TRY(jacobi_svd(s, r, c->S, c->E, c->lamv, c->E2, c->jwork, c->dstat + 7));
+  TRY(jacobi_svd(s, r, c->S, c->E, c->lamv, nullptr, c->jwork, c->dstat + 7));  // S SPD: eigenvectors = U
   double hshift = 0.0;
   HCHK(hipMemcpyAsync(&hshift, c->shift, sizeof(double), hipMemcpyDeviceToHost, s));
   HCHK(hipStreamSynchronize(s));

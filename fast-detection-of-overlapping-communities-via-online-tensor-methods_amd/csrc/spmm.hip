@@ -6,7 +6,9 @@
 // DGEMMs (culaBlockDgemm, gemm_pinv_nys.c:27-145; pre_proc.c:154-366; post_proc.c:87-106);
 // here the adjacency blocks stay in CSR and B is a tall n x r matrix.
 //
-// Mapping: one wave64 per output row. The row's column indices are read 64 at a time with one
+// Mapping: one wave64 per output row, rows taken in the order of an optional permutation (the
+// pipeline passes rows sorted by descending length, so that the longest rows -- hub nodes with
+// thousands of nonzeros on degree-corrected graphs -- start first instead of forming the tail). The row's column indices are read 64 at a time with one
 // coalesced load and broadcast with ds_bpermute (__shfl). Each lane owns float4 column chunks of
 // the dense rows it gathers, so one gather of a B row is a single contiguous 16*64-byte access
 // per wave instruction. When r/4 < 64 the wave is split into G = 64/LPG lane groups that gather
@@ -23,12 +25,14 @@ __global__ __launch_bounds__(256) void spmm_csr_rowwave(int32_t m, const int32_t
                                                         const int32_t* __restrict__ colidx,
                                                         const float* __restrict__ vals, float alpha,
                                                         const float* __restrict__ B, int32_t ldb,
-                                                        int32_t r, float* __restrict__ Y, int32_t ldy) {
+                                                        int32_t r, float* __restrict__ Y, int32_t ldy,
+                                                        const int32_t* __restrict__ order) {
   const int32_t r4 = (r + 3) >> 2;
   constexpr int G = 64 / LPG;  // nonzeros gathered concurrently by one wave
   const int lane = threadIdx.x & 63;
-  const int64_t row = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-  if (row >= m) return;  // uniform per wave
+  const int64_t slot = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (slot >= m) return;  // uniform per wave
+  const int64_t row = order ? order[slot] : slot;
   const int grp = lane / LPG;
   const int gl = lane % LPG;
 
@@ -114,22 +118,22 @@ __global__ __launch_bounds__(256) void spmm_csr_rowwave(int32_t m, const int32_t
 template <int LPG, int P>
 static int launch_spmm(hipStream_t s, int32_t m, const int32_t* rowptr, const int32_t* colidx,
                        const float* vals, float alpha, const float* B, int32_t ldb, int32_t r,
-                       float* Y, int32_t ldy) {
+                       float* Y, int32_t ldy, const int32_t* order) {
   const int waves_per_block = 4;
   const dim3 grid((unsigned)ceil_div(m, waves_per_block));
   if (vals)
     hipLaunchKernelGGL((spmm_csr_rowwave<LPG, P, true>), grid, dim3(64 * waves_per_block), 0, s, m,
-                       rowptr, colidx, vals, alpha, B, ldb, r, Y, ldy);
+                       rowptr, colidx, vals, alpha, B, ldb, r, Y, ldy, order);
   else
     hipLaunchKernelGGL((spmm_csr_rowwave<LPG, P, false>), grid, dim3(64 * waves_per_block), 0, s, m,
-                       rowptr, colidx, vals, alpha, B, ldb, r, Y, ldy);
+                       rowptr, colidx, vals, alpha, B, ldb, r, Y, ldy, order);
   MMSB_LAUNCH_CHECK();
   return MMSB_OK;
 }
 
 int spmm_f32(hipStream_t s, int32_t m, int32_t n, int64_t nnz, const int32_t* rowptr,
              const int32_t* colidx, const float* vals, float alpha, const float* B, int32_t ldb,
-             int32_t r, float* Y, int32_t ldy) {
+             int32_t r, float* Y, int32_t ldy, const int32_t* order) {
   MMSB_CHECK_ARG(m >= 0 && n >= 0 && nnz >= 0 && nnz < (int64_t)INT32_MAX, "spmm: bad sizes m=%d n=%d nnz=%lld", m, n, (long long)nnz);
   MMSB_CHECK_ARG(r >= 1 && r <= ldb && r <= ldy, "spmm: need 1 <= r <= ldb, ldy (r=%d ldb=%d ldy=%d)", r, ldb, ldy);
   MMSB_CHECK_ARG(ldb % 4 == 0 && ldy % 4 == 0, "spmm: ldb and ldy must be multiples of 4");
@@ -138,18 +142,18 @@ int spmm_f32(hipStream_t s, int32_t m, int32_t n, int64_t nnz, const int32_t* ro
   MMSB_CHECK_ARG(rowptr && Y && (nnz == 0 || (colidx && B)), "spmm: null pointer");
   MMSB_CHECK_ARG(aligned16(B) && aligned16(Y), "spmm: B and Y must be 16-byte aligned");
   const int32_t r4 = (r + 3) / 4;
-  if (r4 <= 1) return launch_spmm<1, 1>(s, m, rowptr, colidx, vals, alpha, B, ldb, r, Y, ldy);
-  if (r4 <= 2) return launch_spmm<2, 1>(s, m, rowptr, colidx, vals, alpha, B, ldb, r, Y, ldy);
-  if (r4 <= 4) return launch_spmm<4, 1>(s, m, rowptr, colidx, vals, alpha, B, ldb, r, Y, ldy);
-  if (r4 <= 8) return launch_spmm<8, 1>(s, m, rowptr, colidx, vals, alpha, B, ldb, r, Y, ldy);
-  if (r4 <= 16) return launch_spmm<16, 1>(s, m, rowptr, colidx, vals, alpha, B, ldb, r, Y, ldy);
-  if (r4 <= 32) return launch_spmm<32, 1>(s, m, rowptr, colidx, vals, alpha, B, ldb, r, Y, ldy);
-  if (r4 <= 64) return launch_spmm<64, 1>(s, m, rowptr, colidx, vals, alpha, B, ldb, r, Y, ldy);
-  if (r4 <= 128) return launch_spmm<64, 2>(s, m, rowptr, colidx, vals, alpha, B, ldb, r, Y, ldy);
-  if (r4 <= 192) return launch_spmm<64, 3>(s, m, rowptr, colidx, vals, alpha, B, ldb, r, Y, ldy);
-  if (r4 <= 256) return launch_spmm<64, 4>(s, m, rowptr, colidx, vals, alpha, B, ldb, r, Y, ldy);
-  if (r4 <= 512) return launch_spmm<64, 8>(s, m, rowptr, colidx, vals, alpha, B, ldb, r, Y, ldy);
-  return launch_spmm<64, 16>(s, m, rowptr, colidx, vals, alpha, B, ldb, r, Y, ldy);
+  if (r4 <= 1) return launch_spmm<1, 1>(s, m, rowptr, colidx, vals, alpha, B, ldb, r, Y, ldy, order);
+  if (r4 <= 2) return launch_spmm<2, 1>(s, m, rowptr, colidx, vals, alpha, B, ldb, r, Y, ldy, order);
+  if (r4 <= 4) return launch_spmm<4, 1>(s, m, rowptr, colidx, vals, alpha, B, ldb, r, Y, ldy, order);
+  if (r4 <= 8) return launch_spmm<8, 1>(s, m, rowptr, colidx, vals, alpha, B, ldb, r, Y, ldy, order);
+  if (r4 <= 16) return launch_spmm<16, 1>(s, m, rowptr, colidx, vals, alpha, B, ldb, r, Y, ldy, order);
+  if (r4 <= 32) return launch_spmm<32, 1>(s, m, rowptr, colidx, vals, alpha, B, ldb, r, Y, ldy, order);
+  if (r4 <= 64) return launch_spmm<64, 1>(s, m, rowptr, colidx, vals, alpha, B, ldb, r, Y, ldy, order);
+  if (r4 <= 128) return launch_spmm<64, 2>(s, m, rowptr, colidx, vals, alpha, B, ldb, r, Y, ldy, order);
+  if (r4 <= 192) return launch_spmm<64, 3>(s, m, rowptr, colidx, vals, alpha, B, ldb, r, Y, ldy, order);
+  if (r4 <= 256) return launch_spmm<64, 4>(s, m, rowptr, colidx, vals, alpha, B, ldb, r, Y, ldy, order);
+  if (r4 <= 512) return launch_spmm<64, 8>(s, m, rowptr, colidx, vals, alpha, B, ldb, r, Y, ldy, order);
+  return launch_spmm<64, 16>(s, m, rowptr, colidx, vals, alpha, B, ldb, r, Y, ldy, order);
 }
 
 }  // namespace mmsb
@@ -159,5 +163,14 @@ extern "C" int mmsb_spmm_f32(void* stream, int32_t m, int32_t n, int64_t nnz, co
                              int32_t ldb, int32_t r, float* Y, int32_t ldy) {
   mmsb::clear_error();
   return mmsb::spmm_f32(static_cast<hipStream_t>(stream), m, n, nnz, rowptr, colidx, vals, alpha, B,
-                        ldb, r, Y, ldy);
+                        ldb, r, Y, ldy, nullptr);
+}
+
+extern "C" int mmsb_spmm_f32_ordered(void* stream, int32_t m, int32_t n, int64_t nnz, const int32_t* rowptr,
+                                     const int32_t* colidx, const float* vals, float alpha, const float* B,
+                                     int32_t ldb, int32_t r, float* Y, int32_t ldy, const int32_t* row_order) {
+  mmsb::clear_error();
+  MMSB_CHECK_ARG(row_order, "spmm_ordered: null row order");
+  return mmsb::spmm_f32(static_cast<hipStream_t>(stream), m, n, nnz, rowptr, colidx, vals, alpha, B,
+                        ldb, r, Y, ldy, row_order);
 }

@@ -59,6 +59,15 @@ const char* mmsb_version(void);
 int mmsb_spmm_f32(void* stream, int32_t m, int32_t n, int64_t nnz, const int32_t* rowptr,
                   const int32_t* colidx, const float* vals, float alpha, const float* B,
                   int32_t ldb, int32_t r, float* Y, int32_t ldy);
+/* The same product with the rows visited in the order row_order[0..m) (a permutation of the rows;
+ * Y is identical). No counterpart in the reference: with rows sorted by descending length
+ * (mmsb_csr_row_order) the longest rows start first instead of forming the kernel's tail. */
+int mmsb_spmm_f32_ordered(void* stream, int32_t m, int32_t n, int64_t nnz, const int32_t* rowptr,
+                          const int32_t* colidx, const float* vals, float alpha, const float* B,
+                          int32_t ldb, int32_t r, float* Y, int32_t ldy, const int32_t* row_order);
+/* order[0..m) = the rows of the device CSR matrix rowptr[m+1] sorted by descending length, ties in
+ * ascending row order. Allocates and frees its temporaries; returns when the order is complete. */
+int mmsb_csr_row_order(void* stream, int32_t m, const int32_t* rowptr, int32_t* order);
 
 
 /*

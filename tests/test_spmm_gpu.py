@@ -73,3 +73,31 @@ def test_spmm_row_window(mmsb, dev):
     torch.cuda.synchronize()
     full = csr_spmm(rowptr, colidx, None, B.astype(np.float64), m)
     np.testing.assert_allclose(Y.cpu().numpy(), full[40:50], rtol=1e-5, atol=1e-5)
+
+
+@pytest.mark.parametrize("r", [20, 510])
+def test_row_order_and_ordered_spmm(mmsb, dev, r):
+    """mmsb_csr_row_order sorts rows by descending length with ties in row order (compared with a
+    stable numpy argsort), and the SpMM that visits rows in that order writes the same Y bit for bit."""
+    import torch
+    rng = np.random.default_rng(40 + r)
+    m, n = 3000, 800
+    lens = np.minimum(rng.pareto(1.5, m) * 3, n - 1).astype(np.int64)
+    lens[[0, 17, 2999]] = 0                                   # empty rows, first and last included
+    rowptr = np.zeros(m + 1, np.int32)
+    rowptr[1:] = np.cumsum(lens)
+    colidx = np.concatenate([np.sort(rng.choice(n, size=int(l), replace=False)) for l in lens]).astype(np.int32)
+    B = rng.standard_normal((n, (r + 3) // 4 * 4)).astype(np.float32)
+    t = lambda a: torch.from_numpy(a).to(dev)
+    d_rp, d_ci, d_B = t(rowptr), t(colidx), t(B)
+    order = mmsb.csr_row_order(d_rp)
+    torch.cuda.synchronize()
+    assert np.array_equal(order.cpu().numpy(), np.argsort(-lens, kind="stable"))
+    Y0 = torch.empty((m, B.shape[1]), dtype=torch.float32, device=dev)
+    Y1 = torch.empty_like(Y0)
+    mmsb.spmm_f32(d_rp, d_ci, None, d_B, Y0, r=r, n=n)
+    mmsb.spmm_f32(d_rp, d_ci, None, d_B, Y1, r=r, n=n, order=order)
+    torch.cuda.synchronize()
+    assert torch.equal(Y0[:, :r], Y1[:, :r])
+    np.testing.assert_allclose(Y1[:, :r].cpu().numpy(), csr_spmm(rowptr, colidx, None, B[:, :r].astype(np.float64), m),
+                               rtol=1e-4, atol=1e-4)
