@@ -381,3 +381,32 @@ def test_batch_matches_single_instances(mmsb, prec, degree):
     batch.close()
     for p in singles + members:
         p.close()
+
+
+@pytest.mark.parametrize("prec", ["fp32", "bf16"])
+def test_c1_recovers_planted_communities(mmsb, prec):
+    """The whole pipeline on the 1000-node-per-partition, k = 10 workload (alpha0 = 1, P = 0.6 / 0.02,
+    minibatch 100, eta = 1e-3, 200 epochs): the normalised memberships of the X partition are scored
+    against the generator's Dirichlet memberships with the p-value test of CPU/Pvalue.cpp."""
+    import importlib
+    ev = importlib.import_module("mmsb_amd.evaluate")
+    n, k = 1000, 10
+    g = mmsb.generate(n, n, n, n, k, 1.0, 0.6, 0.02, seed=0, keep_truth=True)
+    cfg = mmsb.config(nx=n, na=n, nb=n, nc=n, ncom=k, alpha_0=1.0, learn_rate=1e-3, minibatch=100, epochs=200,
+                      polish_epochs=1 if prec == "bf16" else 0, oversample=10, power_iters=2, seed=1, precision=prec)
+    print(f"case prec={prec}", flush=True)
+    p = mmsb.Pipeline(cfg)
+    p.set_graph(g)
+    p.whiten()
+    p.stgd(-1, prec)
+    p.eigen()
+    p.post()
+    x_nodes = np.nonzero(g["part"] == 0)[0]
+    truth = np.zeros((n, k))
+    truth[g["local"][x_nodes]] = g["pi"][x_nodes]
+    res = ev.evaluate(p.get("pin_x"), truth)
+    best = res["correlations"].max(0)                 # strongest positive match of each true community
+    print(f"recovery {res['recovery_ratio']:.2f} error {res['error']:.4f} matches {res['matches']} "
+          f"best correlations min {best.min():.3f} median {np.median(best):.3f}", flush=True)
+    assert res["recovery_ratio"] == 1.0 and res["error"] < 0.2
+    p.close()
