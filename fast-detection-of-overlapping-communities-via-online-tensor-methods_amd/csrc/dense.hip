@@ -467,10 +467,104 @@ template int gemm_nn<float, double>(hipStream_t, int64_t, int32_t, int32_t, doub
 template int gemm_nn<double, float>(hipStream_t, int64_t, int32_t, int32_t, double, const double*, int32_t,
                                     const double*, int32_t, double, float*, int32_t, bool);
 
-// ---- Cholesky of G + shift*trace(G)*I and the inverse of its upper factor ------------------
-// One workgroup of 1024 threads; G is r x r row-major FP64 in global memory (L2-resident).
-// Right-looking Crout on the lower triangle, then R = L^T and R^{-1} by column-parallel back
-// substitution. status[0] = 0 on success, j+1 if the pivot of column j is not positive.
+// ---- FP32-MFMA tall product for well-conditioned small factors -----------------------------------
+// Y [n x q] = X [n x p] C [p x q] with X, Y FP32 and C FP64 rounded to FP32 (CT: C given as [q x p]),
+// on v_mfma_f32_32x32x2f32 with FP32 sums: 128 x 128 workgroup tiles, four waves of 64 x 64 (2 x 2
+// MFMA blocks; lane l supplies A[l % 32][l / 32] and B[l / 32][l % 32], accumulator register i of
+// lane l holds D[8 (i / 4) + 4 (l / 32) + i % 4][l % 32]), 16-deep k stages through LDS. Meant for
+// the second CholeskyQR pass (C = R2^{-1}, R2 = I + O(kappa eps)) and products with orthogonal r x r
+// factors, where the FP32 rounding of C and the FP32 sums (~sqrt(p) eps_32 relative) stay at the
+// FP32 storage level of Y.
+constexpr int kFT = 128, kFK = 16, kFPad = kFT + 4;
+
+template <bool CT>
+__global__ __launch_bounds__(256, 2) void nn_f32mfma_kernel(int64_t n, int32_t p, int32_t q, const float* __restrict__ X,
+                                                           int32_t ldx, const double* __restrict__ C, int32_t ldc,
+                                                           float* __restrict__ Y, int32_t ldy) {
+  __shared__ float Xs[kFK][kFPad];  // Xs[k][row]
+  __shared__ float Cs[kFK][kFPad];  // Cs[k][col]
+  const int64_t r0 = (int64_t)blockIdx.x * kFT;
+  const int c0 = blockIdx.y * kFT;
+  const int t = threadIdx.x, w = t >> 6, l = t & 63;
+  const int ai = 64 * (w >> 1), bj = 64 * (w & 1);
+  const int xk = t & 15, xr = t >> 4;  // stage map: k = xk, rows xr + 16 e
+  float rx[8], rc[8];
+  auto load = [&](int kb) {
+    const bool kv = kb + xk < p;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      const int64_t row = r0 + xr + 16 * e;
+      rx[e] = (kv && row < n) ? X[row * ldx + kb + xk] : 0.f;
+    }
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      if constexpr (CT) {  // C[col][k]: k = xk, cols xr + 16 e
+        const int col = c0 + xr + 16 * e;
+        rc[e] = (kv && col < q) ? (float)C[(int64_t)col * ldc + kb + xk] : 0.f;
+      } else {             // C[k][col]: k = xr, cols xk + 16 e
+        const int k = kb + xr, col = c0 + xk + 16 * e;
+        rc[e] = (k < p && col < q) ? (float)C[(int64_t)k * ldc + col] : 0.f;
+      }
+    }
+  };
+  floatx16 acc[2][2];
+#pragma unroll
+  for (int u = 0; u < 2; ++u)
+#pragma unroll
+    for (int v = 0; v < 2; ++v) acc[u][v] = floatx16{};
+  load(0);
+  for (int kb = 0; kb < p; kb += kFK) {
+    __syncthreads();
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      Xs[xk][xr + 16 * e] = rx[e];
+      if constexpr (CT)
+        Cs[xk][xr + 16 * e] = rc[e];
+      else
+        Cs[xr][xk + 16 * e] = rc[e];
+    }
+    __syncthreads();
+    if (kb + kFK < p) load(kb + kFK);
+#pragma unroll
+    for (int kk = 0; kk < kFK; kk += 2) {
+      const int kl = kk + (l >> 5);
+      float a[2], b[2];
+#pragma unroll
+      for (int u = 0; u < 2; ++u) {
+        a[u] = Xs[kl][ai + 32 * u + (l & 31)];
+        b[u] = Cs[kl][bj + 32 * u + (l & 31)];
+      }
+#pragma unroll
+      for (int u = 0; u < 2; ++u)
+#pragma unroll
+        for (int v = 0; v < 2; ++v) acc[u][v] = __builtin_amdgcn_mfma_f32_32x32x2f32(a[u], b[v], acc[u][v], 0, 0, 0);
+    }
+  }
+#pragma unroll
+  for (int u = 0; u < 2; ++u)
+#pragma unroll
+    for (int v = 0; v < 2; ++v)
+#pragma unroll
+      for (int i = 0; i < 16; ++i) {
+        const int64_t row = r0 + ai + 32 * u + 8 * (i >> 2) + 4 * (l >> 5) + (i & 3);
+        const int col = c0 + bj + 32 * v + (l & 31);
+        if (row < n && col < q) Y[row * ldy + col] = acc[u][v][i];
+      }
+}
+
+int gemm_nn_f32(hipStream_t s, int64_t n, int32_t p, int32_t q, const float* X, int32_t ldx, const double* C, int32_t ldc,
+                float* Y, int32_t ldy, bool c_trans) {
+  MMSB_CHECK_ARG(n >= 0 && p >= 1 && q >= 1 && ldx >= p && ldy >= q && ldc >= (c_trans ? p : q), "gemm_nn_f32: bad shape");
+  if (n == 0) return MMSB_OK;
+  const dim3 g((unsigned)ceil_div(n, kFT), (unsigned)ceil_div(q, kFT));
+  if (c_trans)
+    hipLaunchKernelGGL(nn_f32mfma_kernel<true>, g, dim3(256), 0, s, n, p, q, X, ldx, C, ldc, Y, ldy);
+  else
+    hipLaunchKernelGGL(nn_f32mfma_kernel<false>, g, dim3(256), 0, s, n, p, q, X, ldx, C, ldc, Y, ldy);
+  MMSB_LAUNCH_CHECK();
+  return MMSB_OK;
+}
+
 // ---- Cholesky factor and its inverse (CholeskyQR of the tall whitening sketches) ----------------
 // The r x r FP64 matrices stay in L2. Right-looking blocked Cholesky of the lower triangle of
 // G + shift trace(G) I in 16-column panels, two launches per panel: chol_panel_kernel (one

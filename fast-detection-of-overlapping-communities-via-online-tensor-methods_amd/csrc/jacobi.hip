@@ -318,13 +318,11 @@ int jacobi_svd(hipStream_t s, int32_t r, double* A, double* U, double* sigma, do
   double* Vw = static_cast<double*>(work);
   double* sw = Vw + (size_t)r * r;
   int32_t* counters = reinterpret_cast<int32_t*>(sw + r);
-  // block variant while 2B columns of A (and of V when it is wanted) fit in the 160 KiB of LDS; without
-  // V the blocks can be twice as wide, which halves the rounds (and grid barriers) per sweep
+  // block variant while 2B columns of A (and of V when it is wanted) fit in the 160 KiB of LDS. Without
+  // V the rotations touch half the data. (16-column blocks, which V-free would allow at r = 510, were
+  // slower: 51.8 against 48 ms, the longer inner rounds outweighing the halved grid barriers.)
   auto fits = [&](int b, int arrays) { return sizeof(double) * (size_t)2 * arrays * b * r + 256 <= (size_t)160 * 1024; };
-  if (!V) {
-    if (r >= 64 && fits(16, 1)) return launch_block<16, false>(s, r, A, U, sigma, V, Vw, sw, counters, sweeps_out);
-    if (r >= 64 && fits(8, 1)) return launch_block<8, false>(s, r, A, U, sigma, V, Vw, sw, counters, sweeps_out);
-  }
+  if (!V && r >= 64 && fits(8, 1)) return launch_block<8, false>(s, r, A, U, sigma, V, Vw, sw, counters, sweeps_out);
   if (r >= 64 && fits(8, 2)) return launch_block<8, true>(s, r, A, U, sigma, V, Vw, sw, counters, sweeps_out);
   if (r >= 64 && fits(4, 2)) return launch_block<4, true>(s, r, A, U, sigma, V, Vw, sw, counters, sweeps_out);
   int dev = 0;

@@ -187,6 +187,15 @@ static int spmm(mmsb_ctx* c, const DevCsr& g, float alpha, const float* B, int32
 }
 
 // ---- CholeskyQR2: Q = X R^{-1} (Q may alias X); optionally R = R2 R1 -------------------------
+// Y = X C for a well-conditioned r x r factor C (the second CholeskyQR pass, orthogonal factors):
+// FP32 MFMA on tall matrices (the FP32 sums stay at the FP32 storage level of Y), the FP64-accumulating
+// product otherwise.
+static int nn_wellcond(mmsb_ctx* c, int64_t n, int32_t p, int32_t q, const float* X, int32_t ldx, const double* C,
+                       int32_t ldc, float* Y, int32_t ldy, bool c_trans) {
+  if (n >= 4096) return gemm_nn_f32(c->s, n, p, q, X, ldx, C, ldc, Y, ldy, c_trans);
+  return gemm_nn<float, float>(c->s, n, p, q, 1.0, X, ldx, C, ldc, 0.0, Y, ldy, c_trans);
+}
+
 static int orth(mmsb_ctx* c, float* X, int64_t n, int32_t cols, int32_t ld, float* Q, float* tmp, double* R_out,
                 int status_slot) {
   hipStream_t s = c->s;
@@ -195,7 +204,7 @@ static int orth(mmsb_ctx* c, float* X, int64_t n, int32_t cols, int32_t ld, floa
   TRY((gemm_nn<float, float>(s, n, cols, cols, 1.0, X, ld, c->R1i, cols, 0.0, tmp, ld)));
   TRY((gemm_tn<float, float>(s, n, cols, cols, 1.0, tmp, ld, tmp, ld, 0.0, c->gram, c->work)));
   TRY(chol_inv(s, cols, c->gram, kCholShift, c->R2, c->R2i, c->dstat + status_slot + 1));
-  TRY((gemm_nn<float, float>(s, n, cols, cols, 1.0, tmp, ld, c->R2i, cols, 0.0, Q, ld)));
+  TRY(nn_wellcond(c, n, cols, cols, tmp, ld, c->R2i, cols, Q, ld, false));  // R2 = I + O(kappa eps)
   if (R_out) TRY((gemm_nn<double, double>(s, cols, cols, cols, 1.0, c->R2, cols, c->R1, cols, 0.0, R_out, cols)));
   return MMSB_OK;
 }
@@ -323,8 +332,8 @@ static int whiten_impl(mmsb_ctx* c) {
   TRY(orth(c, T[0], ncn, r, ld, T[0], T[3], c->Rq, 2));
   // SVD of R2^T: the row-major R2 buffer is R2^T in column-major order
   TRY(jacobi_svd(s, r, c->Rq, c->Ut, c->sig, c->Vt, c->jwork, c->dstat + 6));
-  TRY((gemm_nn<float, float>(s, nb, r, k, 1.0, T[2], ld, c->Ut, r, 0.0, c->U, kp, true)));   // U = Q Ut[:, :k]
-  TRY((gemm_nn<float, float>(s, ncn, r, k, 1.0, T[0], ld, c->Vt, r, 0.0, c->V, kp, true)));  // V = Q2 Vt[:, :k]
+  TRY(nn_wellcond(c, nb, r, k, T[2], ld, c->Ut, r, c->U, kp, true));   // U = Q Ut[:, :k] (Ut orthogonal)
+  TRY(nn_wellcond(c, ncn, r, k, T[0], ld, c->Vt, r, c->V, kp, true));  // V = Q2 Vt[:, :k]
   TRY(col_sign<float>(s, nb, k, c->U, kp, c->best, c->sgn));
   TRY(scale_cols<float>(s, nb, k, c->U, kp, c->sgn));
   TRY(scale_cols<float>(s, ncn, k, c->V, kp, c->sgn));
@@ -357,7 +366,7 @@ static int whiten_impl(mmsb_ctx* c) {
   HCHK(hipMemcpyAsync(&hshift, c->shift, sizeof(double), hipMemcpyDeviceToHost, s));
   HCHK(hipStreamSynchronize(s));
   TRY(pinv_scale(s, r, c->lamv, hshift, cf.pinv_tol, 1, c->lamis, c->lamv));  // lamv <- sigma - shift
-  TRY((gemm_nn<float, float>(s, na, r, k, 1.0, T[qbuf], ld, c->E, r, 0.0, c->W, kp, true)));
+  TRY(nn_wellcond(c, na, r, k, T[qbuf], ld, c->E, r, c->W, kp, true));  // E orthogonal
   TRY(scale_cols<float>(s, na, k, c->W, kp, c->lamis));
   TRY(col_sign<float>(s, na, k, c->W, kp, c->best, c->sgn));
   TRY(scale_cols<float>(s, na, k, c->W, kp, c->sgn));
