@@ -16,9 +16,14 @@ template <typename TA, typename TB>
 int gemm_tn(hipStream_t s, int64_t n, int32_t p, int32_t q, double alpha, const TA* A, int32_t lda, const TB* B,
             int32_t ldb, double beta, double* C, double* work);
 // Y[n x q] = alpha * X C + beta * Y, C FP64 [p x q] (row-major, or column-major when c_trans).
-// Y = X C in FP32 on MFMA (C FP64 rounded to FP32, FP32 sums): for well-conditioned factors only.
-int gemm_nn_f32(hipStream_t s, int64_t n, int32_t p, int32_t q, const float* X, int32_t ldx, const double* C, int32_t ldc,
-                float* Y, int32_t ldy, bool c_trans = false);
+// Y = alpha X C + beta Y in FP32 on MFMA (C FP64 rounded to FP32, FP32 sums): for well-conditioned
+// factors only.
+int gemm_nn_f32(hipStream_t s, int64_t n, int32_t p, int32_t q, double alpha, const float* X, int32_t ldx,
+                const double* C, int32_t ldc, double beta, float* Y, int32_t ldy, bool c_trans = false);
+// C (FP64) = alpha A^T B + beta C with FP32 MFMA sums over row slices reduced in FP64; work as for
+// gemm_tn (tn_workspace_bytes).
+int gemm_tn_f32(hipStream_t s, int64_t n, int32_t p, int32_t q, double alpha, const float* A, int32_t lda, const float* B,
+                int32_t ldb, double beta, double* C, double* work);
 template <typename TX, typename TY>
 int gemm_nn(hipStream_t s, int64_t n, int32_t p, int32_t q, double alpha, const TX* X, int32_t ldx, const double* C,
             int32_t ldc, double beta, TY* Y, int32_t ldy, bool c_trans = false);

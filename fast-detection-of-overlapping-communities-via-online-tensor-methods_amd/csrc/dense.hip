@@ -478,8 +478,9 @@ template int gemm_nn<double, float>(hipStream_t, int64_t, int32_t, int32_t, doub
 constexpr int kFT = 128, kFK = 16, kFPad = kFT + 4;
 
 template <bool CT>
-__global__ __launch_bounds__(256, 2) void nn_f32mfma_kernel(int64_t n, int32_t p, int32_t q, const float* __restrict__ X,
-                                                           int32_t ldx, const double* __restrict__ C, int32_t ldc,
+__global__ __launch_bounds__(256, 2) void nn_f32mfma_kernel(int64_t n, int32_t p, int32_t q, float alpha,
+                                                           const float* __restrict__ X, int32_t ldx,
+                                                           const double* __restrict__ C, int32_t ldc, float beta,
                                                            float* __restrict__ Y, int32_t ldy) {
   __shared__ float Xs[kFK][kFPad];  // Xs[k][row]
   __shared__ float Cs[kFK][kFPad];  // Cs[k][col]
@@ -548,19 +549,109 @@ __global__ __launch_bounds__(256, 2) void nn_f32mfma_kernel(int64_t n, int32_t p
       for (int i = 0; i < 16; ++i) {
         const int64_t row = r0 + ai + 32 * u + 8 * (i >> 2) + 4 * (l >> 5) + (i & 3);
         const int col = c0 + bj + 32 * v + (l & 31);
-        if (row < n && col < q) Y[row * ldy + col] = acc[u][v][i];
+        if (row < n && col < q) {
+          float* y = Y + row * ldy + col;
+          *y = beta == 0.f ? alpha * acc[u][v][i] : alpha * acc[u][v][i] + beta * *y;
+        }
       }
 }
 
-int gemm_nn_f32(hipStream_t s, int64_t n, int32_t p, int32_t q, const float* X, int32_t ldx, const double* C, int32_t ldc,
-                float* Y, int32_t ldy, bool c_trans) {
+int gemm_nn_f32(hipStream_t s, int64_t n, int32_t p, int32_t q, double alpha, const float* X, int32_t ldx,
+                const double* C, int32_t ldc, double beta, float* Y, int32_t ldy, bool c_trans) {
   MMSB_CHECK_ARG(n >= 0 && p >= 1 && q >= 1 && ldx >= p && ldy >= q && ldc >= (c_trans ? p : q), "gemm_nn_f32: bad shape");
   if (n == 0) return MMSB_OK;
   const dim3 g((unsigned)ceil_div(n, kFT), (unsigned)ceil_div(q, kFT));
+  const float a = (float)alpha, b = (float)beta;
   if (c_trans)
-    hipLaunchKernelGGL(nn_f32mfma_kernel<true>, g, dim3(256), 0, s, n, p, q, X, ldx, C, ldc, Y, ldy);
+    hipLaunchKernelGGL(nn_f32mfma_kernel<true>, g, dim3(256), 0, s, n, p, q, a, X, ldx, C, ldc, b, Y, ldy);
   else
-    hipLaunchKernelGGL(nn_f32mfma_kernel<false>, g, dim3(256), 0, s, n, p, q, X, ldx, C, ldc, Y, ldy);
+    hipLaunchKernelGGL(nn_f32mfma_kernel<false>, g, dim3(256), 0, s, n, p, q, a, X, ldx, C, ldc, b, Y, ldy);
+  MMSB_LAUNCH_CHECK();
+  return MMSB_OK;
+}
+
+// C [p x q] (FP64) = alpha A^T B + beta C with A [n x p], B [n x q] FP32: the rows are cut into the
+// slices of tn_mfma_slices; each workgroup sums its slice of one 128 x 128 tile on
+// v_mfma_f32_32x32x2f32 (FP32 sums over at most n / slices rows), the partials are written as FP64
+// and reduced in FP64 by tn_mfma_reduce_kernel. For products whose relative accuracy need not beat
+// ~sqrt(n / slices) eps_32 (the operator applications of the randomized range finder).
+__global__ __launch_bounds__(256, 2) void tn_f32mfma_kernel(int64_t n, int32_t p, int32_t q, const float* __restrict__ A,
+                                                           int32_t lda, const float* __restrict__ B, int32_t ldb,
+                                                           int32_t slices, double* __restrict__ work) {
+  __shared__ float As[kFK][kFPad];  // As[row][i]
+  __shared__ float Bs[kFK][kFPad];  // Bs[row][j]
+  const int tq = (int)((q + kFT - 1) / kFT);
+  const int ti = blockIdx.x / tq, tj = blockIdx.x % tq;
+  const int slice = blockIdx.y;
+  const int64_t r0 = n * slice / slices, r1 = n * (slice + 1) / slices;
+  const int i0 = ti * kFT, j0 = tj * kFT;
+  const int t = threadIdx.x, w = t >> 6, l = t & 63;
+  const int ai = 64 * (w >> 1), bj = 64 * (w & 1);
+  const int sr = t >> 4, sc = t & 15;  // stage map: row sr, columns sc + 16 e
+  float ra[8], rb[8];
+  auto load = [&](int64_t base) {
+    const int64_t row = base + sr;
+    const bool rv = row < r1;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      const int ci = i0 + sc + 16 * e, cj = j0 + sc + 16 * e;
+      ra[e] = (rv && ci < p) ? A[row * lda + ci] : 0.f;
+      rb[e] = (rv && cj < q) ? B[row * ldb + cj] : 0.f;
+    }
+  };
+  floatx16 acc[2][2];
+#pragma unroll
+  for (int u = 0; u < 2; ++u)
+#pragma unroll
+    for (int v = 0; v < 2; ++v) acc[u][v] = floatx16{};
+  load(r0);
+  for (int64_t base = r0; base < r1; base += kFK) {
+    __syncthreads();
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      As[sr][sc + 16 * e] = ra[e];
+      Bs[sr][sc + 16 * e] = rb[e];
+    }
+    __syncthreads();
+    if (base + kFK < r1) load(base + kFK);
+#pragma unroll
+    for (int kk = 0; kk < kFK; kk += 2) {
+      const int kl = kk + (l >> 5);
+      float a[2], b[2];
+#pragma unroll
+      for (int u = 0; u < 2; ++u) {
+        a[u] = As[kl][ai + 32 * u + (l & 31)];
+        b[u] = Bs[kl][bj + 32 * u + (l & 31)];
+      }
+#pragma unroll
+      for (int u = 0; u < 2; ++u)
+#pragma unroll
+        for (int v = 0; v < 2; ++v) acc[u][v] = __builtin_amdgcn_mfma_f32_32x32x2f32(a[u], b[v], acc[u][v], 0, 0, 0);
+    }
+  }
+  double* wp = work + (int64_t)slice * p * q;
+#pragma unroll
+  for (int u = 0; u < 2; ++u)
+#pragma unroll
+    for (int v = 0; v < 2; ++v)
+#pragma unroll
+      for (int i = 0; i < 16; ++i) {
+        const int ri = i0 + ai + 32 * u + 8 * (i >> 2) + 4 * (l >> 5) + (i & 3);
+        const int cj = j0 + bj + 32 * v + (l & 31);
+        if (ri < p && cj < q) wp[(int64_t)ri * q + cj] = (double)acc[u][v][i];
+      }
+}
+
+int gemm_tn_f32(hipStream_t s, int64_t n, int32_t p, int32_t q, double alpha, const float* A, int32_t lda, const float* B,
+                int32_t ldb, double beta, double* C, double* work) {
+  MMSB_CHECK_ARG(n >= 0 && p >= 1 && q >= 1 && lda >= p && ldb >= q, "gemm_tn_f32: bad shape");
+  const int slices = tn_mfma_slices(n, p, q, false);
+  const dim3 grid((unsigned)(ceil_div(p, kFT) * ceil_div(q, kFT)), (unsigned)slices);
+  hipLaunchKernelGGL(tn_f32mfma_kernel, grid, dim3(256), 0, s, n, p, q, A, lda, B, ldb, slices, work);
+  MMSB_LAUNCH_CHECK();
+  const int64_t pq = (int64_t)p * q;
+  hipLaunchKernelGGL(tn_mfma_reduce_kernel, dim3((unsigned)ceil_div(pq, 256)), dim3(256), 0, s, p, q, slices, false, work,
+                     alpha, beta, C);
   MMSB_LAUNCH_CHECK();
   return MMSB_OK;
 }

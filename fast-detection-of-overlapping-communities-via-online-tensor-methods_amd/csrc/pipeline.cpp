@@ -192,7 +192,7 @@ static int spmm(mmsb_ctx* c, const DevCsr& g, float alpha, const float* B, int32
 // product otherwise.
 static int nn_wellcond(mmsb_ctx* c, int64_t n, int32_t p, int32_t q, const float* X, int32_t ldx, const double* C,
                        int32_t ldc, float* Y, int32_t ldy, bool c_trans) {
-  if (n >= 4096) return gemm_nn_f32(c->s, n, p, q, X, ldx, C, ldc, Y, ldy, c_trans);
+  if (n >= 4096) return gemm_nn_f32(c->s, n, p, q, 1.0, X, ldx, C, ldc, 0.0, Y, ldy, c_trans);
   return gemm_nn<float, float>(c->s, n, p, q, 1.0, X, ldx, C, ldc, 0.0, Y, ldy, c_trans);
 }
 
@@ -228,13 +228,27 @@ static int m2_apply(mmsb_ctx* c, const float* X, float* OUT) {
   const double a0 = c->cfg.alpha_0;
   const double s_main = paper ? a0 + 1.0 : 1.0;
   const double s_shift = paper ? a0 : a0 / (a0 + 1.0);
-  TRY((gemm_tn<float, float>(s, na, k, r, 1.0, c->LB, kp, X, ld, 0.0, c->P1, c->work)));
-  TRY((gemm_tn<float, float>(s, na, k, r, 1.0, c->LC, kp, X, ld, 0.0, c->P2, c->work)));
+  // The range finder needs the operator's action to ~1e-6, not to FP64: on tall inputs its two
+  // products L^T X and the two L P products run on FP32 MFMA (the r x r Rayleigh-Ritz matrix that
+  // fixes the eigenvalues is formed in FP64 by the caller).
+  const bool f32 = na >= 4096;
+  if (f32) {
+    TRY(gemm_tn_f32(s, na, k, r, 1.0, c->LB, kp, X, ld, 0.0, c->P1, c->work));
+    TRY(gemm_tn_f32(s, na, k, r, 1.0, c->LC, kp, X, ld, 0.0, c->P2, c->work));
+  } else {
+    TRY((gemm_tn<float, float>(s, na, k, r, 1.0, c->LB, kp, X, ld, 0.0, c->P1, c->work)));
+    TRY((gemm_tn<float, float>(s, na, k, r, 1.0, c->LC, kp, X, ld, 0.0, c->P2, c->work)));
+  }
   TRY((gemm_tn<double, float>(s, na, 1, r, 1.0, c->mu[0], 1, X, ld, 0.0, c->mrow, c->work)));
   TRY(scale_rows(s, k, r, c->P1, r, c->sig, 0.5 * s_main));
   TRY(scale_rows(s, k, r, c->P2, r, c->sig, 0.5 * s_main));
-  TRY((gemm_nn<float, float>(s, na, k, r, 1.0, c->LC, kp, c->P1, r, 0.0, OUT, ld)));
-  TRY((gemm_nn<float, float>(s, na, k, r, 1.0, c->LB, kp, c->P2, r, 1.0, OUT, ld)));
+  if (f32) {
+    TRY(gemm_nn_f32(s, na, k, r, 1.0, c->LC, kp, c->P1, r, 0.0, OUT, ld));
+    TRY(gemm_nn_f32(s, na, k, r, 1.0, c->LB, kp, c->P2, r, 1.0, OUT, ld));
+  } else {
+    TRY((gemm_nn<float, float>(s, na, k, r, 1.0, c->LC, kp, c->P1, r, 0.0, OUT, ld)));
+    TRY((gemm_nn<float, float>(s, na, k, r, 1.0, c->LB, kp, c->P2, r, 1.0, OUT, ld)));
+  }
   TRY((gemm_nn<double, float>(s, na, 1, r, -s_shift, c->mu[0], 1, c->mrow, r, 1.0, OUT, ld)));
   if (!paper) TRY(diag_axpy(s, na, r, c->mu[0], s_shift, X, ld, OUT, ld));
   return MMSB_OK;
