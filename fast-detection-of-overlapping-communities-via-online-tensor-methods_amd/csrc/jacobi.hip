@@ -157,15 +157,15 @@ __global__ __launch_bounds__(kJacobiThreads) void jacobi_svd_kernel(int32_t r, d
 // 256 / B threads per pair), and writes them back. A sweep needs nblk - 1 grid barriers instead
 // of r - 1. Convergence test, ordering of the output and exit protocol as above.
 template <int B, bool WV>
-__global__ __launch_bounds__(kJacobiThreads) void jacobi_block_kernel(int32_t r, double* __restrict__ A,
+__global__ __launch_bounds__(64 * B) void jacobi_block_kernel(int32_t r, double* __restrict__ A,
                                                                        double* __restrict__ U, double* __restrict__ sigma,
                                                                        double* __restrict__ V, double* __restrict__ Vw,
                                                                        double* __restrict__ sw, int32_t* __restrict__ counters,
                                                                        int32_t* __restrict__ sweeps_out) {
-  constexpr int TPG = kJacobiThreads / B;  // threads per pair (<= 64: one wave or half)
+  constexpr int TPG = 64;                  // one wave per column pair (64 B threads per workgroup)
   extern __shared__ double lds[];          // [2B][r] columns of A, then (WV) [2B][r] columns of V
   cg::grid_group grid = cg::this_grid();
-  __shared__ double red[kJacobiThreads / 64];
+  __shared__ double red[B];
   __shared__ int go_on, nrot;
   double* la = lds;
   double* lv = lds + (int64_t)2 * B * r;
@@ -302,14 +302,14 @@ static int launch_block(hipStream_t s, int32_t r, double* A, double* U, double* 
   MMSB_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
   int dev = 0, per_cu = 0, cus = 0;
   MMSB_HIP(hipGetDevice(&dev));
-  MMSB_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, kJacobiThreads, shm));
+  MMSB_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, 64 * B, shm));
   MMSB_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
   if (per_cu < 1) return set_error(MMSB_ERR_HIP, "jacobi: block kernel does not fit (%zu bytes of LDS)", shm);
   const int nblk = (r + B - 1) / B;
   const int npairs = (nblk + (nblk & 1)) / 2;
   const int nwg = std::max(1, std::min(npairs, per_cu * cus));
   void* args[] = {&r, &A, &U, &sigma, &V, &Vw, &sw, &counters, &sweeps_out};
-  MMSB_HIP(hipLaunchCooperativeKernel(fn, dim3(nwg), dim3(kJacobiThreads), args, shm, s));
+  MMSB_HIP(hipLaunchCooperativeKernel(fn, dim3(nwg), dim3(64 * B), args, shm, s));
   return MMSB_OK;
 }
 
