@@ -68,6 +68,7 @@ def main():
     ap.add_argument("--precision", default="bf16", choices=["fp32", "bf16"])
     ap.add_argument("--no-oracle", action="store_true", help="skip the FP64 oracle epochs (accuracy, cpu_baseline)")
     ap.add_argument("--accuracy-epochs", type=int, default=2)
+    ap.add_argument("--minibatch", type=int, default=None, help="STGD minibatch (default: the workload's)")
     ap.add_argument("--cpu-e2e-sample", type=int, default=50000,
                     help="nodes per partition of the sample on which the oracle pipeline is timed for the "
                          "end-to-end CPU estimate (0: skip)")
@@ -91,7 +92,9 @@ def main():
     if args.workload == "c4":
         return bench_batch(m, args)
 
-    w = WORKLOADS[args.workload]
+    w = dict(WORKLOADS[args.workload])
+    if args.minibatch:
+        w["b"] = args.minibatch
     n, k, b = w["n"], w["k"], w["b"]
     if "mean_degree" in w:
         p_in, p_out = c3_probs(n, k, w["mean_degree"])
