@@ -17,9 +17,10 @@
 // Work decomposition: every GEMM output is cut into 32 x 32 blocks, one per workgroup of four
 // waves. The waves split the reduction dimension and load their MFMA fragments straight from
 // L2/HBM into registers (each wave owns its K range, so nothing is shared through LDS) through a
-// ring of D fragment sets, so that up to D groups of loads are in flight ahead of the MFMAs. The
-// four partial accumulators meet in a padded LDS tile that all 256 threads read back for
-// coalesced epilogue stores. At k = 500 a phase launches 256-768 workgroups.
+// ring of D fragment sets, so that up to D groups of loads are in flight ahead of the MFMAs (in
+// P1's data tiles each wave computes both Y_A Phi and Y_B Phi over its K range, loading each Phi^T
+// fragment once for the two). The partial accumulators meet in padded LDS tiles that all 256
+// threads read back for coalesced epilogue stores. At k = 500 a phase launches 256-768 workgroups.
 // K order inside a fragment: for v_mfma_f32_32x32x2f32 lane half h holds k = 4h + i of a chunk
 // of eight k for instruction i (the order of a sum over k is free as long as A and B agree);
 // for v_mfma_f32_32x32x16_bf16 lane half h holds the eight contiguous k = 8h .. 8h + 7.
