@@ -76,11 +76,13 @@ class StpmConfigC(ctypes.Structure):
                 ("polish_epochs", ctypes.c_int32), ("oversample", ctypes.c_int32), ("power_iters", ctypes.c_int32),
                 ("m2_scaling", ctypes.c_int32), ("member_tau", ctypes.c_double), ("seed", ctypes.c_uint64),
                 ("ba_first_is_row", ctypes.c_int32), ("feof_repeat", ctypes.c_int32), ("device", ctypes.c_int32),
-                ("verbose", ctypes.c_int32), ("use_graphs", ctypes.c_int32)]
+                ("verbose", ctypes.c_int32), ("use_graphs", ctypes.c_int32),
+                ("y_mode", ctypes.c_int32)]
 
 
 PREC = {"fp32": 0, "bf16": 1}
 M2_SCALING = {"paper": 0, "gpu": 1}
+Y_MODE = {"auto": 0, "stored": 1, "recompute": 2}
 BLOCKS = {"XA": 0, "XB": 1, "XC": 2, "BA": 3, "CA": 4}
 QUANTITIES = ["mu_a", "mu_b", "mu_c", "W", "U", "V", "sigma", "LB", "LC", "lam_m2", "YA", "YB", "YC",
               "muw_a", "muw_b", "muw_c", "phi", "frob", "lambda", "lambda_p", "alpha_hat",
@@ -247,6 +249,8 @@ def config(**kw) -> StpmConfigC:
             val = PREC[val]
         if key == "m2_scaling" and isinstance(val, str):
             val = M2_SCALING[val]
+        if key == "y_mode" and isinstance(val, str):
+            val = Y_MODE[val]
         if isinstance(val, str):
             val = val.encode()
         if not hasattr(c, key):

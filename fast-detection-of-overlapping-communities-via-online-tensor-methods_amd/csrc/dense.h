@@ -62,6 +62,25 @@ int copy2d_f32(hipStream_t s, int32_t rows, int32_t cols, const float* in, int32
 int spmm_f32(hipStream_t s, int32_t m, int32_t n, int64_t nnz, const int32_t* rowptr, const int32_t* colidx, const float* vals,
              float alpha, const float* B, int32_t ldb, int32_t r, float* Y, int32_t ldy, const int32_t* order = nullptr);
 
+// Whitened rows of one STGD minibatch: for each of the three jobs (y_A, y_B, y_C), rows [0, nb) of
+// the CSR block whose rowptr is given (offset to the minibatch's first row) times R [n_cols x ldr]
+// (r columns), written row-major ([nb x ldo]) or transposed ([r x ldo], trans != 0) as FP32 or BF16.
+// Each row is summed in the order spmm_f32 uses and rounded like f32_to_bf16.
+struct YRowsJob {
+  const int32_t* rowptr;
+  const int32_t* colidx;
+  const float* vals;  // null: binary block
+  const float* R;
+  void* out;
+  int64_t ldo;
+  int32_t trans;
+};
+struct YRowsJobs {
+  YRowsJob j[3];
+  int32_t count;  // jobs used (1..3)
+};
+int spmm_yrows(hipStream_t s, const YRowsJobs& jobs, int32_t nb, int32_t r, int32_t ldr, bool bf16);
+
 size_t grad_workspace_bytes(int32_t k, int32_t nb);
 template <typename TY>
 int stgd_grad(hipStream_t s, int32_t k, int32_t nb, const TY* ya, const TY* yb, const TY* yc, int32_t ldy,

@@ -95,7 +95,12 @@ struct mmsb_ctx {
   // neighbour vectors
   float *YA = nullptr, *YB = nullptr, *YCt = nullptr;
   double* muw[3] = {nullptr, nullptr, nullptr};
-  bool have_y = false;
+  bool have_y = false;       // Y_A, Y_B, Y_C^T resident for all of X
+  bool y_window = false;     // checkpointed STGD: each step recomputes its minibatch rows from CSR into yw
+  // [b x kp], [b x kp], [kp x b] in FP32 and in BF16: one set per precision, so that the padding
+  // columns (k..kp) and rows of each stay zero in its own format
+  float* yw[3] = {nullptr, nullptr, nullptr};
+  bf16_bits* yw16[3] = {nullptr, nullptr, nullptr};
   // STGD
   float* phi[2] = {nullptr, nullptr};
   int cur = 0;
@@ -143,6 +148,28 @@ static int dalloc(mmsb_ctx* c, T*& p, size_t count) {
   HCHK(hipMemsetAsync(q, 0, bytes, c->s));
   c->allocs.push_back(q);
   p = static_cast<T*>(q);
+  return MMSB_OK;
+}
+
+template <typename T>
+static void dfree(mmsb_ctx* c, T*& p) {
+  if (!p) return;
+  (void)hipStreamSynchronize(c->s);
+  auto it = std::find(c->allocs.begin(), c->allocs.end(), (void*)p);
+  if (it != c->allocs.end()) c->allocs.erase(it);
+  (void)hipFree(p);
+  p = nullptr;
+}
+
+// Captured epoch graphs hold the step parameters and buffer addresses in their kernel arguments.
+static int drop_epoch_graphs(mmsb_ctx* c) {
+  HCHK(hipStreamSynchronize(c->s));
+  for (auto& row : c->epoch_graph)
+    for (hipGraphExec_t& g : row)
+      if (g) {
+        HCHK(hipGraphExecDestroy(g));
+        g = nullptr;
+      }
   return MMSB_OK;
 }
 
@@ -303,6 +330,52 @@ static int alloc_whitening(mmsb_ctx* c) {
   return MMSB_OK;
 }
 
+// Bytes the resident form would add from whitening on: sketches and factors, Y_A/Y_B/Y_C^T with their
+// BF16 copies, and the post_proc buffers.
+static double resident_bytes(const mmsb_ctx* c) {
+  const double kp4 = 4.0 * c->kp;
+  const double whiten = 4.0 * (double)c->nmax * c->rld * 4.0 + (3.0 * c->n[1] + 2.0 * c->n[2] + 2.0 * c->n[3]) * kp4;
+  const double y = 3.0 * (double)c->n_pad * c->kp * 6.0;
+  const double post = (double)(c->n[1] + c->n[2]) * kp4 + 2.0 * (double)(c->n[0] + c->n[1] + c->n[2] + c->n[3]) * kp4;
+  return whiten + y + post;
+}
+
+// MMSB_Y_AUTO: recompute the minibatches when the resident form would not fit in 90% of the free
+// device memory.
+static int choose_y_window(mmsb_ctx* c, bool& window) {
+  const int32_t mode = c->cfg.y_mode;
+  if (mode != MMSB_Y_AUTO) {
+    window = mode == MMSB_Y_RECOMPUTE;
+    return MMSB_OK;
+  }
+  size_t free_b = 0, total_b = 0;
+  HCHK(hipMemGetInfo(&free_b, &total_b));
+  const double need = resident_bytes(c);
+  window = need > 0.9 * (double)free_b;
+  c->log("whiten: resident form needs %.1f GB of %.1f GB free: %s\n", need * 1e-9, (double)free_b * 1e-9,
+         window ? "minibatches recomputed from CSR" : "neighbour vectors kept");
+  return MMSB_OK;
+}
+
+static int alloc_window(mmsb_ctx* c) {
+  for (float*& p : c->yw) TRY(dalloc(c, p, (size_t)c->b * c->kp));
+  for (bf16_bits*& p : c->yw16) TRY(dalloc(c, p, (size_t)c->b * c->kp));
+  return MMSB_OK;
+}
+
+// Enqueues the whitened rows [row0, row0 + nb) of X into the window buffers (FP32 or BF16).
+static int fill_window(mmsb_ctx* c, bool bf16, int64_t row0, int32_t nb) {
+  YRowsJobs jobs{};
+  const float* R[3] = {c->W, c->MB, c->MC};
+  for (int i = 0; i < 3; ++i) {
+    const DevCsr& g = c->blk[i];
+    void* out = bf16 ? (void*)c->yw16[i] : (void*)c->yw[i];
+    jobs.j[i] = YRowsJob{g.rowptr + row0, g.colidx, g.vals, R[i], out, i == 2 ? (int64_t)c->b : (int64_t)c->kp, i == 2};
+  }
+  jobs.count = 3;
+  return spmm_yrows(c->s, jobs, nb, c->k, c->kp, bf16);
+}
+
 static int alloc_y(mmsb_ctx* c) {
   TRY(dalloc(c, c->YA, (size_t)c->n_pad * c->kp));
   TRY(dalloc(c, c->YB, (size_t)c->n_pad * c->kp));
@@ -320,8 +393,17 @@ static int whiten_impl(mmsb_ctx* c) {
   const DevCsr &XA = c->blk[0], &XB = c->blk[1], &XC = c->blk[2];
   const DevCsr &XAt = c->blkT[0], &XBt = c->blkT[1], &XCt = c->blkT[2];
   float **T = c->T;
+  bool window = false;
+  TRY(choose_y_window(c, window));
+  if (window != c->y_window) TRY(drop_epoch_graphs(c));
+  c->y_window = window;
   TRY(alloc_whitening(c));
-  TRY(alloc_y(c));
+  if (window) {
+    for (int i = 0; i < 3; ++i) TRY(dalloc(c, c->muw[i], (size_t)c->kp));
+    TRY(alloc_window(c));
+  } else {
+    TRY(alloc_y(c));
+  }
   // means (pre_proc.c:154-162)
   TRY(csr_rowsum(s, XAt.n_rows, XAt.rowptr, XAt.vals, 1.0 / (double)nx, c->mu[0]));
   TRY(csr_rowsum(s, XBt.n_rows, XBt.rowptr, XBt.vals, 1.0 / (double)nx, c->mu[1]));
@@ -385,22 +467,29 @@ static int whiten_impl(mmsb_ctx* c) {
   TRY(col_sign<float>(s, na, k, c->W, kp, c->best, c->sgn));
   TRY(scale_cols<float>(s, na, k, c->W, kp, c->sgn));
   // neighbour vectors (pre_proc.c:339-366)
-  c->log("whiten: neighbour vectors\n");
-  TRY(spmm(c, XA, 1.f, c->W, kp, k, c->YA, kp));
+  // y rows are G_X* rows times W, Z_B^T W = U (L_B^T W) and Z_C^T W = V (L_C^T W)
+  c->log(window ? "whiten: neighbour-vector factors (rows recomputed per STGD minibatch)\n" : "whiten: neighbour vectors\n");
   TRY((gemm_tn<float, float>(s, na, k, k, 1.0, c->LB, kp, c->W, kp, 0.0, c->LBW, c->work)));
   TRY((gemm_nn<float, float>(s, nb, k, k, 1.0, c->U, kp, c->LBW, k, 0.0, c->MB, kp)));
-  TRY(spmm(c, XB, 1.f, c->MB, kp, k, c->YB, kp));
   TRY((gemm_tn<float, float>(s, na, k, k, 1.0, c->LC, kp, c->W, kp, 0.0, c->LCW, c->work)));
   TRY((gemm_nn<float, float>(s, ncn, k, k, 1.0, c->V, kp, c->LCW, k, 0.0, c->MC, kp)));
-  TRY(spmm(c, XC, 1.f, c->MC, kp, k, T[1], ld));
-  TRY(transpose_f32(s, nx, k, T[1], ld, c->YCt, c->n_pad));
+  if (!window) {
+    TRY(spmm(c, XA, 1.f, c->W, kp, k, c->YA, kp));
+    TRY(spmm(c, XB, 1.f, c->MB, kp, k, c->YB, kp));
+    TRY(spmm(c, XC, 1.f, c->MC, kp, k, T[1], ld));
+    TRY(transpose_f32(s, nx, k, T[1], ld, c->YCt, c->n_pad));
+  }
   TRY((gemm_tn<float, double>(s, na, k, 1, 1.0, c->W, kp, c->mu[0], 1, 0.0, c->muw[0], c->work)));
   TRY((gemm_tn<float, double>(s, nb, k, 1, 1.0, c->MB, kp, c->mu[1], 1, 0.0, c->muw[1], c->work)));
   TRY((gemm_tn<float, double>(s, ncn, k, 1, 1.0, c->MC, kp, c->mu[2], 1, 0.0, c->muw[2], c->work)));
   TRY(check_status(c, "whitening"));
   c->whitened = true;
-  c->have_y = true;
+  c->have_y = !window;
   c->y16_valid = false;
+  if (window) {  // what STGD and post_proc do not read goes back to the device: the sketches and V
+    for (float*& p : c->T) dfree(c, p);
+    dfree(c, c->V);
+  }
   return MMSB_OK;
 }
 
@@ -438,14 +527,16 @@ static int init_phi(mmsb_ctx* c) {
 
 static int prepare_bf16(mmsb_ctx* c) {
   const size_t ny = (size_t)c->n_pad * c->kp;
-  TRY(dalloc(c, c->YA16, ny));
-  TRY(dalloc(c, c->YB16, ny));
-  TRY(dalloc(c, c->YCt16, ny));
+  if (c->have_y) {
+    TRY(dalloc(c, c->YA16, ny));
+    TRY(dalloc(c, c->YB16, ny));
+    TRY(dalloc(c, c->YCt16, ny));
+  }
   const size_t kk2 = (size_t)2 * c->kp * c->kp;
   for (bf16_bits** p : {&c->phi16[0], &c->phi16[1], &c->phit16, &c->phia16, &c->phiT16, &c->phitT16, &c->phiaT16, &c->gt16,
                         &c->h16, &c->nsq16})
     TRY(dalloc(c, *p, kk2));
-  if (!c->y16_valid) {
+  if (c->have_y && !c->y16_valid) {
     TRY(f32_to_bf16(c->s, (int64_t)ny, c->YA, c->YA16));
     TRY(f32_to_bf16(c->s, (int64_t)ny, c->YB, c->YB16));
     TRY(f32_to_bf16(c->s, (int64_t)ny, c->YCt, c->YCt16));
@@ -537,14 +628,24 @@ static int run_stgd_steps(mmsb_ctx* c, int64_t nsteps, int32_t precision, StgdTi
     return set_error(MMSB_ERR_ARG, "stgd: unknown precision %d", precision);
   if (precision == MMSB_PREC_BF16) TRY(prepare_bf16(c));
   TRY(ensure_frob(c, nsteps));
-  const StgdArgs a = stgd_args(c);
+  StgdArgs a = stgd_args(c);
   const StgdBuffers bufs = stgd_bufs(c);
   const void *ya, *yb, *yc;
   y_ptrs(c, precision, ya, yb, yc);
+  StgdFill fill;
+  if (!c->have_y) {  // checkpointed: the minibatch's rows are rebuilt from CSR before each step
+    a.y_window = 1;
+    a.n_pad = c->b;
+    const bool bf = precision == MMSB_PREC_BF16;
+    ya = bf ? (const void*)c->yw16[0] : (const void*)c->yw[0];
+    yb = bf ? (const void*)c->yw16[1] : (const void*)c->yw[1];
+    yc = bf ? (const void*)c->yw16[2] : (const void*)c->yw[2];
+    fill = [c, bf](int64_t row0, int32_t nb) { return fill_window(c, bf, row0, nb); };
+  }
   const int64_t spe = (c->n[0] + c->bl - 1) / c->bl;
   auto launch = [&](int64_t step0, int64_t ns, double* frob_out, int& cur, StgdTiming* tm) -> int {
     return stgd_run(c->s, precision, a, ya, yb, yc, step0, ns, c->cfg.renorm_every, c->cfg.ns_iters, c->cfg.ns_degree, bufs,
-                    cur, frob_out, tm);
+                    cur, frob_out, tm, fill);
   };
   // Whole epochs that start at an epoch boundary replay a captured HIP graph of the epoch's
   // launches (one graph per precision and Phi ping-pong parity); anything else launches directly.
@@ -656,7 +757,10 @@ extern "C" int mmsb_batch_create(mmsb_ctx* const* ctxs, int32_t count, int32_t p
   for (int32_t i = 0; i < count; ++i) {
     const mmsb_ctx* c = ctxs[i];
     MMSB_CHECK_ARG(c, "batch_create: null context %d", i);
-    if (!c->have_y) return set_error(MMSB_ERR_STATE, "batch_create: context %d has no neighbour vectors", i);
+    if (!c->have_y)
+      return set_error(MMSB_ERR_STATE, c->y_window ? "batch_create: context %d recomputes its minibatches (y_mode); "
+                                                     "batches need resident neighbour vectors"
+                                                   : "batch_create: context %d has no neighbour vectors", i);
     MMSB_CHECK_ARG(c->cfg.device == c0->cfg.device, "batch_create: contexts on different devices");
     MMSB_CHECK_ARG(c->cfg.ns_iters == c0->cfg.ns_iters && c->cfg.ns_degree == c0->cfg.ns_degree &&
                        c->cfg.renorm_every == c0->cfg.renorm_every,
@@ -889,7 +993,7 @@ static View view_of(mmsb_ctx* c, int what) {
       return c->mu[what - MMSB_Q_MU_A] ? vec(c->mu[what - MMSB_Q_MU_A], c->n[1 + what - MMSB_Q_MU_A]) : v;
     case MMSB_Q_W: return c->whitened ? tall(c->W, c->n[1]) : v;
     case MMSB_Q_U: return c->whitened ? tall(c->U, c->n[2]) : v;
-    case MMSB_Q_V: return c->whitened ? tall(c->V, c->n[3]) : v;
+    case MMSB_Q_V: return c->whitened && c->V ? tall(c->V, c->n[3]) : v;
     case MMSB_Q_SIGMA: return c->whitened ? vec(c->sig, k) : v;
     case MMSB_Q_LB: return c->whitened ? tall(c->LB, c->n[1]) : v;
     case MMSB_Q_LC: return c->whitened ? tall(c->LC, c->n[1]) : v;
@@ -901,7 +1005,7 @@ static View view_of(mmsb_ctx* c, int what) {
       v.p = c->YCt; v.rows = c->n[0]; v.cols = k; v.ld = c->n_pad; v.trans = true;
       return v;
     case MMSB_Q_MUW_A: case MMSB_Q_MUW_B: case MMSB_Q_MUW_C:
-      return c->have_y ? vec(c->muw[what - MMSB_Q_MUW_A], k) : v;
+      return c->have_y || (c->y_window && c->whitened) ? vec(c->muw[what - MMSB_Q_MUW_A], k) : v;
     case MMSB_Q_PHI:
       if (!c->have_phi) return v;
       v.p = c->phi[c->cur]; v.rows = k; v.cols = k; v.ld = kp;
@@ -1053,6 +1157,7 @@ extern "C" int mmsb_create(const stpm_config* cfg, mmsb_ctx** out) {
                  "create: k + oversample must not exceed the partition sizes");
   MMSB_CHECK_ARG(cfg->minibatch >= 1 && cfg->power_iters >= 0 && cfg->ns_iters >= 0, "create: bad STGD settings");
   MMSB_CHECK_ARG(cfg->ns_degree == 3 || cfg->ns_degree == 5, "create: ns_degree must be 3 or 5 (got %d)", cfg->ns_degree);
+  MMSB_CHECK_ARG(cfg->y_mode >= MMSB_Y_AUTO && cfg->y_mode <= MMSB_Y_RECOMPUTE, "create: unknown y_mode %d", cfg->y_mode);
   mmsb_ctx* c = new (std::nothrow) mmsb_ctx();
   if (!c) return set_error(MMSB_ERR_NOMEM, "create: out of host memory");
   c->cfg = *cfg;
@@ -1064,18 +1169,6 @@ extern "C" int mmsb_create(const stpm_config* cfg, mmsb_ctx** out) {
     return set_error(MMSB_ERR_HIP, "create: %s", hipGetErrorString(e));
   }
   *out = c;
-  return MMSB_OK;
-}
-
-// Captured epoch graphs hold the step parameters in their kernel arguments.
-static int drop_epoch_graphs(mmsb_ctx* c) {
-  HCHK(hipStreamSynchronize(c->s));
-  for (auto& row : c->epoch_graph)
-    for (hipGraphExec_t& g : row)
-      if (g) {
-        HCHK(hipGraphExecDestroy(g));
-        g = nullptr;
-      }
   return MMSB_OK;
 }
 
@@ -1191,6 +1284,8 @@ extern "C" int mmsb_set_yvectors(mmsb_ctx* c, int64_t n_x, int32_t k, const doub
   MMSB_CHECK_ARG(n_x == c->n[0] && k == c->k, "set_yvectors: shape %lld x %d, context has %lld x %d", (long long)n_x, k,
                  (long long)c->n[0], c->k);
   HCHK(hipSetDevice(c->cfg.device));
+  if (c->y_window) TRY(drop_epoch_graphs(c));
+  c->y_window = false;
   TRY(alloc_y(c));
   const int32_t kp = c->kp;
   std::vector<float> a((size_t)c->n_pad * kp, 0.f), b((size_t)c->n_pad * kp, 0.f), ct((size_t)kp * c->n_pad, 0.f);
@@ -1215,7 +1310,8 @@ extern "C" int mmsb_stgd_timed(mmsb_ctx* c, int64_t steps, int32_t precision, do
   clear_error();
   MMSB_CHECK_ARG(c && phase_ms, "stgd_timed: null argument");
   MMSB_CHECK_ARG(steps >= 0, "stgd_timed: negative step count");
-  if (!c->have_y) return set_error(MMSB_ERR_STATE, "stgd_timed: neighbour vectors not computed (call mmsb_whiten)");
+  if (!c->have_y && !(c->y_window && c->whitened))
+    return set_error(MMSB_ERR_STATE, "stgd_timed: neighbour vectors not computed (call mmsb_whiten)");
   HCHK(hipSetDevice(c->cfg.device));
   TRY(alloc_stgd(c));
   if (!c->have_phi) TRY(init_phi(c));
@@ -1246,7 +1342,8 @@ extern "C" int mmsb_stgd_timed(mmsb_ctx* c, int64_t steps, int32_t precision, do
 extern "C" int mmsb_stgd(mmsb_ctx* c, int64_t steps, int32_t precision) {
   clear_error();
   MMSB_CHECK_ARG(c, "stgd: null context");
-  if (!c->have_y) return set_error(MMSB_ERR_STATE, "stgd: neighbour vectors not computed (call mmsb_whiten)");
+  if (!c->have_y && !(c->y_window && c->whitened))
+    return set_error(MMSB_ERR_STATE, "stgd: neighbour vectors not computed (call mmsb_whiten)");
   HCHK(hipSetDevice(c->cfg.device));
   TRY(alloc_stgd(c));
   if (!c->have_phi) TRY(init_phi(c));
@@ -1281,6 +1378,7 @@ extern "C" int mmsb_grad(mmsb_ctx* c, int64_t row0, int32_t nrows, const double*
   MMSB_CHECK_ARG(c && phi && grad && loss, "grad: null argument");
   MMSB_CHECK_ARG(nrows >= 1 && row0 >= 0 && row0 + nrows <= c->n[0], "grad: rows [%lld, %lld) outside X", (long long)row0,
                  (long long)(row0 + nrows));
+  if (!recompute && !c->have_y && c->y_window) recompute = 1;  // nothing resident to read
   if (recompute && !c->whitened) return set_error(MMSB_ERR_STATE, "grad: recomputing needs the whitening factors");
   if (!recompute && !c->have_y) return set_error(MMSB_ERR_STATE, "grad: neighbour vectors not available");
   HCHK(hipSetDevice(c->cfg.device));
@@ -1348,7 +1446,9 @@ extern "C" int mmsb_grad(mmsb_ctx* c, int64_t row0, int32_t nrows, const double*
 extern "C" int mmsb_power_method(mmsb_ctx* c, int32_t iters, int32_t orthogonalize, double* lambda) {
   clear_error();
   MMSB_CHECK_ARG(c && lambda && iters >= 0, "power_method: bad arguments");
-  if (!c->have_y) return set_error(MMSB_ERR_STATE, "power_method: neighbour vectors not available");
+  if (!c->have_y)
+    return set_error(MMSB_ERR_STATE, c->y_window ? "power_method: needs resident neighbour vectors (y_mode recomputes them)"
+                                                 : "power_method: neighbour vectors not available");
   HCHK(hipSetDevice(c->cfg.device));
   TRY(alloc_stgd(c));
   if (!c->have_phi) TRY(init_phi(c));
@@ -1419,10 +1519,44 @@ extern "C" int mmsb_post(mmsb_ctx* c) {
   return MMSB_OK;
 }
 
+namespace mmsb {
+// Y_A / Y_B / Y_C rows of a context that recomputes its minibatches: rebuilt from CSR by the STGD
+// fill kernel (FP32) in chunks of rows, [nrows x k] row-major doubles.
+static bool y_recomputed(const mmsb_ctx* c, int32_t what) {
+  return what >= MMSB_Q_YA && what <= MMSB_Q_YC && !c->have_y && c->y_window && c->whitened;
+}
+
+static int yrows_host(mmsb_ctx* c, int32_t what, int64_t row0, int64_t nrows, double* host) {
+  const int i = what - MMSB_Q_YA;
+  const int32_t k = c->k, kp = c->kp;
+  const int64_t chunk = std::max<int64_t>(1, std::min<int64_t>(nrows, 65536));
+  float* d = nullptr;
+  HCHK(hipMalloc(&d, sizeof(float) * (size_t)chunk * kp));
+  std::vector<float> h((size_t)chunk * kp);
+  const float* R[3] = {c->W, c->MB, c->MC};
+  const DevCsr& g = c->blk[i];
+  int st = MMSB_OK;
+  for (int64_t r = 0; r < nrows && st == MMSB_OK; r += chunk) {
+    const int32_t m = (int32_t)std::min<int64_t>(chunk, nrows - r);
+    YRowsJobs jobs{};
+    jobs.j[0] = YRowsJob{g.rowptr + row0 + r, g.colidx, g.vals, R[i], d, kp, 0};
+    jobs.count = 1;
+    st = spmm_yrows(c->s, jobs, m, k, kp, false);
+    if (st == MMSB_OK) st = d2h(c->s, h.data(), d, sizeof(float) * (size_t)m * kp);
+    if (st == MMSB_OK)
+      for (int64_t x = 0; x < m; ++x)
+        for (int32_t j = 0; j < k; ++j) host[(r + x) * k + j] = h[(size_t)x * kp + j];
+  }
+  (void)hipFree(d);
+  return st;
+}
+}  // namespace mmsb
+
 extern "C" int64_t mmsb_get_size(mmsb_ctx* c, int32_t what) {
   if (!c) return 0;
   if (what == MMSB_Q_TIMES) return 4;
   if (what == MMSB_Q_SWEEPS) return c->whitened ? 2 : 0;
+  if (y_recomputed(c, what)) return c->n[0] * c->k;
   const View v = view_of(c, what);
   return v.p ? v.rows * v.cols : 0;
 }
@@ -1446,6 +1580,11 @@ extern "C" int mmsb_get(mmsb_ctx* c, int32_t what, double* host, int64_t count) 
     host[1] = st[7];
     return MMSB_OK;
   }
+  if (y_recomputed(c, what)) {
+    MMSB_CHECK_ARG(count >= c->n[0] * c->k, "get: buffer too small");
+    HCHK(hipSetDevice(c->cfg.device));
+    return yrows_host(c, what, 0, c->n[0], host);
+  }
   const View v = view_of(c, what);
   if (!v.p) return set_error(MMSB_ERR_STATE, "get: quantity %d not computed", what);
   MMSB_CHECK_ARG(count >= v.rows * v.cols, "get: buffer of %lld doubles, need %lld", (long long)count, (long long)(v.rows * v.cols));
@@ -1456,6 +1595,13 @@ extern "C" int mmsb_get(mmsb_ctx* c, int32_t what, double* host, int64_t count) 
 extern "C" int mmsb_get_rows(mmsb_ctx* c, int32_t what, int64_t row0, int64_t nrows, double* host, int64_t count) {
   clear_error();
   MMSB_CHECK_ARG(c && host, "get_rows: null argument");
+  if (y_recomputed(c, what)) {
+    MMSB_CHECK_ARG(row0 >= 0 && nrows >= 0 && row0 + nrows <= c->n[0], "get_rows: rows [%lld, %lld) outside [0, %lld)",
+                   (long long)row0, (long long)(row0 + nrows), (long long)c->n[0]);
+    MMSB_CHECK_ARG(count >= nrows * c->k, "get_rows: buffer too small");
+    HCHK(hipSetDevice(c->cfg.device));
+    return yrows_host(c, what, row0, nrows, host);
+  }
   View v = view_of(c, what);
   if (!v.p) return set_error(MMSB_ERR_STATE, "get_rows: quantity %d not computed", what);
   MMSB_CHECK_ARG(row0 >= 0 && nrows >= 0 && row0 + nrows <= v.rows, "get_rows: rows [%lld, %lld) outside [0, %lld)",

@@ -17,26 +17,20 @@
 // kernel is HBM-bound (each nonzero moves 4r bytes of B) and needs that memory-level
 // parallelism to approach the bandwidth roofline.
 #include "common.h"
+#include "dense.h"
 
 namespace mmsb {
 
+// Gathers one output row: acc[p] (on lanes of group 0) = sum over the row's nonzeros of v * B[c, :],
+// float4 chunk q = gl + p * LPG. Shared by the whole-matrix kernel and the minibatch-rows kernel, so
+// that a row is summed in the same order by both.
 template <int LPG, int P, bool VALS>
-__global__ __launch_bounds__(256) void spmm_csr_rowwave(int32_t m, const int32_t* __restrict__ rowptr,
-                                                        const int32_t* __restrict__ colidx,
-                                                        const float* __restrict__ vals, float alpha,
-                                                        const float* __restrict__ B, int32_t ldb,
-                                                        int32_t r, float* __restrict__ Y, int32_t ldy,
-                                                        const int32_t* __restrict__ order) {
-  const int32_t r4 = (r + 3) >> 2;
+__device__ __forceinline__ void row_gather(floatx4 (&acc)[P], int64_t row, const int32_t* __restrict__ rowptr,
+                                           const int32_t* __restrict__ colidx, const float* __restrict__ vals,
+                                           const float* __restrict__ B, int32_t ldb, int32_t r4, int lane) {
   constexpr int G = 64 / LPG;  // nonzeros gathered concurrently by one wave
-  const int lane = threadIdx.x & 63;
-  const int64_t slot = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-  if (slot >= m) return;  // uniform per wave
-  const int64_t row = order ? order[slot] : slot;
   const int grp = lane / LPG;
   const int gl = lane % LPG;
-
-  floatx4 acc[P];
 #pragma unroll
   for (int p = 0; p < P; ++p) acc[p] = floatx4{0.f, 0.f, 0.f, 0.f};
 
@@ -98,6 +92,24 @@ __global__ __launch_bounds__(256) void spmm_csr_rowwave(int32_t m, const int32_t
       acc[p].w += __shfl_xor(acc[p].w, off);
     }
   }
+}
+
+template <int LPG, int P, bool VALS>
+__global__ __launch_bounds__(256) void spmm_csr_rowwave(int32_t m, const int32_t* __restrict__ rowptr,
+                                                        const int32_t* __restrict__ colidx,
+                                                        const float* __restrict__ vals, float alpha,
+                                                        const float* __restrict__ B, int32_t ldb,
+                                                        int32_t r, float* __restrict__ Y, int32_t ldy,
+                                                        const int32_t* __restrict__ order) {
+  const int32_t r4 = (r + 3) >> 2;
+  const int lane = threadIdx.x & 63;
+  const int64_t slot = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (slot >= m) return;  // uniform per wave
+  const int64_t row = order ? order[slot] : slot;
+  const int grp = lane / LPG;
+  const int gl = lane % LPG;
+  floatx4 acc[P];
+  row_gather<LPG, P, VALS>(acc, row, rowptr, colidx, vals, B, ldb, r4, lane);
   if (grp == 0) {
     float* yrow = Y + row * (int64_t)ldy;
 #pragma unroll
@@ -154,6 +166,93 @@ int spmm_f32(hipStream_t s, int32_t m, int32_t n, int64_t nnz, const int32_t* ro
   if (r4 <= 256) return launch_spmm<64, 4>(s, m, rowptr, colidx, vals, alpha, B, ldb, r, Y, ldy, order);
   if (r4 <= 512) return launch_spmm<64, 8>(s, m, rowptr, colidx, vals, alpha, B, ldb, r, Y, ldy, order);
   return launch_spmm<64, 16>(s, m, rowptr, colidx, vals, alpha, B, ldb, r, Y, ldy, order);
+}
+
+// ---- whitened rows of one STGD minibatch -----------------------------------------------------------
+// Recomputes y_A, y_B, y_C of rows [row0, row0 + nb) of X from the CSR blocks (y = G_X* rows times
+// W, Z_B^T W, Z_C^T W; pre_proc.c:339-345 forms all of them up front) for STGD runs that do not keep
+// the n_X x k neighbour vectors resident. blockIdx.y selects the block; each row is summed by
+// row_gather exactly as the whole-matrix kernel sums it, and converted to the output type by the same
+// rounding as the stored vectors, so both forms feed STGD identical operands.
+template <int LPG, int P, typename TO>
+__global__ __launch_bounds__(256) void yrows_kernel(YRowsJobs jobs, int32_t nb, int32_t r, int32_t ldr, float alpha) {
+  const YRowsJob J = blockIdx.y == 0 ? jobs.j[0] : (blockIdx.y == 1 ? jobs.j[1] : jobs.j[2]);
+  const int32_t r4 = (r + 3) >> 2;
+  const int lane = threadIdx.x & 63;
+  const int64_t row = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (row >= nb) return;  // uniform per wave
+  const int grp = lane / LPG;
+  const int gl = lane % LPG;
+  floatx4 acc[P];
+  if (J.vals)
+    row_gather<LPG, P, true>(acc, row, J.rowptr, J.colidx, J.vals, J.R, ldr, r4, lane);
+  else
+    row_gather<LPG, P, false>(acc, row, J.rowptr, J.colidx, nullptr, J.R, ldr, r4, lane);
+  if (grp != 0) return;
+  TO* out = static_cast<TO*>(J.out);
+#pragma unroll
+  for (int p = 0; p < P; ++p) {
+    const int q = gl + p * LPG;
+    if (q >= r4) continue;
+    const floatx4 o = alpha * acc[p];
+    const TO v[4] = {(TO)o.x, (TO)o.y, (TO)o.z, (TO)o.w};
+    if (J.trans) {  // [r x ldo]: column `row` of the transposed block
+#pragma unroll
+      for (int e = 0; e < 4; ++e)
+        if (4 * q + e < r) out[(int64_t)(4 * q + e) * J.ldo + row] = v[e];
+    } else {
+      TO* yrow = out + row * J.ldo;
+      if (4 * q + 4 <= r) {  // whole chunk: one 16-byte (FP32) or 8-byte (BF16) store
+        if constexpr (sizeof(TO) == 4) {
+          *reinterpret_cast<floatx4*>(yrow + 4 * q) = o;
+        } else {
+          typedef TO tox4 __attribute__((ext_vector_type(4)));
+          *reinterpret_cast<tox4*>(yrow + 4 * q) = tox4{v[0], v[1], v[2], v[3]};
+        }
+      } else {
+#pragma unroll
+        for (int e = 0; e < 4; ++e)
+          if (4 * q + e < r) yrow[4 * q + e] = v[e];
+      }
+    }
+  }
+}
+
+template <int LPG, int P>
+static int launch_yrows(hipStream_t s, const YRowsJobs& jobs, int32_t nb, int32_t r, int32_t ldr, bool bf16) {
+  const dim3 grid((unsigned)ceil_div(nb, 4), (unsigned)jobs.count), block(256);
+  if (bf16)
+    hipLaunchKernelGGL((yrows_kernel<LPG, P, __bf16>), grid, block, 0, s, jobs, nb, r, ldr, 1.f);
+  else
+    hipLaunchKernelGGL((yrows_kernel<LPG, P, float>), grid, block, 0, s, jobs, nb, r, ldr, 1.f);
+  MMSB_LAUNCH_CHECK();
+  return MMSB_OK;
+}
+
+int spmm_yrows(hipStream_t s, const YRowsJobs& jobs, int32_t nb, int32_t r, int32_t ldr, bool bf16) {
+  MMSB_CHECK_ARG(nb >= 0 && r >= 1 && r <= ldr && r <= 4096 && ldr % 4 == 0, "yrows: bad sizes nb=%d r=%d ldr=%d", nb, r, ldr);
+  MMSB_CHECK_ARG(jobs.count >= 1 && jobs.count <= 3, "yrows: 1 to 3 jobs (got %d)", jobs.count);
+  if (nb == 0) return MMSB_OK;
+  for (int32_t i = 0; i < jobs.count; ++i) {
+    const YRowsJob& j = jobs.j[i];
+    MMSB_CHECK_ARG(j.rowptr && j.colidx && j.R && j.out, "yrows: null pointer");
+    MMSB_CHECK_ARG(aligned16(j.R), "yrows: R must be 16-byte aligned");
+    MMSB_CHECK_ARG(j.trans ? j.ldo >= nb : j.ldo >= r, "yrows: output leading dimension %lld too small", (long long)j.ldo);
+    MMSB_CHECK_ARG(j.trans || (j.ldo % 4 == 0 && aligned16(j.out)), "yrows: row-major output needs ldo %% 4 == 0 and 16-byte alignment");
+  }
+  const int32_t r4 = (r + 3) / 4;
+  if (r4 <= 1) return launch_yrows<1, 1>(s, jobs, nb, r, ldr, bf16);
+  if (r4 <= 2) return launch_yrows<2, 1>(s, jobs, nb, r, ldr, bf16);
+  if (r4 <= 4) return launch_yrows<4, 1>(s, jobs, nb, r, ldr, bf16);
+  if (r4 <= 8) return launch_yrows<8, 1>(s, jobs, nb, r, ldr, bf16);
+  if (r4 <= 16) return launch_yrows<16, 1>(s, jobs, nb, r, ldr, bf16);
+  if (r4 <= 32) return launch_yrows<32, 1>(s, jobs, nb, r, ldr, bf16);
+  if (r4 <= 64) return launch_yrows<64, 1>(s, jobs, nb, r, ldr, bf16);
+  if (r4 <= 128) return launch_yrows<64, 2>(s, jobs, nb, r, ldr, bf16);
+  if (r4 <= 192) return launch_yrows<64, 3>(s, jobs, nb, r, ldr, bf16);
+  if (r4 <= 256) return launch_yrows<64, 4>(s, jobs, nb, r, ldr, bf16);
+  if (r4 <= 512) return launch_yrows<64, 8>(s, jobs, nb, r, ldr, bf16);
+  return launch_yrows<64, 16>(s, jobs, nb, r, ldr, bf16);
 }
 
 }  // namespace mmsb

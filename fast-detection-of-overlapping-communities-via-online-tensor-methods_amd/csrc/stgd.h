@@ -4,6 +4,7 @@
 #include <hip/hip_runtime_api.h>
 
 #include <cstdint>
+#include <functional>
 #include <vector>
 
 namespace mmsb {
@@ -14,7 +15,7 @@ struct StgdArgs {
   int32_t b;            // minibatch tile rows (multiple of 64, >= bl)
   int32_t bl;           // logical minibatch: step s covers rows [s*bl, s*bl + bl) of X
   int64_t n_x;          // nodes in X
-  int64_t n_pad;        // rows of Y_A/Y_B and columns of Y_C^T (>= (ceil(n_x / bl) - 1) * bl + b)
+  int64_t n_pad;        // rows of Y_A/Y_B and columns of Y_C^T (>= (ceil(n_x / bl) - 1) * bl + b; >= b with y_window)
   float c3, c1, c0;     // tensor coefficients (GPU scaling of init_buff)
   float eta, theta;     // LEARN_RATE and the weight of the orthogonality term
   const double* mu_a;   // [kp] whitened means
@@ -24,6 +25,8 @@ struct StgdArgs {
   float* phit;          // [kp x kp] Phi~ after the gradient step; set per step
   float* cs_part;       // [b/32 x kp] per-row-tile column sums of P_A o P_B
   float* mab;           // [2 x kp] m_A, m_B
+  int32_t y_window;     // 0: Y holds all of X; 1: Y holds the current minibatch only (rows from 0), refilled
+                        // before each step by the caller's fill callback (checkpointed STGD)
 };
 
 // Device buffers of a run (all [kp x kp] FP32 row-major unless noted). The *T buffers hold the
@@ -70,9 +73,13 @@ struct StgdTiming {
 // result on exit. frob_log[st] receives ||Phi_{t+1} - Phi_t||_F^2 of the st-th step of this call.
 // precision is MMSB_PREC_FP32 (Y as float) or MMSB_PREC_BF16 (Y as BF16). Without timing every
 // launch is capturable in a HIP graph (no memset nodes, no host synchronisation).
+// With args.y_window, fill(row0, nb) is called before each step's first launch to enqueue (on s) the
+// kernels that write rows [row0, row0 + nb) of X into the Y buffers.
+using StgdFill = std::function<int(int64_t row0, int32_t nb)>;
 int stgd_run(hipStream_t s, int32_t precision, const StgdArgs& args, const void* YA, const void* YB, const void* YCt,
              int64_t step0, int64_t nsteps, int32_t renorm_every, int32_t ns_iters, int32_t ns_degree,
-             const StgdBuffers& bufs, int& cur, double* frob_log, StgdTiming* timing = nullptr);
+             const StgdBuffers& bufs, int& cur, double* frob_log, StgdTiming* timing = nullptr,
+             const StgdFill& fill = StgdFill());
 
 // ---- many instances in one persistent launch ----------------------------------------------------
 // One member of a batch: its step arguments, buffers and neighbour vectors (FP32 or BF16 as the

@@ -439,6 +439,7 @@ struct StepCtx {
   int32_t ns_iters;
   int32_t ns_degree;   // 3 or 5
   int64_t row0;        // first X row of the minibatch
+  int64_t yrow0;       // its first row in the Y buffers (row0, or 0 when they hold only the minibatch)
   int32_t nb;          // rows in the minibatch
   double* frob;        // this step's ||Phi_{t+1} - Phi_t||_F^2
   double* frob_zero;   // SETUP: the run's frob entries, cleared by tile 0
@@ -582,8 +583,8 @@ __device__ void p1_tile(const StepCtx& P, int tile, StgdSmem& sm) {
     // every wave: P_A and P_B over its quarter of the k reduction, sharing the Phi^T fragments
     floatx16 acc_b = {};
     kslice<DMODE>(kp, 4, w, k0, kl);
-    wave_mm_pair<DMODE>(acc, acc_b, static_cast<const TY*>(P.YA) + P.row0 * (int64_t)kp,
-                        static_cast<const TY*>(P.YB) + P.row0 * (int64_t)kp, kp, pick<DMODE>(P.B.phiT, P.B.phiT16, nel, kp).hi,
+    wave_mm_pair<DMODE>(acc, acc_b, static_cast<const TY*>(P.YA) + P.yrow0 * (int64_t)kp,
+                        static_cast<const TY*>(P.YB) + P.yrow0 * (int64_t)kp, kp, pick<DMODE>(P.B.phiT, P.B.phiT16, nel, kp).hi,
                         kp, x0, j0, k0, kl);
     {
       const float m = mpart_reduce(ps, P.B.mpart, kp, ntk, j0);
@@ -661,9 +662,9 @@ __device__ void p2_tile(const StepCtx& P, int tile, StgdSmem& sm) {
                          k0, kl);
   } else {      // Y_C^T C_g over two halves of the minibatch tile
     kslice<DMODE>(a.b, 2, w & 1, k0, kl);
-    const Opnd<TY> yc = opnd<TY>(static_cast<const TY*>(P.YCt) + P.row0, nullptr, a.n_pad);
+    const Opnd<TY> yc = opnd<TY>(static_cast<const TY*>(P.YCt) + P.yrow0, nullptr, a.n_pad);
     const Opnd<TPH> cg = opnd<TPH>(static_cast<const TPH*>(P.B.cgt), nullptr, a.b);
-    if (P.row0 % VEC == 0)
+    if (P.yrow0 % VEC == 0)
       wave_mm<DMODE, true>(acc, yc, cg, i0, j0, k0, kl);
     else
       wave_mm<DMODE, false>(acc, yc, cg, i0, j0, k0, kl);
@@ -912,6 +913,7 @@ __device__ inline void make_step(const BatchInst& I, int64_t st, StepCtx& P) {
   P.ns_degree = I.ns_degree;
   const int64_t g = I.step0 + st;
   P.row0 = (g % I.spe) * I.a.bl;
+  P.yrow0 = P.row0;
   P.nb = (int32_t)min((int64_t)I.a.bl, I.a.n_x - P.row0);
   P.a.phi = I.B.phi[P.cur];
   P.a.phit = I.B.phit;
@@ -1000,7 +1002,7 @@ static bool step_renorms(int64_t gstep, int32_t renorm_every, int32_t ns_iters) 
 template <int DMODE, int KMODE, typename TY, typename TPH>
 static int run_steps(hipStream_t s, const StgdArgs& args, const void* YA, const void* YB, const void* YCt, int64_t step0,
                      int64_t nsteps, int32_t renorm_every, int32_t ns_iters, int32_t ns_degree, const StgdBuffers& B,
-                     int& cur, double* frob_log, StgdTiming* timing) {
+                     int& cur, double* frob_log, StgdTiming* timing, const StgdFill& fill) {
   const int kp = args.kp;
   const int64_t spe = ceil_div(args.n_x, args.bl);
   StepCtx P{};
@@ -1036,9 +1038,11 @@ static int run_steps(hipStream_t s, const StgdArgs& args, const void* YA, const 
     P.cur = cur;
     P.row0 = (gstep % spe) * args.bl;
     P.nb = (int32_t)std::min<int64_t>(args.bl, args.n_x - P.row0);
+    P.yrow0 = args.y_window ? 0 : P.row0;
     P.a.phi = B.phi[cur];
     P.frob = frob_log + st;
     if (timing) MMSB_HIP(hipEventRecord(timing->ev[0], s));
+    if (args.y_window) MMSB_TRY(fill(P.row0, P.nb));
     MMSB_TRY(launch(PH_P1, 0));
     if (timing) MMSB_HIP(hipEventRecord(timing->ev[1], s));
     MMSB_TRY(launch(PH_P2, 0));
@@ -1072,7 +1076,7 @@ static int check_args(const StgdArgs& args, int32_t ns_iters, int32_t ns_degree)
   MMSB_CHECK_ARG(args.bl >= 1 && args.bl <= args.b, "stgd: logical minibatch %d outside [1, %d]", args.bl, args.b);
   MMSB_CHECK_ARG(args.n_pad % 64 == 0, "stgd: n_pad must be a multiple of 64");
   const int64_t spe = ceil_div(args.n_x, args.bl);
-  MMSB_CHECK_ARG(args.n_pad >= (spe - 1) * args.bl + args.b, "stgd: n_pad too small");
+  MMSB_CHECK_ARG(args.n_pad >= (args.y_window ? (int64_t)args.b : (spe - 1) * args.bl + args.b), "stgd: n_pad too small");
   MMSB_CHECK_ARG(ns_iters >= 0, "stgd: negative iteration count");
   MMSB_CHECK_ARG(ns_degree == 3 || ns_degree == 5, "stgd: Newton-Schulz degree must be 3 or 5 (got %d)", ns_degree);
   return MMSB_OK;
@@ -1087,18 +1091,19 @@ static int check_split(const StgdBuffers& bufs) {
 
 int stgd_run(hipStream_t s, int32_t precision, const StgdArgs& args, const void* YA, const void* YB, const void* YCt,
              int64_t step0, int64_t nsteps, int32_t renorm_every, int32_t ns_iters, int32_t ns_degree,
-             const StgdBuffers& bufs, int& cur, double* frob_log, StgdTiming* timing) {
+             const StgdBuffers& bufs, int& cur, double* frob_log, StgdTiming* timing, const StgdFill& fill) {
   MMSB_TRY(check_args(args, ns_iters, ns_degree));
   MMSB_CHECK_ARG(nsteps >= 0, "stgd: negative step count");
+  MMSB_CHECK_ARG(!args.y_window || fill, "stgd: a minibatch window needs a fill callback");
   if (nsteps == 0) return MMSB_OK;
   if (precision == MMSB_PREC_BF16) {
     MMSB_TRY(check_split(bufs));
     return run_steps<MM_BF16, MM_BF16X3, __bf16, __bf16>(s, args, YA, YB, YCt, step0, nsteps, renorm_every, ns_iters,
-                                                         ns_degree, bufs, cur, frob_log, timing);
+                                                         ns_degree, bufs, cur, frob_log, timing, fill);
   }
   MMSB_CHECK_ARG(precision == MMSB_PREC_FP32, "stgd: unknown precision %d", precision);
   return run_steps<MM_F32, MM_F32, float, float>(s, args, YA, YB, YCt, step0, nsteps, renorm_every, ns_iters, ns_degree,
-                                                 bufs, cur, frob_log, timing);
+                                                 bufs, cur, frob_log, timing, fill);
 }
 
 // ---- batch ------------------------------------------------------------------------------------------
@@ -1135,6 +1140,7 @@ int stgd_batch_plan(StgdBatch*& out, int32_t precision, const std::vector<StgdBa
   for (size_t i = 0; i < members.size(); ++i) {
     const StgdBatchMember& m = members[i];
     MMSB_TRY(check_args(m.args, ns_iters, ns_degree));
+    MMSB_CHECK_ARG(!m.args.y_window, "stgd batch: instance %zu recomputes its minibatches (not supported in a batch)", i);
     if (precision == MMSB_PREC_BF16) MMSB_TRY(check_split(m.bufs));
     MMSB_CHECK_ARG(m.nsteps >= 1, "stgd batch: instance %zu has no steps", i);
     BatchInst& I = insts[i];

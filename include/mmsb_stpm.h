@@ -174,6 +174,12 @@ void mmsb_graph_free(mmsb_graph* g);
 
 enum { MMSB_PREC_FP32 = 0, MMSB_PREC_BF16 = 1 };
 enum { MMSB_M2_PAPER = 0, MMSB_M2_GPU = 1 };
+/* Neighbour vectors: MMSB_Y_STORED keeps Y_A, Y_B, Y_C (3 n_X x k, as pre_proc.c:339-345 does);
+ * MMSB_Y_RECOMPUTE keeps only W, Z_B^T W, Z_C^T W and rebuilds each STGD minibatch's rows from
+ * CSR before the step (checkpointed STGD; the whitening sketches and V are released after
+ * whitening); MMSB_Y_AUTO recomputes when the stored form would not fit in 90% of the free
+ * device memory. Both forms give identical STGD results. */
+enum { MMSB_Y_AUTO = 0, MMSB_Y_STORED = 1, MMSB_Y_RECOMPUTE = 2 };
 
 typedef struct stpm_config {
   int32_t nx, na, nb, nc;     /* NX, NA, NB, NC (stpm.h:72-75)                                   */
@@ -209,6 +215,7 @@ typedef struct stpm_config {
   int32_t device;             /* HIP device ordinal                                              */
   int32_t verbose;            /* progress lines on stdout                                        */
   int32_t use_graphs;         /* replay whole STGD epochs as captured HIP graphs                 */
+  int32_t y_mode;             /* MMSB_Y_*: neighbour vectors stored or recomputed per minibatch  */
 } stpm_config;
 
 /* Fills the reference defaults of stpm.h:47-106 (DBLP sizes, NCOM 100, ALPHA_0 1, LEARN_RATE 1e-3,
@@ -307,7 +314,8 @@ int mmsb_post(mmsb_ctx* ctx);
 int mmsb_write_outputs(mmsb_ctx* ctx);
 /* Copies a quantity to host memory as double; count is the number of doubles available. */
 int mmsb_get(mmsb_ctx* ctx, int32_t what, double* host, int64_t count);
-/* Rows [row0, row0 + nrows) of a row-indexed quantity (tall matrices, vectors, counts). */
+/* Rows [row0, row0 + nrows) of a row-indexed quantity (tall matrices, vectors, counts). Y_A, Y_B
+ * and Y_C of a context that recomputes its minibatches are rebuilt from CSR for the call. */
 int mmsb_get_rows(mmsb_ctx* ctx, int32_t what, int64_t row0, int64_t nrows, double* host, int64_t count);
 /* Copies an FP32-resident quantity (tall matrices: W, U, V, L_B, L_C, Y_A, Y_B, Pi_*) in its
  * native precision, compact row-major [rows x k]. */
