@@ -64,7 +64,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="c3", choices=sorted(WORKLOADS) + ["c4"])
+    ap.add_argument("--workload", default="c3", choices=sorted(WORKLOADS) + ["c4", "c5"])
+    ap.add_argument("--stress-steps", type=int, default=64,
+                    help="c5: STGD steps compared against the FP64 oracle")
     ap.add_argument("--precision", default="bf16", choices=["fp32", "bf16"])
     ap.add_argument("--no-oracle", action="store_true", help="skip the FP64 oracle epochs (accuracy, cpu_baseline)")
     ap.add_argument("--accuracy-epochs", type=int, default=2)
@@ -91,6 +93,8 @@ def main():
     m = ge.load_package()
     if args.workload == "c4":
         return bench_batch(m, args)
+    if args.workload == "c5":
+        return bench_stress(m, args)
 
     w = dict(WORKLOADS[args.workload])
     if args.minibatch:
@@ -384,6 +388,134 @@ def spmm_roofline(m, blk, r):
             "ms": ms, "rows": nr, "nnz": int(nnz), "r": r, "bytes_per_launch": bytes_alg,
             "row_order": "longest rows first (mmsb_csr_row_order), as in the pipeline",
             "ms_natural_row_order": ms_natural}
+
+
+def bench_stress(m, args):
+    """Memory-bound stress workload: 4M nodes per partition, k = 1000, alpha0 = 1, 40 nonzeros per row
+    in each of G_XA, G_XB, G_XC (480M over the three; mean total degree 160, P = (k+1) p_out on the
+    diagonal, p_out off it), binary CSR. The resident neighbour vectors and their BF16 copies, with the
+    whitening and post_proc buffers, would exceed the device memory, so the library's automatic
+    y_mode recomputes each STGD minibatch's whitened rows from CSR (checkpointed STGD). BF16 STGD with
+    FP32 accumulation; FP64 Gram, Cholesky and Jacobi factorisations in whitening.
+
+    Accuracy: from the same Phi0, `--stress-steps` BF16 and FP32 steps against the FP64 oracle step
+    (numpy) on the same whitened rows (served by mmsb_get_rows, i.e. recomputed from CSR), then Pi_X
+    and its normalisation on a 10k-node sample of X against the oracle's Pi = y_A Phi Lambda'."""
+    import numpy as np
+    from oracle import core
+    n, k, b, eta, alpha0 = 4000000, 1000, args.minibatch or 1024, 1e-5, 1.0
+    p_in, p_out = c3_probs(n, k, 160.0)
+    t0 = time.time()
+    g = m.generate(n, n, n, n, k, alpha0, p_in, p_out, seed=0, threads=16, verbose=True)
+    t_gen = time.time() - t0
+    nnz = {nm: int(g[nm][1].size) for nm in ("XA", "XB", "XC", "BA", "CA")}
+    log(f"generated c5: {g['n_edges']} edges, nnz {nnz}, {t_gen:.1f}s")
+    cfg = m.config(nx=n, na=n, nb=n, nc=n, ncom=k, alpha_0=alpha0, learn_rate=eta, minibatch=b, epochs=1,
+                   polish_epochs=0, oversample=10, power_iters=1, seed=1, precision="bf16", verbose=1)
+    t0 = time.time()
+    p = m.Pipeline(cfg)
+    p.set_graph(g)
+    t_up = time.time() - t0
+    xa = g["XA"]
+    del g
+    p.whiten()
+    t_wh = time.time() - t0 - t_up
+    recomputed = m.lib.mmsb_get_size(p.ctx, m.Q["V"]) == 0      # V is released in the recompute form
+    log(f"whitened in {t_wh:.1f}s; neighbour vectors {'recomputed per minibatch' if recomputed else 'resident'}")
+    p.stgd(0, "fp32")
+    phi0 = p.get("phi")
+    t1 = time.time()
+    p.stgd(-1, "bf16")
+    t_st = time.time() - t1
+    t2 = time.time()
+    p.eigen()
+    p.post()
+    t_post = time.time() - t2
+    nonfinite = 0
+    for s_ in "xabc":
+        a = p.get_f32("pin_" + s_)
+        nonfinite += int((~np.isfinite(a)).sum())
+        del a
+    t_e2e = time.time() - t0
+    t_dl = time.time() - t2 - t_post
+    log(f"end-to-end {t_e2e:.1f}s (upload {t_up:.1f}, whiten {t_wh:.1f}, stgd {t_st:.2f}, post {t_post:.2f}, "
+        f"download {t_dl:.1f})")
+
+    spe = math.ceil(n / b)
+    rates = {}
+    for prec in ("fp32", "bf16"):
+        for _ in range(args.warmup):
+            p.stgd(spe, prec)
+        m.check(m.lib.mmsb_sync(p.ctx))
+        ts = time.time()
+        for _ in range(args.steps):
+            p.stgd(spe, prec)
+        m.check(m.lib.mmsb_sync(p.ctx))
+        dt = time.time() - ts
+        rates[prec] = (n * args.steps / dt, 1e3 * dt / args.steps)
+        log(f"STGD {prec}: {rates[prec][0]:.3e} nodes/s, {rates[prec][1]:.1f} ms/epoch")
+    ph_ms = p.stgd_timed(64, "bf16")
+    phases_us = {nm: 1e3 * t / 64 for nm, t in zip(("fill_and_p1", "p2", "renorm"), ph_ms)}
+    log(f"phases (us/step, bf16): {phases_us}")
+
+    # ---- accuracy on a prefix of X (S steps) and a 10k-node sample, against the FP64 oracle --------
+    S = args.stress_steps
+    rows = S * b
+    ya, yb, yc = (p.get_rows(nm, 0, rows) for nm in ("YA", "YB", "YC"))
+    mus = [p.get(nm) for nm in ("muw_a", "muw_b", "muw_c")]
+    t0 = time.time()
+    phi64, _ = core.stgd(ya, yb, yc, *mus, phi0, alpha0=alpha0, eta=eta, batch=b, epochs=1, ns_iters=2, ns_degree=3)
+    t_oracle = time.time() - t0
+    del ya, yb, yc
+    sample0, ns_ = n // 2, 10000
+    ys = p.get_rows("YA", sample0, ns_)
+    _, lamp64, _ = core.eigenvalues(phi64)
+    pi64 = ys @ (phi64 * lamp64)
+    pin64 = core.norm_prob_rows(pi64)
+    acc = {"steps": S, "nodes": rows, "eta": eta, "sample_rows": [sample0, sample0 + ns_],
+           "reference": "oracle FP64 STGD (numpy) over the first S minibatches of X on the same whitened rows; "
+                        "Pi_X = y_A Phi Lambda' on the sample",
+           "phi_rel_err": {}, "pi_x_rel_err": {}, "pin_x_rel_err": {}, "oracle_phi_moved": rel(phi64, phi0)}
+    for prec in ("fp32", "bf16"):
+        p.set_phi(phi0)
+        p.stgd(S, prec)
+        phi = p.get("phi")
+        p.eigen()
+        p.post()
+        pi = p.get_rows("pi_x", sample0, ns_)
+        pin = p.get_rows("pin_x", sample0, ns_)
+        acc["phi_rel_err"][prec] = rel(phi, phi64)
+        acc["pi_x_rel_err"][prec] = rel(pi, pi64)
+        acc["pin_x_rel_err"][prec] = rel(pin, pin64)
+    log(f"accuracy {acc}")
+    p.close()
+    spmm = spmm_roofline(m, xa, r=k + 10)
+    log(f"spmm {spmm}")
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    out = {
+        "metric": f"STGD nodes/s (n={n}/partition, k={k})",
+        "value": rates["bf16"][0], "unit": "nodes/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": rates["bf16"][1], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "bf16", "data": "synthetic (seeded MMSB generator)",
+        "config": {"workload": f"mmsb-c5 (memory-bound stress): {n} nodes/partition, k={k}, alpha0={alpha0}, "
+                               f"40 nonzeros per row per X block, minibatch {b}, eta {eta:g}, neighbour vectors "
+                               f"{'recomputed from CSR per minibatch' if recomputed else 'resident'}, one step = "
+                               f"one STGD epoch over X",
+                   "n_per_partition": n, "k": k, "minibatch": b, "nnz": nnz, "y_recomputed": bool(recomputed),
+                   "parallelism": "none"},
+        "end_to_end_s": t_e2e,
+        "stages_s": {"upload_and_transpose": t_up, "whiten": t_wh, "stgd_1_epoch": t_st, "eigen_post": t_post,
+                     "download_pin": t_dl, "generate": t_gen},
+        "nonfinite_pin": nonfinite,
+        "stgd_nodes_per_s": {pr: v[0] for pr, v in rates.items()},
+        "bf16_speedup_vs_fp32": rates["bf16"][0] / rates["fp32"][0],
+        "phases_us_per_step_bf16": phases_us,
+        "accuracy": acc,
+        "spmm": spmm,
+        "cpu_baseline": {"value": rows / t_oracle, "unit": "nodes/s", "cores": threads, "kind": "port",
+                         "sample": f"{S} STGD steps (b={b}, k={k}) of the oracle's FP64 numpy step, {t_oracle:.1f}s"},
+    }
+    print(json.dumps(out), flush=True)
 
 
 def bench_batch(m, args):

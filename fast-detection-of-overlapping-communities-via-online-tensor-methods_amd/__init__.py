@@ -57,7 +57,8 @@ class GenParamsC(ctypes.Structure):
     _fields_ = [("n_x", ctypes.c_int32), ("n_a", ctypes.c_int32), ("n_b", ctypes.c_int32), ("n_c", ctypes.c_int32),
                 ("k", ctypes.c_int32), ("alpha0", ctypes.c_double), ("p_in", ctypes.c_double),
                 ("p_out", ctypes.c_double), ("seed", ctypes.c_uint64), ("degree_corrected", ctypes.c_int32),
-                ("pareto_shape", ctypes.c_double), ("mode", ctypes.c_int32), ("threads", ctypes.c_int32)]
+                ("pareto_shape", ctypes.c_double), ("mode", ctypes.c_int32), ("threads", ctypes.c_int32),
+                ("verbose", ctypes.c_int32)]
 
 
 GEN_MODES = {"auto": 0, "exact": 1, "balls": 2}
@@ -217,12 +218,12 @@ def _csr_to_numpy(c: CsrC):
 
 
 def generate(n_x, n_a, n_b, n_c, k, alpha0, p_in, p_out, seed=0, degree_corrected=False, pareto_shape=2.5,
-             mode="auto", threads=0, keep_truth=False) -> dict:
+             mode="auto", threads=0, keep_truth=False, verbose=False) -> dict:
     """Seeded MMSB graph from the library generator; returns numpy copies of the five CSR blocks
     (rowptr, colidx, vals, n_rows, n_cols) and, optionally, the ground truth."""
     import numpy as np
     p = GenParamsC(n_x, n_a, n_b, n_c, k, alpha0, p_in, p_out, seed, int(degree_corrected), pareto_shape,
-                   GEN_MODES[mode], threads)
+                   GEN_MODES[mode], threads, int(verbose))
     g = ctypes.POINTER(GraphC)()
     check(lib.mmsb_generate(ctypes.byref(p), int(keep_truth), ctypes.byref(g)))
     try:

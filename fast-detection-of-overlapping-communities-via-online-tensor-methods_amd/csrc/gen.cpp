@@ -13,6 +13,8 @@
 #include <algorithm>
 #include <atomic>
 #include <cmath>
+#include <cstdarg>
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <functional>
@@ -26,6 +28,15 @@
 namespace mmsb {
 
 static constexpr double kCut = -20.0;
+
+static void glog(const mmsb_gen_params& p, const char* fmt, ...) {
+  if (!p.verbose) return;
+  va_list ap;
+  va_start(ap, fmt);
+  std::vprintf(fmt, ap);
+  va_end(ap);
+  std::fflush(stdout);
+}
 
 static double gamma1p(uint64_t seed, uint32_t node, uint32_t comp, double alpha) {
   const double a = 1.0 + alpha;
@@ -194,6 +205,41 @@ static int64_t pick(const double* cum, int64_t size, double u) {
   return idx < size - 1 ? idx : size - 1;
 }
 
+// pick() over one large cumulative array through a guide table: guide[j] is the first index whose
+// cum exceeds t_j = j * W / G. For t_j <= x < t_{j+1} the first index with cum > x lies in
+// [guide[j], guide[j+1]], so the search there returns exactly what the search over the whole array
+// returns, touching one or two cache lines instead of ~log2(size) scattered ones.
+struct GuidedCum {
+  const double* cum = nullptr;
+  int64_t size = 0;
+  int64_t G = 0;
+  std::vector<double> t;        // [G + 1] bucket starts
+  std::vector<int64_t> guide;   // [G + 1]
+
+  GuidedCum(const double* c, int64_t n, int threads) : cum(c), size(n) {
+    G = std::max<int64_t>(1, n / 16);
+    const double W = cum[size - 1];
+    t.resize(G + 1);
+    guide.resize(G + 1);
+    parallel_for(G + 1, threads, [&](int64_t lo, int64_t hi) {
+      for (int64_t j = lo; j < hi; ++j) {
+        t[j] = (double)j * W / (double)G;
+        guide[j] = std::upper_bound(cum, cum + size, t[j]) - cum;
+      }
+    });
+  }
+  int64_t operator()(double u) const {
+    const double x = u * cum[size - 1];
+    int64_t j = (int64_t)(x / cum[size - 1] * (double)G);
+    j = std::min<int64_t>(std::max<int64_t>(j, 0), G - 1);
+    while (j > 0 && x < t[j]) --j;
+    while (j < G - 1 && x >= t[j + 1]) ++j;
+    const int64_t lo = guide[j], hi = std::min<int64_t>(guide[j + 1] + 1, size);
+    const int64_t idx = std::upper_bound(cum + lo, cum + hi, x) - cum;
+    return idx < size - 1 ? idx : size - 1;
+  }
+};
+
 static void edges_balls(const mmsb_gen_params& p, int threads, const std::vector<double>& theta,
                         const SparsePi& sp, std::vector<uint64_t>& keys) {
   const int64_t n = (int64_t)theta.size();
@@ -231,27 +277,35 @@ static void edges_balls(const mmsb_gen_params& p, int threads, const std::vector
   e0[0] = m_bg;
   for (int c = 0; c < k; ++c) e0[c + 1] = e0[c] + m_c[c];
   const int64_t total = e0[k];
+  glog(p, "generate: dropping %lld balls\n", (long long)total);
+  const GuidedCum bg(cum_theta.data(), n, threads);
   std::vector<uint64_t> raw(total);
-  std::atomic<int64_t> kept{0};
-  parallel_for(total, threads, [&](int64_t lo, int64_t hi) {
-    int c = (int)(std::upper_bound(e0.begin(), e0.end(), lo) - e0.begin()) - 1;  // -1 = background
-    for (int64_t e = lo; e < hi; ++e) {
-      while (c < k - 1 && e >= e0[c + 1]) ++c;
-      const u32x4 w = draw(p.seed, kStreamBall, (uint32_t)((uint64_t)e & 0xFFFFFFFFu), (uint32_t)((uint64_t)e >> 32), 0);
-      const double u1 = u53(w.x, w.y), u2 = u53(w.z, w.w);
-      int64_t i, j;
-      if (c < 0) {
-        i = pick(cum_theta.data(), n, u1);
-        j = pick(cum_theta.data(), n, u2);
-      } else {
-        const int64_t b = cptr[c], sz = cptr[c + 1] - cptr[c];
-        i = mem[b + pick(cw.data() + b, sz, u1)];
-        j = mem[b + pick(cw.data() + b, sz, u2)];
+  // every ball is a function of its index alone; the parts only pace the progress lines
+  const int parts = total > ((int64_t)1 << 28) ? 8 : 1;
+  for (int part = 0; part < parts; ++part) {
+    const int64_t plo = total * part / parts, phi = total * (part + 1) / parts;
+    parallel_for(phi - plo, threads, [&](int64_t lo0, int64_t hi0) {
+      const int64_t lo = plo + lo0, hi = plo + hi0;
+      int c = (int)(std::upper_bound(e0.begin(), e0.end(), lo) - e0.begin()) - 1;  // -1 = background
+      for (int64_t e = lo; e < hi; ++e) {
+        while (c < k - 1 && e >= e0[c + 1]) ++c;
+        const u32x4 w = draw(p.seed, kStreamBall, (uint32_t)((uint64_t)e & 0xFFFFFFFFu), (uint32_t)((uint64_t)e >> 32), 0);
+        const double u1 = u53(w.x, w.y), u2 = u53(w.z, w.w);
+        int64_t i, j;
+        if (c < 0) {
+          i = bg(u1);
+          j = bg(u2);
+        } else {
+          const int64_t b = cptr[c], sz = cptr[c + 1] - cptr[c];
+          i = mem[b + pick(cw.data() + b, sz, u1)];
+          j = mem[b + pick(cw.data() + b, sz, u2)];
+        }
+        raw[e] = (i == j) ? ~0ull : (uint64_t)std::min(i, j) * (uint64_t)n + (uint64_t)std::max(i, j);
       }
-      raw[e] = (i == j) ? ~0ull : (uint64_t)std::min(i, j) * (uint64_t)n + (uint64_t)std::max(i, j);
-    }
-  });
-  (void)kept;
+    });
+    if (parts > 1) glog(p, "generate: balls %d/%d\n", part + 1, parts);
+  }
+  glog(p, "generate: sorting the edge keys\n");
   parallel_sort_unique(raw, threads);
   if (!raw.empty() && raw.back() == ~0ull) raw.pop_back();
   keys.swap(raw);
@@ -284,22 +338,42 @@ static int build_blocks(const mmsb_gen_params& p, int threads, const std::vector
   const int sizes[4] = {p.n_x, p.n_a, p.n_b, p.n_c};
   const int pr[5] = {0, 0, 0, 2, 3}, pc[5] = {1, 2, 3, 1, 1};
   mmsb_csr* out[5] = {&g->xa, &g->xb, &g->xc, &g->ba, &g->ca};
+  // Each unordered pair of partitions feeds at most one block: route[part_i][part_j] = block, with
+  // bit 3 set when node j indexes the block's rows.
+  int8_t route[4][4];
+  std::memset(route, -1, sizeof(route));
+  for (int b = 0; b < 5; ++b) {
+    route[pr[b]][pc[b]] = (int8_t)b;
+    route[pc[b]][pr[b]] = (int8_t)(b | 8);
+  }
+  auto locate = [&](uint64_t key, int& b, int32_t& row, int32_t& col) -> bool {
+    const int64_t i = (int64_t)(key / (uint64_t)n), j = (int64_t)(key % (uint64_t)n);
+    const int8_t r = route[part[i]][part[j]];
+    if (r < 0) return false;
+    b = r & 7;
+    if (r & 8) { row = local[j]; col = local[i]; } else { row = local[i]; col = local[j]; }
+    return true;
+  };
+  std::vector<std::vector<int64_t>> cnt(5);
+  for (int b = 0; b < 5; ++b) cnt[b].assign((size_t)sizes[pr[b]] + 1, 0);
+  // Rows are filled in any order and sorted afterwards (keys are unique, so a row holds distinct
+  // columns): the CSR does not depend on the thread count.
+  const int64_t nkeys = (int64_t)keys.size();
+  parallel_for(nkeys, threads, [&](int64_t lo, int64_t hi) {
+    for (int64_t t = lo; t < hi; ++t) {
+      int b;
+      int32_t r, c;
+      if (locate(keys[t], b, r, c)) __atomic_fetch_add(&cnt[b][r + 1], (int64_t)1, __ATOMIC_RELAXED);
+    }
+  });
+  static const char* const kNames[5] = {"XA", "XB", "XC", "BA", "CA"};
+  std::vector<std::vector<int64_t>> fill(5);
   for (int b = 0; b < 5; ++b) {
     const int nr = sizes[pr[b]], nc = sizes[pc[b]];
-    std::vector<int64_t> cnt(nr + 1, 0);
-    auto route = [&](uint64_t key, int32_t& row, int32_t& col) -> bool {
-      const int64_t i = (int64_t)(key / (uint64_t)n), j = (int64_t)(key % (uint64_t)n);
-      if (part[i] == pr[b] && part[j] == pc[b]) { row = local[i]; col = local[j]; return true; }
-      if (part[j] == pr[b] && part[i] == pc[b]) { row = local[j]; col = local[i]; return true; }
-      return false;
-    };
-    for (uint64_t key : keys) {
-      int32_t r, c;
-      if (route(key, r, c)) cnt[r + 1]++;
-    }
-    for (int i = 0; i < nr; ++i) cnt[i + 1] += cnt[i];
-    const int64_t nnz = cnt[nr];
+    for (int i = 0; i < nr; ++i) cnt[b][i + 1] += cnt[b][i];
+    const int64_t nnz = cnt[b][nr];
     if (nnz >= INT32_MAX) return set_error(MMSB_ERR_ARG, "generate: block %d has %lld nonzeros (int32 CSR)", b, (long long)nnz);
+    glog(p, "generate: block %s (%lld nonzeros)\n", kNames[b], (long long)nnz);
     mmsb_csr* m = out[b];
     m->n_rows = nr;
     m->n_cols = nc;
@@ -308,16 +382,23 @@ static int build_blocks(const mmsb_gen_params& p, int threads, const std::vector
     m->colidx = (int32_t*)std::malloc(sizeof(int32_t) * std::max<int64_t>(nnz, 1));
     m->vals = nullptr;
     if (!m->rowptr || !m->colidx) return set_error(MMSB_ERR_NOMEM, "generate: out of host memory");
-    for (int i = 0; i <= nr; ++i) m->rowptr[i] = (int32_t)cnt[i];
-    std::vector<int64_t> fill(cnt.begin(), cnt.end() - 1);
-    for (uint64_t key : keys) {
+    for (int i = 0; i <= nr; ++i) m->rowptr[i] = (int32_t)cnt[b][i];
+    fill[b].assign(cnt[b].begin(), cnt[b].end() - 1);
+  }
+  parallel_for(nkeys, threads, [&](int64_t lo, int64_t hi) {
+    for (int64_t t = lo; t < hi; ++t) {
+      int b;
       int32_t r, c;
-      if (route(key, r, c)) m->colidx[fill[r]++] = c;
+      if (locate(keys[t], b, r, c)) out[b]->colidx[__atomic_fetch_add(&fill[b][r], (int64_t)1, __ATOMIC_RELAXED)] = c;
     }
-    parallel_for(nr, threads, [&](int64_t lo, int64_t hi) {
+  });
+  for (int b = 0; b < 5; ++b) {
+    mmsb_csr* m = out[b];
+    parallel_for(m->n_rows, threads, [&](int64_t lo, int64_t hi) {
       for (int64_t i = lo; i < hi; ++i) std::sort(m->colidx + m->rowptr[i], m->colidx + m->rowptr[i + 1]);
     });
   }
+  glog(p, "generate: blocks built\n");
   return MMSB_OK;
 }
 
@@ -352,6 +433,7 @@ extern "C" int mmsb_generate(const mmsb_gen_params* pp, int32_t keep_truth, mmsb
   SparsePi sp;
   gen_theta(p, threads, theta);
   gen_pi(p, threads, sp);
+  glog(p, "generate: memberships of %lld nodes (%lld nonzero components)\n", (long long)n, (long long)sp.comp.size());
   std::vector<uint64_t> keys;
   if (mode == MMSB_GEN_EXACT)
     edges_exact(p, threads, theta, sp, keys);

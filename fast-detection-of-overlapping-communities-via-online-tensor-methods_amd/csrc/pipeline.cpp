@@ -1032,35 +1032,28 @@ static View view_of(mmsb_ctx* c, int what) {
   }
 }
 
-// Device -> host copy of a (possibly strided or transposed) quantity: compacted on the device
-// into one contiguous staging buffer, then a single linear copy (strided 2D copies of ~1e6 short
-// rows into pageable memory are orders of magnitude slower).
-static int stage_compact(mmsb_ctx* c, const View& v, void** staged, size_t* bytes) {
+// Device -> host copy of a (possibly strided or transposed) quantity: compacted on the device into a
+// contiguous staging buffer, then linear copies (strided 2D copies of ~1e6 short rows into pageable
+// memory are orders of magnitude slower). The staging goes in slabs of at most 256 MiB, so that a
+// copy of a tall matrix does not need a second full-size device allocation.
+static int copy_raw(mmsb_ctx* c, const View& v, void* host) {
   const size_t esz = v.dtype == 1 ? 8 : 4;
   const int64_t outer = v.trans ? v.cols : v.rows;
   const int64_t inner = v.trans ? v.rows : v.cols;
-  *bytes = (size_t)outer * inner * esz;
-  if (v.ld == inner) {
-    *staged = nullptr;
-    return MMSB_OK;
-  }
+  const size_t bytes = (size_t)outer * inner * esz;
+  if (v.ld == inner) return d2h(c->s, host, v.p, bytes);
+  const int64_t slab = std::max<int64_t>(1, std::min<int64_t>(outer, ((int64_t)256 << 20) / std::max<int64_t>(1, inner * esz)));
   void* d = nullptr;
-  HCHK(hipMalloc(&d, std::max<size_t>(*bytes, 1)));
-  hipError_t e = hipMemcpy2DAsync(d, inner * esz, v.p, v.ld * esz, inner * esz, outer, hipMemcpyDeviceToDevice, c->s);
-  if (e != hipSuccess) {
-    (void)hipFree(d);
-    return set_error(MMSB_ERR_HIP, "copy_out: %s", hipGetErrorString(e));
+  HCHK(hipMalloc(&d, std::max<size_t>((size_t)slab * inner * esz, 1)));
+  int st = MMSB_OK;
+  for (int64_t o = 0; o < outer && st == MMSB_OK; o += slab) {
+    const int64_t m = std::min<int64_t>(slab, outer - o);
+    hipError_t e = hipMemcpy2DAsync(d, inner * esz, static_cast<const unsigned char*>(v.p) + (size_t)o * v.ld * esz,
+                                    v.ld * esz, inner * esz, m, hipMemcpyDeviceToDevice, c->s);
+    if (e != hipSuccess) st = set_error(MMSB_ERR_HIP, "copy_out: %s", hipGetErrorString(e));
+    if (st == MMSB_OK) st = d2h(c->s, static_cast<unsigned char*>(host) + (size_t)o * inner * esz, d, (size_t)m * inner * esz);
   }
-  *staged = d;
-  return MMSB_OK;
-}
-
-static int copy_raw(mmsb_ctx* c, const View& v, void* host) {
-  void* staged = nullptr;
-  size_t bytes = 0;
-  TRY(stage_compact(c, v, &staged, &bytes));
-  const int st = d2h(c->s, host, staged ? staged : v.p, bytes);
-  if (staged) (void)hipFree(staged);
+  (void)hipFree(d);
   return st;
 }
 

@@ -137,6 +137,7 @@ typedef struct mmsb_gen_params {
   double pareto_shape;
   int32_t mode;              /* MMSB_GEN_*: exact pairwise Bernoulli or ball dropping  */
   int32_t threads;           /* 0 = hardware concurrency (capped at 16)               */
+  int32_t verbose;           /* progress lines on stdout, one per stage               */
 } mmsb_gen_params;
 
 /* ---- partition files (read_G, pre_proc.c:49-80; read_G_sparse, CPU IOfn.cpp:21-54) ---------- */
