@@ -454,11 +454,13 @@ def bench_stress(m, args):
         dt = time.time() - ts
         rates[prec] = (n * args.steps / dt, 1e3 * dt / args.steps)
         log(f"STGD {prec}: {rates[prec][0]:.3e} nodes/s, {rates[prec][1]:.1f} ms/epoch")
-    ph_ms = p.stgd_timed(64, "bf16")
-    phases_us = {nm: 1e3 * t / 64 for nm, t in zip(("fill_and_p1", "p2", "renorm"), ph_ms)}
-    log(f"phases (us/step, bf16): {phases_us}")
-
     # ---- accuracy on a prefix of X (S steps) and a 10k-node sample, against the FP64 oracle --------
+    # (every run starts at an epoch boundary, i.e. at the first minibatch of X, as the oracle does)
+    def to_epoch_boundary():
+        done = int(p.get("frob").size)
+        if done % spe:
+            p.stgd(spe - done % spe, "bf16")
+    to_epoch_boundary()
     S = args.stress_steps
     rows = S * b
     ya, yb, yc = (p.get_rows(nm, 0, rows) for nm in ("YA", "YB", "YC"))
@@ -475,7 +477,8 @@ def bench_stress(m, args):
     acc = {"steps": S, "nodes": rows, "eta": eta, "sample_rows": [sample0, sample0 + ns_],
            "reference": "oracle FP64 STGD (numpy) over the first S minibatches of X on the same whitened rows; "
                         "Pi_X = y_A Phi Lambda' on the sample",
-           "phi_rel_err": {}, "pi_x_rel_err": {}, "pin_x_rel_err": {}, "oracle_phi_moved": rel(phi64, phi0)}
+           "phi_rel_err": {}, "update_rel_err": {}, "pi_x_rel_err": {}, "pin_x_rel_err": {},
+           "oracle_phi_moved": rel(phi64, phi0)}
     for prec in ("fp32", "bf16"):
         p.set_phi(phi0)
         p.stgd(S, prec)
@@ -484,10 +487,16 @@ def bench_stress(m, args):
         p.post()
         pi = p.get_rows("pi_x", sample0, ns_)
         pin = p.get_rows("pin_x", sample0, ns_)
+        to_epoch_boundary()
+        acc["update_rel_err"][prec] = rel(phi - phi0, phi64 - phi0)   # error relative to how far Phi moved
         acc["phi_rel_err"][prec] = rel(phi, phi64)
         acc["pi_x_rel_err"][prec] = rel(pi, pi64)
         acc["pin_x_rel_err"][prec] = rel(pin, pin64)
     log(f"accuracy {acc}")
+    ph_ms = p.stgd_timed(64, "bf16")
+    # the first interval holds the minibatch's fill (its rows rebuilt from CSR) and P1
+    phases_us = {nm: 1e3 * t / 64 for nm, t in zip(("fill_and_p1", "p2", "renorm"), ph_ms)}
+    log(f"phases (us/step, bf16): {phases_us}")
     p.close()
     spmm = spmm_roofline(m, xa, r=k + 10)
     log(f"spmm {spmm}")
