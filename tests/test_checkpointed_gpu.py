@@ -82,3 +82,42 @@ def test_recompute_mode_state_rules(mmsb):
     assert np.array_equal(p.get("V"), res.get("V"))
     for q in (res, win, p):
         q.close()
+
+
+def test_weighted_blocks_match_oracle_and_recompute(mmsb):
+    """EXPECTED-style weighted blocks (a value per nonzero, pre_proc.c:64-65): whitening and the
+    neighbour vectors against the FP64 oracle with the values, and the recompute form's weighted
+    gather equal to the resident vectors through BF16 STGD."""
+    from oracle import core
+    n, k = 800, 6
+    g = mmsb.generate(n, n, n, n, k, 1.0, 0.6, 0.02, seed=4)
+    rng = np.random.default_rng(4)
+    for nm in ("XA", "XB", "XC", "BA", "CA"):
+        rp, ci, _, nr, nc = g[nm]
+        g[nm] = (rp, ci, rng.uniform(0.5, 2.0, ci.size).astype(np.float32), nr, nc)
+    bl = {nm: core.Csr(g[nm][0], g[nm][1], g[nm][3], g[nm][4], g[nm][2].astype(np.float64))
+          for nm in ("XA", "XB", "XC")}
+    wcfg = core.WhitenConfig(k=k, alpha0=1.0, oversample=10, power_iters=1, seed=5)
+    ref = core.whiten(bl["XA"], bl["XB"], bl["XC"], wcfg)
+    ys = core.yvectors(bl["XA"], bl["XB"], bl["XC"], ref)
+    ps = []
+    for mode in ("stored", "recompute"):
+        cfg = mmsb.config(nx=n, na=n, nb=n, nc=n, ncom=k, alpha_0=1.0, oversample=10, power_iters=1, seed=5,
+                          minibatch=128, binary=0, y_mode=mode)
+        p = mmsb.Pipeline(cfg)
+        p.set_graph(g)
+        p.whiten()
+        ps.append(p)
+    res, win = ps
+
+    def rel_err(a, b):
+        return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
+    assert rel_err(res.get("W"), ref["W"]) < 1e-4
+    for nm, key in (("YA", "Y_A"), ("YB", "Y_B"), ("YC", "Y_C")):
+        assert rel_err(res.get(nm), ys[key]) < 1e-4, nm
+        assert np.array_equal(res.get(nm), win.get(nm)), nm
+    for p in ps:
+        p.stgd(-1, "bf16")
+    assert np.array_equal(res.get("phi"), win.get("phi"))
+    for p in ps:
+        p.close()
