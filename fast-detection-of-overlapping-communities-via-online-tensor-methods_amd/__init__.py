@@ -141,6 +141,8 @@ SIGNATURES: dict = {
     "mmsb_stgd_grad_f32": (ctypes.c_int, [_vp, ctypes.c_int32, ctypes.c_int32, _vp, _vp, _vp, ctypes.c_int32, _vp, _vp, _vp,
                                           _vp, _vp, ctypes.c_double, _vp, _vp, _vp]),
     "mmsb_power_method": (ctypes.c_int, [_vp, ctypes.c_int32, ctypes.c_int32, _vp]),
+    "mmsb_request_stop": (ctypes.c_int, [_vp]),
+    "stpm_force_exit": (None, []),
     "mmsb_grad": (ctypes.c_int, [_vp, ctypes.c_int64, ctypes.c_int32, _vp, ctypes.c_int32, ctypes.c_double, ctypes.c_int32,
                                  _vp, _vp]),
     "pre_proc": (ctypes.c_int, []),
@@ -333,6 +335,10 @@ class Pipeline:
 
     def set_learn_rate(self, eta):
         check(lib.mmsb_set_learn_rate(self.ctx, float(eta)))
+
+    def request_stop(self):
+        """Ends a running (or the next) stgd() at its next epoch boundary (thread-safe)."""
+        check(lib.mmsb_request_stop(self.ctx))
 
     def stgd_timed(self, steps, precision="fp32"):
         """Runs `steps` direct STGD steps with HIP events around each phase; returns the summed

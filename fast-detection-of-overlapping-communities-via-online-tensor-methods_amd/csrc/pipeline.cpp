@@ -21,6 +21,7 @@
 #include <hip/hip_runtime_api.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstdarg>
@@ -116,6 +117,8 @@ struct mmsb_ctx {
   float *phiT = nullptr, *phitT = nullptr, *phiaT = nullptr;
   bool y16_valid = false;
   bool have_phi = false;
+  // stop request (forceExit_flag, stpm.c:61,754): set from any thread, honoured between epochs
+  std::atomic<int> stop_req{0};
   // eigen / post
   double *lam = nullptr, *lamp = nullptr, *ahat = nullptr, *phil = nullptr, *LBU = nullptr;
   float *Upost = nullptr, *ZBtU = nullptr;
@@ -681,8 +684,12 @@ static int run_stgd_steps(mmsb_ctx* c, int64_t nsteps, int32_t precision, StgdTi
       TRY(launch(step0, ns, c->frob + step0, c->cur, timing));
       done += ns;
     }
+    if (done < nsteps) {  // a stop request ends the run at this epoch boundary
+      HCHK(hipStreamSynchronize(c->s));
+      if (c->stop_req.load()) break;
+    }
   }
-  c->steps_done += nsteps;
+  c->steps_done += done;
   return MMSB_OK;
 }
 
@@ -1351,6 +1358,13 @@ extern "C" int mmsb_stgd(mmsb_ctx* c, int64_t steps, int32_t precision) {
     for (int32_t e = 0; e < cf.epochs; ++e) {
       const bool low = precision == MMSB_PREC_BF16 && e < cf.epochs - cf.polish_epochs;
       TRY(run_stgd_steps(c, spe, low ? MMSB_PREC_BF16 : MMSB_PREC_FP32));
+      if (e + 1 < cf.epochs) {  // stpm.c:754: a stop request ends the loop like convergence
+        HCHK(hipStreamSynchronize(c->s));
+        if (c->stop_req.load()) {
+          c->log("Exiting STPM loop now\n");
+          break;
+        }
+      }
       if (cf.sgd_iter_max > 0 && c->steps_done >= cf.sgd_iter_max) break;          // stpm.c:761
       if (c->steps_done >= min_steps) {
         double last = 0.0;
@@ -1361,7 +1375,15 @@ extern "C" int mmsb_stgd(mmsb_ctx* c, int64_t steps, int32_t precision) {
     }
   }
   HCHK(hipStreamSynchronize(c->s));
+  c->stop_req.store(0);
   c->times[1] += now_s() - t0;
+  return MMSB_OK;
+}
+
+extern "C" int mmsb_request_stop(mmsb_ctx* c) {
+  clear_error();
+  MMSB_CHECK_ARG(c, "request_stop: null context");
+  c->stop_req.store(1);
   return MMSB_OK;
 }
 
@@ -1714,6 +1736,11 @@ extern "C" int post_proc(void) {
   if (!g_ctx) return set_error(MMSB_ERR_STATE, "post_proc before pre_proc");
   TRY(mmsb_post(g_ctx));
   return mmsb_write_outputs(g_ctx);
+}
+
+extern "C" void stpm_force_exit(void) {
+  mmsb_ctx* c = g_ctx;
+  if (c) c->stop_req.store(1);
 }
 
 extern "C" void* GPU_Task(void*) {

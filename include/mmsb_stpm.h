@@ -272,6 +272,12 @@ int mmsb_stgd(mmsb_ctx* ctx, int64_t steps, int32_t precision);
 /* Changes LEARN_RATE (stpm.h:79, a compile-time constant in the reference) for the following
  * mmsb_stgd calls of this context, e.g. a different rate for polishing epochs. */
 int mmsb_set_learn_rate(mmsb_ctx* ctx, double learn_rate);
+/* Asks a running (or the next) mmsb_stgd of this context to stop at the next epoch boundary, as
+ * the reference's forceExit_flag (stpm.c:61, tested at stpm.c:754, set from the console at
+ * stpm.c:1198) ends the STGD loop: that call returns MMSB_OK with Phi after its last completed
+ * epoch (at least one), and eigenvalues / post_proc proceed as after convergence. Safe to call from
+ * another thread; the request is consumed when mmsb_stgd returns. */
+int mmsb_request_stop(mmsb_ctx* ctx);
 /* Measurement entry (no counterpart in the reference): `steps` STGD steps as mmsb_stgd, launched
  * directly (never from a captured graph) with HIP events on the context stream around each phase;
  * phase_ms[0..2] receive the summed device time of the P1 kernel (Y Phi contractions and
@@ -337,6 +343,9 @@ int pre_proc(void);
 void* GPU_Task(void* arg);
 /* post_proc.c:62: Pi_X..Pi_C and their files (reference return type void, declared int at stpm.c:57). */
 int post_proc(void);
+/* forceExit_flag = T_TRUE (stpm.c:1198): the STGD loop of a running GPU_Task ends at the next epoch
+ * boundary and the task goes on to the eigenvalues, post_proc and the output files. */
+void stpm_force_exit(void);
 
 #ifdef __cplusplus
 }

@@ -410,3 +410,38 @@ def test_c1_recovers_planted_communities(mmsb, prec):
           f"best correlations min {best.min():.3f} median {np.median(best):.3f}", flush=True)
     assert res["recovery_ratio"] == 1.0 and res["error"] < 0.2
     p.close()
+
+
+def test_stop_request_ends_stgd_at_an_epoch_boundary(mmsb):
+    """forceExit_flag (stpm.c:61,754,1198): a stop request ends the STGD epochs at an epoch boundary
+    and returns normally -- requested before the call (one epoch runs), and from another thread while
+    the epochs run. Phi is that of the same number of uninterrupted steps."""
+    import threading
+    n, k, b = 4096, 32, 256
+    spe = n // b
+    rng = np.random.default_rng(2)
+    ys = [rng.standard_normal((n, k)) for _ in range(3)]
+    mus = [0.1 * rng.standard_normal(k) for _ in range(3)]
+    phi0 = core.init_phi(3, k)
+    cfg = mmsb.config(nx=n, na=n, nb=n, nc=n, ncom=k, oversample=0, minibatch=b, epochs=5000, learn_rate=1e-4,
+                      sgd_iter_min=0, sgd_iter_max=0)
+    p = mmsb.Pipeline(cfg)
+    p.set_yvectors(*ys, *mus)
+    p.set_phi(phi0)
+    p.request_stop()
+    p.stgd(-1, "fp32")
+    assert p.get("frob").size == spe
+    timer = threading.Timer(0.05, p.request_stop)
+    timer.start()
+    p.stgd(-1, "fp32")
+    timer.join()
+    done = p.get("frob").size
+    print(f"stopped after {done // spe} epochs", flush=True)
+    assert done % spe == 0 and 2 * spe <= done < 5001 * spe
+    q = mmsb.Pipeline(cfg)
+    q.set_yvectors(*ys, *mus)
+    q.set_phi(phi0)
+    q.stgd(done, "fp32")
+    assert np.array_equal(p.get("phi"), q.get("phi"))
+    p.close()
+    q.close()
