@@ -45,3 +45,15 @@ def test_argument_errors_are_status_codes_not_exits(mmsb):
     st = mmsb.lib.mmsb_spmm_f32(None, 4, 4, 0, None, None, None, 1.0, None, 3, 3, None, 3)
     assert st == mmsb.MMSB_ERR_ARG
     assert "multiples of 4" in mmsb.last_error()
+
+
+def test_host_checks_of_stop_and_y_mode(mmsb):
+    """Host-side validation of the stop request and the neighbour-vector mode (no GPU call)."""
+    import ctypes
+    assert mmsb.lib.mmsb_request_stop(None) == mmsb.MMSB_ERR_ARG
+    mmsb.lib.stpm_force_exit()                      # no program-level context: nothing to stop
+    assert mmsb.config(y_mode="recompute").y_mode == 2 and mmsb.config().y_mode == 0
+    out = ctypes.c_void_p()
+    cfg = mmsb.config(nx=100, na=100, nb=100, nc=100, ncom=4, y_mode=7)
+    assert mmsb.lib.mmsb_create(ctypes.byref(cfg), ctypes.byref(out)) == mmsb.MMSB_ERR_ARG
+    assert "y_mode" in mmsb.last_error()
