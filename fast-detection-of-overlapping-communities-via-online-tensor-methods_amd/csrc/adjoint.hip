@@ -164,9 +164,9 @@ size_t power_workspace_bytes(int32_t k, int32_t chunk) {
 // Util.cpp:131-206): Phi_j <- T(I, phi_j, phi_j) over all n rows, lambda_j = ||.||, columns
 // normalised, then (optionally) the symmetric orthonormalisation Phi (Phi^T Phi)^-1/2.
 // Y_A, Y_B row-major [n x ldy]; Y_C given transposed [k x ldyc] (node-contiguous).
-int power_iteration(hipStream_t s, int32_t k, int64_t n, const float* ya, const float* yb, const float* yct, int32_t ldy,
-                    int64_t ldyc, const double* mua, const double* mub, const double* muc, double c3, double c1, double c0,
-                    double* phi, double* lam, int orthogonalize, int32_t chunk, void* work, float* yc_rows) {
+int power_iteration(hipStream_t s, int32_t k, int64_t n, const PowerRows& rows, int32_t ldy, const double* mua,
+                    const double* mub, const double* muc, double c3, double c1, double c0, double* phi, double* lam,
+                    int orthogonalize, int32_t chunk, void* work) {
   unsigned char* w = static_cast<unsigned char*>(work);
   auto take = [&](size_t bytes) { void* p = w; w += align256(bytes); return p; };
   const size_t ck = sizeof(double) * (size_t)chunk * k, kk = sizeof(double) * (size_t)k * k;
@@ -184,12 +184,13 @@ int power_iteration(hipStream_t s, int32_t k, int64_t n, const float* ya, const 
   MMSB_LAUNCH_CHECK();
   for (int64_t r0 = 0; r0 < n; r0 += chunk) {
     const int32_t nb = (int32_t)std::min<int64_t>(chunk, n - r0);
-    MMSB_TRY(transpose_f32(s, k, nb, yct + r0, ldyc, yc_rows, ldy));  // [k x nb] -> [nb x k]
-    MMSB_TRY((gemm_nn<float, double>(s, nb, k, k, 1.0, yb + r0 * ldy, ldy, phi, k, 0.0, B, k)));
-    MMSB_TRY((gemm_nn<float, double>(s, nb, k, k, 1.0, yc_rows, ldy, phi, k, 0.0, C, k)));
+    const float *ya = nullptr, *yb = nullptr, *yc = nullptr;
+    MMSB_TRY(rows(r0, nb, &ya, &yb, &yc));
+    MMSB_TRY((gemm_nn<float, double>(s, nb, k, k, 1.0, yb, ldy, phi, k, 0.0, B, k)));
+    MMSB_TRY((gemm_nn<float, double>(s, nb, k, k, 1.0, yc, ldy, phi, k, 0.0, C, k)));
     hipLaunchKernelGGL(pm_elem_kernel, dim3((unsigned)ceil_div(k, 64)), dim3(64), 0, s, nb, k, B, C, m, c3, c1, KA, colsum);
     MMSB_LAUNCH_CHECK();
-    MMSB_TRY((gemm_tn<float, double>(s, nb, k, k, 1.0, ya + r0 * ldy, ldy, KA, k, r0 == 0 ? 0.0 : 1.0, T, tn)));
+    MMSB_TRY((gemm_tn<float, double>(s, nb, k, k, 1.0, ya, ldy, KA, k, r0 == 0 ? 0.0 : 1.0, T, tn)));
   }
   hipLaunchKernelGGL(pm_finish_kernel, dim3(1), dim3(256), 0, s, k, 1.0 / (double)n, T, mua, m, colsum, c1, c0, phi, lam);
   MMSB_LAUNCH_CHECK();

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime_api.h>
 #include <cstddef>
 #include <cstdint>
+#include <functional>
 
 namespace mmsb {
 
@@ -88,8 +89,11 @@ int stgd_grad(hipStream_t s, int32_t k, int32_t nb, const TY* ya, const TY* yb, 
               double theta, double* grad, double* loss, void* work);
 
 size_t power_workspace_bytes(int32_t k, int32_t chunk);
-int power_iteration(hipStream_t s, int32_t k, int64_t n, const float* ya, const float* yb, const float* yct, int32_t ldy,
-                    int64_t ldyc, const double* mua, const double* mub, const double* muc, double c3, double c1, double c0,
-                    double* phi, double* lam, int orthogonalize, int32_t chunk, void* work, float* yc_rows);
+// rows(r0, nb, &ya, &yb, &yc) enqueues whatever makes rows [r0, r0 + nb) of Y_A, Y_B, Y_C available
+// row-major with leading dimension ldy, and points ya, yb, yc at them (valid until the next call).
+using PowerRows = std::function<int(int64_t r0, int32_t nb, const float** ya, const float** yb, const float** yc)>;
+int power_iteration(hipStream_t s, int32_t k, int64_t n, const PowerRows& rows, int32_t ldy, const double* mua,
+                    const double* mub, const double* muc, double c3, double c1, double c0, double* phi, double* lam,
+                    int orthogonalize, int32_t chunk, void* work);
 
 }  // namespace mmsb

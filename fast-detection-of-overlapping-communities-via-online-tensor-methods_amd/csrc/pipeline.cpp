@@ -1461,9 +1461,8 @@ extern "C" int mmsb_grad(mmsb_ctx* c, int64_t row0, int32_t nrows, const double*
 extern "C" int mmsb_power_method(mmsb_ctx* c, int32_t iters, int32_t orthogonalize, double* lambda) {
   clear_error();
   MMSB_CHECK_ARG(c && lambda && iters >= 0, "power_method: bad arguments");
-  if (!c->have_y)
-    return set_error(MMSB_ERR_STATE, c->y_window ? "power_method: needs resident neighbour vectors (y_mode recomputes them)"
-                                                 : "power_method: neighbour vectors not available");
+  if (!c->have_y && !(c->y_window && c->whitened))
+    return set_error(MMSB_ERR_STATE, "power_method: neighbour vectors not available");
   HCHK(hipSetDevice(c->cfg.device));
   TRY(alloc_stgd(c));
   if (!c->have_phi) TRY(init_phi(c));
@@ -1478,25 +1477,51 @@ extern "C" int mmsb_power_method(mmsb_ctx* c, int32_t iters, int32_t orthogonali
   for (int32_t m = 0; m < k; ++m)
     for (int32_t i = 0; i < k; ++i) hd[(size_t)m * k + i] = hf[(size_t)m * kp + i];
   double *dphi = nullptr, *dlam = nullptr;
-  float* ycr = nullptr;
+  float* yr[3] = {nullptr, nullptr, nullptr};  // a chunk of Y_C rows (resident form) or of all three (recompute form)
   void* work = nullptr;
   auto cleanup = [&]() {
-    for (void* p : {(void*)dphi, (void*)dlam, (void*)ycr, work})
+    for (void* p : {(void*)dphi, (void*)dlam, (void*)yr[0], (void*)yr[1], (void*)yr[2], work})
       if (p) (void)hipFree(p);
   };
   hipError_t e = hipMalloc(&dphi, sizeof(double) * k * k);
   if (e == hipSuccess) e = hipMalloc(&dlam, sizeof(double) * k);
-  if (e == hipSuccess) e = hipMalloc(&ycr, sizeof(float) * (size_t)chunk * kp);
+  for (int i = 0; i < (c->have_y ? 1 : 3); ++i)
+    if (e == hipSuccess) e = hipMalloc(&yr[i], sizeof(float) * (size_t)chunk * kp);
   if (e == hipSuccess) e = hipMalloc(&work, power_workspace_bytes(k, chunk));
   if (e == hipSuccess) e = hipMemcpyAsync(dphi, hd.data(), sizeof(double) * k * k, hipMemcpyHostToDevice, c->s);
   if (e != hipSuccess) {
     cleanup();
     return set_error(MMSB_ERR_NOMEM, "power_method: %s", hipGetErrorString(e));
   }
+  PowerRows rows;
+  if (c->have_y) {  // resident: Y_A, Y_B rows in place, Y_C^T columns transposed into a chunk
+    rows = [c, &yr](int64_t r0, int32_t nb, const float** ya, const float** yb, const float** yc) -> int {
+      TRY(transpose_f32(c->s, c->k, nb, c->YCt + r0, c->n_pad, yr[0], c->kp));  // [k x nb] -> [nb x k]
+      *ya = c->YA + r0 * c->kp;
+      *yb = c->YB + r0 * c->kp;
+      *yc = yr[0];
+      return MMSB_OK;
+    };
+  } else {  // recompute form: the chunk's rows rebuilt from CSR as STGD's minibatches are
+    rows = [c, &yr](int64_t r0, int32_t nb, const float** ya, const float** yb, const float** yc) -> int {
+      YRowsJobs jobs{};
+      const float* R[3] = {c->W, c->MB, c->MC};
+      for (int i = 0; i < 3; ++i) {
+        const DevCsr& g = c->blk[i];
+        jobs.j[i] = YRowsJob{g.rowptr + r0, g.colidx, g.vals, R[i], yr[i], c->kp, 0};
+      }
+      jobs.count = 3;
+      TRY(spmm_yrows(c->s, jobs, nb, c->k, c->kp, false));
+      *ya = yr[0];
+      *yb = yr[1];
+      *yc = yr[2];
+      return MMSB_OK;
+    };
+  }
   int st = MMSB_OK;
   for (int32_t it = 0; it < iters && st == MMSB_OK; ++it)
-    st = power_iteration(c->s, k, c->n[0], c->YA, c->YB, c->YCt, kp, c->n_pad, c->muw[0], c->muw[1], c->muw[2], co[0], co[1],
-                         co[2], dphi, dlam, orthogonalize, chunk, work, ycr);
+    st = power_iteration(c->s, k, c->n[0], rows, kp, c->muw[0], c->muw[1], c->muw[2], co[0], co[1], co[2], dphi, dlam,
+                         orthogonalize, chunk, work);
   if (st == MMSB_OK) {
     e = hipMemcpyAsync(hd.data(), dphi, sizeof(double) * k * k, hipMemcpyDeviceToHost, c->s);
     if (e == hipSuccess) e = hipMemcpyAsync(lambda, dlam, sizeof(double) * k, hipMemcpyDeviceToHost, c->s);

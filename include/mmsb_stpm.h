@@ -306,8 +306,8 @@ void mmsb_batch_destroy(mmsb_batch* batch);
  * instead of reading the stored neighbour vectors -- the memory-saving form for large n k. */
 int mmsb_grad(mmsb_ctx* ctx, int64_t row0, int32_t nrows, const double* phi, int32_t scaling, double theta,
               int32_t recompute, double* grad, double* loss);
-/* Batch implicit tensor power method (CPU tensorDecom_alpha0, Util.cpp:131-206) on the stored
- * neighbour vectors, as an alternative to STGD: `iters` iterations of phi_j <- T(I, phi_j, phi_j)
+/* Batch implicit tensor power method (CPU tensorDecom_alpha0, Util.cpp:131-206) on the neighbour
+ * vectors (resident, or rebuilt from CSR chunk by chunk in the recompute form), as an alternative to STGD: `iters` iterations of phi_j <- T(I, phi_j, phi_j)
  * over all of X with the paper coefficients, lambda_j = ||.||, column normalisation, and, when
  * orthogonalize != 0, the symmetric orthonormalisation Phi (Phi^T Phi)^-1/2 the reference lacks
  * (without it the columns collapse onto the dominant component). Starts from the current Phi

@@ -62,16 +62,17 @@ def test_recomputed_minibatches_match_resident(mmsb, k, n, b, prec):
 
 
 def test_recompute_mode_state_rules(mmsb):
-    """What the recompute form gives back after whitening (the sketches and V) and what it refuses:
-    batches and the power method need resident neighbour vectors."""
+    """What the recompute form gives back after whitening (the sketches and V), what it refuses
+    (batches need resident neighbour vectors), and the power method (CPU tensorDecom_alpha0,
+    Util.cpp:131-206) over recomputed row chunks, identical to the resident form."""
     res, win = build_pair(mmsb, 600, 8, 64)
     res.get("V")
     with pytest.raises(mmsb.MMSBError):
         win.get("V")
     with pytest.raises(mmsb.MMSBError):
-        win.power_method(2)
-    with pytest.raises(mmsb.MMSBError):
         mmsb.Batch([win], "fp32")
+    lam_r, lam_w = res.power_method(3), win.power_method(3)
+    assert np.array_equal(lam_r, lam_w) and np.array_equal(res.get("phi"), win.get("phi"))
     # y_mode "auto" keeps the resident form when it fits: V is still there
     g = mmsb.generate(600, 600, 600, 600, 8, 1.0, 0.5, 0.02, seed=0)
     cfg = mmsb.config(nx=600, na=600, nb=600, nc=600, ncom=8, oversample=10, power_iters=1, seed=1, minibatch=64,
