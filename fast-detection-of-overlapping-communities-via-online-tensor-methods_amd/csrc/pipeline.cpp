@@ -226,12 +226,20 @@ static int nn_wellcond(mmsb_ctx* c, int64_t n, int32_t p, int32_t q, const float
   return gemm_nn<float, float>(c->s, n, p, q, 1.0, X, ldx, C, ldc, 0.0, Y, ldy, c_trans);
 }
 
+// A basis that only feeds the next product of a subspace iteration needs to span the right space
+// with a well-conditioned basis, not to be orthonormal to working precision: one_pass stops after the
+// first CholeskyQR pass (Q = X R1^-1, orthogonality error ~kappa(X)^2 eps64); the bases that are used
+// as such (U, V, the M2 Rayleigh-Ritz basis, R for the SVD) take both passes.
 static int orth(mmsb_ctx* c, float* X, int64_t n, int32_t cols, int32_t ld, float* Q, float* tmp, double* R_out,
-                int status_slot) {
+                int status_slot, bool one_pass = false) {
   hipStream_t s = c->s;
   TRY((gemm_tn<float, float>(s, n, cols, cols, 1.0, X, ld, X, ld, 0.0, c->gram, c->work)));
   TRY(chol_inv(s, cols, c->gram, kCholShift, c->R1, c->R1i, c->dstat + status_slot));
   TRY((gemm_nn<float, float>(s, n, cols, cols, 1.0, X, ld, c->R1i, cols, 0.0, tmp, ld)));
+  if (one_pass && !R_out) {  // the tiled product cannot run in place (Q may alias X): copy back
+    HCHK(hipMemcpyAsync(Q, tmp, sizeof(float) * (size_t)n * ld, hipMemcpyDeviceToDevice, s));
+    return MMSB_OK;
+  }
   TRY((gemm_tn<float, float>(s, n, cols, cols, 1.0, tmp, ld, tmp, ld, 0.0, c->gram, c->work)));
   TRY(chol_inv(s, cols, c->gram, kCholShift, c->R2, c->R2i, c->dstat + status_slot + 1));
   TRY(nn_wellcond(c, n, cols, cols, tmp, ld, c->R2i, cols, Q, ld, false));  // R2 = I + O(kappa eps)
@@ -416,14 +424,14 @@ static int whiten_impl(mmsb_ctx* c) {
   TRY(gaussian_fill(s, cf.seed, 0, ncn, r, T[0], ld));
   TRY(spmm(c, XC, 1.f, T[0], ld, r, T[1], ld));
   TRY(spmm(c, XBt, inv, T[1], ld, r, T[2], ld));
-  TRY(orth(c, T[2], nb, r, ld, T[2], T[3], nullptr, 0));
+  TRY(orth(c, T[2], nb, r, ld, T[2], T[3], nullptr, 0, cf.power_iters > 0));
   for (int q = 0; q < cf.power_iters; ++q) {
     TRY(spmm(c, XB, 1.f, T[2], ld, r, T[1], ld));
     TRY(spmm(c, XCt, inv, T[1], ld, r, T[0], ld));
-    TRY(orth(c, T[0], ncn, r, ld, T[0], T[3], nullptr, 2));
+    TRY(orth(c, T[0], ncn, r, ld, T[0], T[3], nullptr, 2, true));
     TRY(spmm(c, XC, 1.f, T[0], ld, r, T[1], ld));
     TRY(spmm(c, XBt, inv, T[1], ld, r, T[2], ld));
-    TRY(orth(c, T[2], nb, r, ld, T[2], T[3], nullptr, 0));
+    TRY(orth(c, T[2], nb, r, ld, T[2], T[3], nullptr, 0, q + 1 < cf.power_iters));
   }
   // P_CB Q = Q2 R2  =>  P_BC ~ Q R2^T Q2^T
   TRY(spmm(c, XB, 1.f, T[2], ld, r, T[1], ld));
@@ -449,12 +457,12 @@ static int whiten_impl(mmsb_ctx* c) {
   c->log("whiten: M2 eigenpairs (n_A=%lld)\n", (long long)na);
   TRY(gaussian_fill(s, cf.seed, 1, na, r, T[0], ld));
   TRY(m2_apply(c, T[0], T[2]));
-  TRY(orth(c, T[2], na, r, ld, T[2], T[3], nullptr, 4));
+  TRY(orth(c, T[2], na, r, ld, T[2], T[3], nullptr, 4, cf.power_iters > 0));
   int qbuf = 2;
   for (int q = 0; q < cf.power_iters; ++q) {
     const int other = qbuf == 2 ? 0 : 2;
     TRY(m2_apply(c, T[qbuf], T[other]));
-    TRY(orth(c, T[other], na, r, ld, T[other], T[3], nullptr, 4));
+    TRY(orth(c, T[other], na, r, ld, T[other], T[3], nullptr, 4, q + 1 < cf.power_iters));
     qbuf = other;
   }
   TRY(m2_apply(c, T[qbuf], T[1]));
