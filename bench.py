@@ -126,7 +126,7 @@ def main():
     p.stgd(0, "fp32")                      # seeded orthonormal Phi0 ...
     phi0 = p.get("phi")                    # ... kept for the precision comparison below
     if args.traffic_probe:
-        return traffic_probe(p, args.precision)
+        return traffic_probe(m, p, args.precision, g["XA"], k + 10)
     t1 = time.time()
     p.stgd(-1, args.precision)
     t_st = time.time() - t1
@@ -194,6 +194,7 @@ def main():
 
     # ---- SpMM roofline on G_XA (n x n, nnz) times a tall n x r FP32 matrix ----------------------
     spmm = spmm_roofline(m, g["XA"], r=k + 10)
+    spmm["traffic"] = measured_spmm_traffic(args.workload, k + 10)
 
     out = {
         "metric": f"STGD nodes/s (n={n}/partition, k={k})",
@@ -238,21 +239,48 @@ def main():
     print(json.dumps(out), flush=True)
 
 
-def traffic_probe(p, prec, steps=16):
+def traffic_probe(m, p, prec, xa, r, steps=16):
     """Counter passes (rocprofv3 --pmc) see one dispatch per kernel here: `steps` STGD steps launched
-    directly, then three launches of a streaming negate over 2^26 floats (256 MiB read, 256 MiB
-    written), the calibration for FETCH_SIZE and WRITE_SIZE."""
+    directly, three SpMMs G_XA x (n x r) in the pipeline's row order, then three launches of a
+    streaming negate over 2^26 floats (256 MiB read, 256 MiB written), the calibration for
+    FETCH_SIZE and WRITE_SIZE."""
     import torch
     p.stgd_timed(steps, prec)
+    p.close()
+    rowptr, colidx, _, nr, nc = xa
+    dev = torch.device("cuda:0")
+    ld = (r + 3) // 4 * 4
+    d_rp, d_ci = torch.from_numpy(rowptr).to(dev), torch.from_numpy(colidx).to(dev)
+    B = torch.randn(nc, ld, device=dev, dtype=torch.float32)
+    Y = torch.empty(nr, ld, device=dev, dtype=torch.float32)
+    order = m.csr_row_order(d_rp)
+    for _ in range(3):
+        m.spmm_f32(d_rp, d_ci, None, B, Y, r=r, n=nc, order=order)
+    del B, Y
     src = torch.empty(1 << 26, dtype=torch.float32, device="cuda:0").uniform_()
     dst = torch.empty_like(src)
     for _ in range(3):
         torch.neg(src, out=dst)
     torch.cuda.synchronize()
-    log(f"traffic probe: {steps} direct {prec} STGD steps, 3 x negate of 256 MiB")
+    log(f"traffic probe: {steps} direct {prec} STGD steps, 3 SpMMs (r={r}), 3 x negate of 256 MiB")
 
 
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "stgd_c3_p1_traffic.md")
+SPMM_TRAFFIC_FILE = os.path.join(ROOT, "profiles", "spmm_c3_traffic.md")
+
+
+def measured_spmm_traffic(workload, r):
+    """HBM bytes per SpMM launch (G_XA x n x r) from the rocprofv3 --pmc summary of this workload, or
+    None where none covers it."""
+    import re
+    if workload != "c3" or not os.path.exists(SPMM_TRAFFIC_FILE):
+        return None
+    with open(SPMM_TRAFFIC_FILE) as f:
+        for line in f:
+            m = re.match(r"\s*spmm_traffic_bytes_per_launch\[(\d+)\]\s*=\s*(\d+)", line)
+            if m and int(m.group(1)) == r:
+                return int(m.group(2))
+    return None
 
 
 def measured_traffic(workload, prec):
