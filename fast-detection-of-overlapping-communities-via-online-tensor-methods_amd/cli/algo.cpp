@@ -5,6 +5,7 @@
 // Exit codes follow the reference: 1 for file errors, 2 for device / numerical errors.
 #include <pthread.h>
 
+#include <csignal>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -64,6 +65,14 @@ static int set_field(stpm_config& c, const std::string& key, const std::string& 
   return 0;
 }
 
+// Ctrl-C plays the part of the reference's console 'e' (forceExit_flag = T_TRUE, stpm.c:1198): STGD
+// stops at the next epoch boundary and the eigenvalues, post_proc and the output files follow. A
+// second Ctrl-C terminates the process.
+static void on_sigint(int) {
+  stpm_force_exit();
+  std::signal(SIGINT, SIG_DFL);
+}
+
 int main(int argc, char** argv) {
   if (argc < 2 || (std::strcmp(argv[1], "GPU_A") != 0 && std::strcmp(argv[1], "GPU_B") != 0)) {
     usage();
@@ -91,6 +100,7 @@ int main(int argc, char** argv) {
   }
   pthread_t th;
   void* ret = nullptr;
+  std::signal(SIGINT, on_sigint);
   if (pthread_create(&th, nullptr, GPU_Task, nullptr) != 0) return 2;
   pthread_join(th, &ret);
   if (ret) {
