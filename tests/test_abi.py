@@ -57,3 +57,15 @@ def test_host_checks_of_stop_and_y_mode(mmsb):
     cfg = mmsb.config(nx=100, na=100, nb=100, nc=100, ncom=4, y_mode=7)
     assert mmsb.lib.mmsb_create(ctypes.byref(cfg), ctypes.byref(out)) == mmsb.MMSB_ERR_ARG
     assert "y_mode" in mmsb.last_error()
+
+
+def test_cli_argument_errors(mmsb):
+    """./algo rejects a missing or unknown option and an unknown KEY=VALUE with exit status 1 and the
+    usage text, before touching a GPU (the reference's main prints its choices, stpm.c:1068-1118)."""
+    import os
+    import subprocess
+    exe = os.path.join(os.path.dirname(mmsb.__file__), "algo")
+    for args in ([], ["GPU_C"], ["GPU_A", "NOPE=1"], ["GPU_A", "NX"]):
+        r = subprocess.run([exe, *args], capture_output=True, text=True, timeout=60)
+        assert r.returncode == 1, args
+        assert "usage: ./algo" in r.stdout, args

@@ -445,3 +445,41 @@ def test_stop_request_ends_stgd_at_an_epoch_boundary(mmsb):
     assert np.array_equal(p.get("phi"), q.get("phi"))
     p.close()
     q.close()
+
+
+def test_cli_matches_the_api(mmsb, tmp_path):
+    """./algo GPU_A (the reference's main, stpm.c:1068-1218: pre_proc, GPU_Task on a pthread,
+    post_proc) on partition files writes the same Pi / evecs / evals files, byte for byte, as the
+    same configuration through the context API (frob_norm to its FP64 atomic summation order)."""
+    import os
+    import subprocess
+    from oracle import io as oio
+    n, k = 200, 3
+    g = mmsb.generate(n, n, n, n, k, 1.0, 0.5, 0.03, seed=4)
+    api, cli = tmp_path / "api", tmp_path / "cli"
+    for d in (api, cli):
+        d.mkdir()
+        for nm, stem in (("XA", "Gx_a"), ("XB", "Gx_b"), ("XC", "Gx_c"), ("BA", "Gb_a"), ("CA", "Gc_a")):
+            oio.write_coo(str(d / f"DBLP_{stem}.txt"), g[nm][0], g[nm][1])
+    cfg = mmsb.config(nx=n, na=n, nb=n, nc=n, ncom=k, folder_name=str(api) + "/", minibatch=64, epochs=2,
+                      alpha_0_str="1.0", learn_rate_str="1e-3")
+    p = mmsb.Pipeline(cfg)
+    p.read_inputs()
+    p.whiten()
+    p.stgd(-1)
+    p.eigen()
+    p.post()
+    p.write_outputs()
+    p.close()
+    exe = os.path.join(os.path.dirname(mmsb.__file__), "algo")
+    r = subprocess.run([exe, "GPU_A", f"NX={n}", f"NA={n}", f"NB={n}", f"NC={n}", f"NCOM={k}", f"FOLDER_NAME={cli}/",
+                        "MINIBATCH=64", "EPOCHS=2", "ALPHA_0=1.0", "LEARN_RATE=1e-3"],
+                       capture_output=True, text=True, timeout=110)
+    assert r.returncode == 0, r.stderr
+    assert "Task completed." in r.stdout
+    for stem in ("Pi_X", "Pi_A", "Pi_B", "Pi_C", "evecs", "evals"):
+        name = f"{stem}_{k}_1.0_1e-3.txt"
+        assert (cli / name).read_bytes() == (api / name).read_bytes(), stem
+    name = f"frob_norm_{k}_1.0_1e-3.txt"
+    np.testing.assert_allclose(np.loadtxt(cli / name), np.loadtxt(api / name), rtol=1e-12)
+    assert (cli / f"sim_info_{k}_1.0_1e-3.txt").exists()
