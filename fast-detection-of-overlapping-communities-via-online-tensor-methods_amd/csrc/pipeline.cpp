@@ -713,6 +713,7 @@ struct mmsb_batch {
   hipStream_t s = nullptr;
   int device = 0;
   bool slots = false;          // true: grouped per-slot launches replayed from a graph per parity
+  bool coop_refused = false;   // a cooperative launch was refused: grouped launches from then on
   StgdBatch* plan[2] = {nullptr, nullptr};
   hipGraphExec_t graph[2] = {nullptr, nullptr};
   std::vector<int> cur0;       // members' cur at even batch epochs
@@ -731,9 +732,17 @@ static void batch_drop_graphs(mmsb_batch* b) {
     }
 }
 
-// One epoch of plan p, by the batch's current mode.
+// One epoch of plan p, by the batch's current mode. A refused cooperative launch (another process
+// on the GPU can hold the global-wave-sync resource it needs) enqueues nothing, so the epoch then
+// runs as grouped launches, and so do the later ones.
 static int batch_epoch(mmsb_batch* b, int p) {
-  if (!b->slots) return stgd_batch_launch(b->s, b->plan[p]);
+  if (!b->slots) {
+    if (stgd_batch_launch(b->s, b->plan[p]) == MMSB_OK) return MMSB_OK;
+    (void)hipGetLastError();
+    clear_error();
+    b->coop_refused = true;
+    b->slots = true;
+  }
   if (!b->graph[p]) {
     hipGraph_t g = nullptr;
     HCHK(hipStreamBeginCapture(b->s, hipStreamCaptureModeThreadLocal));
@@ -906,6 +915,7 @@ extern "C" int mmsb_batch_autotune(mmsb_batch* b, int32_t* best_grid, double* be
   int st = MMSB_OK;
   const int p = (int)(b->epochs_done & 1);
   for (int32_t g : cands) {
+    if (g > 0 && b->coop_refused) continue;  // persistent candidates cannot launch here
     b->slots = g == 0;
     if (g > 0) stgd_batch_set_grid(plan, g);
     for (int rep = 0; rep < 2 && st == MMSB_OK; ++rep) {
@@ -916,6 +926,7 @@ extern "C" int mmsb_batch_autotune(mmsb_batch* b, int32_t* best_grid, double* be
       if (st == MMSB_OK && hipEventRecord(e1, b->s) != hipSuccess) st = set_error(MMSB_ERR_HIP, "batch_autotune: event record failed");
       if (st == MMSB_OK && hipEventSynchronize(e1) != hipSuccess) st = set_error(MMSB_ERR_HIP, "batch_autotune: sync failed");
       if (st == MMSB_OK && hipEventElapsedTime(&ms, e0, e1) != hipSuccess) st = set_error(MMSB_ERR_HIP, "batch_autotune: timing failed");
+      if (g > 0 && b->coop_refused) break;        // timed the grouped fallback, not this candidate
       if (st == MMSB_OK && rep == 1 && ms < bms) {  // the first repetition of a mode may include graph set-up
         bms = ms;
         best = g;
@@ -932,6 +943,7 @@ extern "C" int mmsb_batch_autotune(mmsb_batch* b, int32_t* best_grid, double* be
   TRY(st);
   TRY(restore());
   HCHK(hipStreamSynchronize(b->s));
+  if (b->coop_refused) best = 0;
   b->slots = best == 0;
   for (StgdBatch* q : b->plan)
     if (best > 0) stgd_batch_set_grid(q, best);

@@ -483,3 +483,57 @@ def test_cli_matches_the_api(mmsb, tmp_path):
     name = f"frob_norm_{k}_1.0_1e-3.txt"
     np.testing.assert_allclose(np.loadtxt(cli / name), np.loadtxt(api / name), rtol=1e-12)
     assert (cli / f"sim_info_{k}_1.0_1e-3.txt").exists()
+
+
+_CHILD_BATCH = r'''
+import sys
+import numpy as np
+sys.path.insert(0, sys.argv[1])
+import __graft_entry__ as ge
+from oracle import core
+m = ge.load_package()
+specs = [(700, 8, 64, 0.7, 2e-3), (1300, 24, 128, 1.5, 1e-3)]
+singles, members = [], []
+for idx, (n, k, b, a0, eta) in enumerate(specs):
+    rng = np.random.default_rng(idx)
+    ys = [rng.standard_normal((n, k)) for _ in range(3)]
+    mus = [0.2 * rng.standard_normal(k) for _ in range(3)]
+    for lst in (singles, members):
+        p = m.Pipeline(m.config(nx=n, na=n, nb=n, nc=n, ncom=k, alpha_0=a0, oversample=0, minibatch=b, learn_rate=eta))
+        p.set_yvectors(*ys, *mus)
+        p.set_phi(core.init_phi(idx + 1, k))
+        lst.append(p)
+batch = m.Batch(members, "bf16")
+batch.run(1)
+grid, ms = batch.autotune()
+batch.run(1)
+for (n, k, b, a0, eta), ps, pb in zip(specs, singles, members):
+    ps.stgd(2 * -(-n // b), "bf16")
+    assert np.array_equal(ps.get("phi"), pb.get("phi")), n
+print("child batch ok, autotuned grid", grid, flush=True)
+'''
+
+
+def test_batch_in_a_second_process(mmsb, tmp_path):
+    """A batch in a second process on the same GPU, after this process has run a persistent
+    (cooperative) batch launch: the child's cooperative launch may be refused (only one process can
+    hold the global-wave-sync resource); its epochs then run as grouped launches, with the same Phi
+    as single instances."""
+    import os
+    import subprocess
+    import sys
+    n, k = 640, 8
+    rng = np.random.default_rng(5)
+    p = mmsb.Pipeline(mmsb.config(nx=n, na=n, nb=n, nc=n, ncom=k, oversample=0, minibatch=64))
+    p.set_yvectors(*[rng.standard_normal((n, k)) for _ in range(3)], *[np.zeros(k)] * 3)
+    batch = mmsb.Batch([p], "fp32")
+    batch.run(1)                                   # persistent launch in this process
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    script = tmp_path / "child_batch.py"
+    script.write_text(_CHILD_BATCH)
+    r = subprocess.run([sys.executable, str(script), root], capture_output=True, text=True, timeout=110, cwd=root)
+    print(r.stdout.strip(), flush=True)
+    assert r.returncode == 0, r.stderr[-2000:]
+    assert "child batch ok" in r.stdout
+    batch.close()
+    p.close()
