@@ -409,13 +409,20 @@ def spmm_roofline(m, blk, r):
         torch.cuda.synchronize()
         return ev0.elapsed_time(ev1) / reps
     ms, ms_natural = timed(order), timed(None)
+    # the same product with the CSR left in host memory and streamed in row chunks (PCIe included)
+    m.spmm_f32_host_csr(rowptr, colidx, None, B, Y, r=r, n=nc)      # first call sizes the staging
+    t0 = time.time()
+    for _ in range(3):
+        m.spmm_f32_host_csr(rowptr, colidx, None, B, Y, r=r, n=nc)
+    ms_host_csr = (time.time() - t0) * 1e3 / 3
     nnz = colidx.size
     bytes_alg = 4 * (nr + 1) + 4 * nnz + 4 * r * nnz + 4 * r * nr   # binary CSR: no value array
     gbs = bytes_alg / (ms * 1e-3) / 1e9
     return {"bound": "hbm", "achieved": gbs, "peak": PEAK["hbm"] / 1e9, "unit": "GB/s", "frac": gbs * 1e9 / PEAK["hbm"],
             "ms": ms, "rows": nr, "nnz": int(nnz), "r": r, "bytes_per_launch": bytes_alg,
             "row_order": "longest rows first (mmsb_csr_row_order), as in the pipeline",
-            "ms_natural_row_order": ms_natural}
+            "ms_natural_row_order": ms_natural,
+            "ms_csr_streamed_from_host": ms_host_csr}
 
 
 def bench_stress(m, args):

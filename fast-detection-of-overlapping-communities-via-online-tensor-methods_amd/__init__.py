@@ -100,6 +100,9 @@ SIGNATURES: dict = {
     "mmsb_spmm_f32_ordered": (ctypes.c_int, [_vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int64, _vp, _vp, _vp,
                                              ctypes.c_float, _vp, ctypes.c_int32, ctypes.c_int32, _vp, ctypes.c_int32,
                                              _vp]),
+    "mmsb_spmm_f32_host_csr": (ctypes.c_int, [_vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int64, _vp, _vp, _vp,
+                                              ctypes.c_float, _vp, ctypes.c_int32, ctypes.c_int32, _vp, ctypes.c_int32,
+                                              ctypes.c_int64]),
     "mmsb_csr_row_order": (ctypes.c_int, [_vp, ctypes.c_int32, _vp, _vp]),
     "mmsb_gen_params_default": (None, [ctypes.POINTER(GenParamsC)]),
     "mmsb_generate": (ctypes.c_int, [ctypes.POINTER(GenParamsC), ctypes.c_int32, ctypes.POINTER(ctypes.POINTER(GraphC))]),
@@ -199,6 +202,20 @@ def spmm_f32(rowptr, colidx, vals, B, Y, r: int, n: int, alpha: float = 1.0, str
     args = (stream_handle(stream), m, n, colidx.numel(), ptr(rowptr), ptr(colidx), ptr(vals), float(alpha), ptr(B),
             B.stride(0), r, ptr(Y), Y.stride(0))
     check(lib.mmsb_spmm_f32(*args) if order is None else lib.mmsb_spmm_f32_ordered(*args, ptr(order)))
+    return Y
+
+
+def spmm_f32_host_csr(rowptr, colidx, vals, B, Y, r: int, n: int, alpha: float = 1.0, chunk_nnz: int = 1 << 22,
+                      stream=None):
+    """Y[:m, :r] = alpha * G @ B[:, :r] through mmsb_spmm_f32_host_csr: G as host numpy arrays (int32
+    rowptr / colidx, float32 vals or None), B and Y device tensors; returns when Y is complete."""
+    import numpy as np
+    rp = np.ascontiguousarray(rowptr, np.int32)
+    ci = np.ascontiguousarray(colidx, np.int32)
+    v = None if vals is None else np.ascontiguousarray(vals, np.float32)
+    check(lib.mmsb_spmm_f32_host_csr(stream_handle(stream), rp.size - 1, n, ci.size, rp.ctypes.data, ci.ctypes.data,
+                                     None if v is None else v.ctypes.data, float(alpha), ptr(B), B.stride(0), r,
+                                     ptr(Y), Y.stride(0), int(chunk_nnz)))
     return Y
 
 

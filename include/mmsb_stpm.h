@@ -59,6 +59,15 @@ const char* mmsb_version(void);
 int mmsb_spmm_f32(void* stream, int32_t m, int32_t n, int64_t nnz, const int32_t* rowptr,
                   const int32_t* colidx, const float* vals, float alpha, const float* B,
                   int32_t ldb, int32_t r, float* Y, int32_t ldy);
+/* The same product with the CSR block in host memory (rowptr, colidx, vals host pointers; B, Y on
+ * the device): rows are cut into chunks of at most chunk_nnz nonzeros, staged through pinned memory
+ * and copied on a copy stream while the previous chunk is multiplied on `stream` -- the blocking
+ * idea of culaBlockDgemm (gemm_pinv_nys.c:27-145) applied to the sparse operand, for blocks not
+ * kept in HBM. Y is identical to mmsb_spmm_f32's. The CSR is validated on the host (monotone
+ * rowptr, column indices below n). Returns when Y is complete; calls are serialised. */
+int mmsb_spmm_f32_host_csr(void* stream, int32_t m, int32_t n, int64_t nnz, const int32_t* rowptr,
+                           const int32_t* colidx, const float* vals, float alpha, const float* B,
+                           int32_t ldb, int32_t r, float* Y, int32_t ldy, int64_t chunk_nnz);
 /* The same product with the rows visited in the order row_order[0..m) (a permutation of the rows;
  * Y is identical). No counterpart in the reference: with rows sorted by descending length
  * (mmsb_csr_row_order) the longest rows start first instead of forming the kernel's tail. */

@@ -101,3 +101,36 @@ def test_row_order_and_ordered_spmm(mmsb, dev, r):
     assert torch.equal(Y0[:, :r], Y1[:, :r])
     np.testing.assert_allclose(Y1[:, :r].cpu().numpy(), csr_spmm(rowptr, colidx, None, B[:, :r].astype(np.float64), m),
                                rtol=1e-4, atol=1e-4)
+
+
+@pytest.mark.parametrize("chunk_nnz", [1, 50, 700, 1 << 22])
+@pytest.mark.parametrize("weighted", [False, True])
+def test_spmm_with_host_csr_matches_device(mmsb, dev, chunk_nnz, weighted):
+    """The CSR block in host memory, streamed in row chunks (mmsb_spmm_f32_host_csr): Y identical to
+    the device-resident product, for chunks of one nonzero (every row its own chunk, rows longer than
+    the chunk included), a few hundred, and the whole block; empty rows at both ends."""
+    import torch
+    rng = np.random.default_rng(chunk_nnz + weighted)
+    m, n, r = 1500, 900, 130
+    rowptr, colidx = random_csr(rng, m, n, 0.05, empty_rows=(0, 1, 700, 1499))
+    vals = rng.uniform(0.5, 2.0, colidx.size).astype(np.float32) if weighted else None
+    B = torch.from_numpy(rng.standard_normal((n, 132)).astype(np.float32)).to(dev)
+    want = torch.full((m, 132), 3.0, dtype=torch.float32, device=dev)
+    got = torch.full((m, 132), 3.0, dtype=torch.float32, device=dev)
+    t = lambda a: torch.from_numpy(a).to(dev)
+    mmsb.spmm_f32(t(rowptr), t(colidx), None if vals is None else t(vals), B, want, r=r, n=n, alpha=0.25)
+    mmsb.spmm_f32_host_csr(rowptr, colidx, vals, B, got, r=r, n=n, alpha=0.25, chunk_nnz=chunk_nnz)
+    torch.cuda.synchronize()
+    assert torch.equal(got, want)
+
+
+def test_spmm_host_csr_rejects_bad_blocks(mmsb, dev):
+    """Host-side checks before any upload: a column index out of range and a decreasing rowptr."""
+    import torch
+    B = torch.zeros((10, 4), dtype=torch.float32, device=dev)
+    Y = torch.zeros((3, 4), dtype=torch.float32, device=dev)
+    rowptr = np.array([0, 1, 2, 3], np.int32)
+    with pytest.raises(mmsb.MMSBError, match="column index"):
+        mmsb.spmm_f32_host_csr(rowptr, np.array([0, 10, 2], np.int32), None, B, Y, r=4, n=10)
+    with pytest.raises(mmsb.MMSBError, match="decreases"):
+        mmsb.spmm_f32_host_csr(np.array([0, 2, 1, 3], np.int32), np.array([0, 1, 2], np.int32), None, B, Y, r=4, n=10)
