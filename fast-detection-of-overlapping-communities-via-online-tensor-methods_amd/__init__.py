@@ -205,7 +205,7 @@ def spmm_f32(rowptr, colidx, vals, B, Y, r: int, n: int, alpha: float = 1.0, str
     return Y
 
 
-def spmm_f32_host_csr(rowptr, colidx, vals, B, Y, r: int, n: int, alpha: float = 1.0, chunk_nnz: int = 1 << 22,
+def spmm_f32_host_csr(rowptr, colidx, vals, B, Y, r: int, n: int, alpha: float = 1.0, chunk_nnz: int = 1 << 21,
                       stream=None):
     """Y[:m, :r] = alpha * G @ B[:, :r] through mmsb_spmm_f32_host_csr: G as host numpy arrays (int32
     rowptr / colidx, float32 vals or None), B and Y device tensors; returns when Y is complete."""

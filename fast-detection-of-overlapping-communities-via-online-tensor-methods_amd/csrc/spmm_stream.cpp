@@ -6,7 +6,12 @@
 // are cut into chunks of at most `chunk_nnz` nonzeros, each chunk is rebased and staged in pinned
 // memory by the host, copied on a copy stream, and multiplied by the resident SpMM kernel on the
 // caller's stream. Two slots alternate, so the copy of chunk i + 1 overlaps the product of chunk i.
-// Each row is summed by the same kernel as with a device-resident block, so Y is identical.
+// When colidx (and vals) are already pinned host memory (hipHostMalloc), their chunks are copied
+// straight from there and only the rebased row pointers and the row order go through the staging, so
+// a block kept pinned between calls streams at the PCIe rate.
+// The host also orders each chunk's rows longest first (a counting sort of the row lengths), the
+// schedule the resident blocks use. Each row is summed by the same kernel as with a device-resident
+// block, so Y is identical.
 #include <hip/hip_runtime_api.h>
 
 #include <algorithm>
@@ -38,9 +43,10 @@ namespace {
 constexpr int64_t kChunkRows = (int64_t)1 << 20;  // rows per chunk at most (bounds the rowptr slot)
 
 struct Slot {
-  void* host = nullptr;         // pinned staging: rowptr, then colidx, then vals
+  void* host = nullptr;         // pinned staging: rowptr, row order, colidx, then vals
   size_t host_bytes = 0;
   int32_t* rowptr = nullptr;    // device
+  int32_t* order = nullptr;
   int32_t* colidx = nullptr;
   float* vals = nullptr;
   int64_t rows_cap = 0, nnz_cap = 0;
@@ -69,19 +75,20 @@ int ensure_slot(Slot& sl, int64_t rows, int64_t nnz, bool vals) {
   }
   if (rows > sl.rows_cap || nnz > sl.nnz_cap || (vals && !sl.vals)) {
     SHIP(hipEventSynchronize(sl.used));
-    if (sl.rowptr) (void)hipFree(sl.rowptr);
-    if (sl.colidx) (void)hipFree(sl.colidx);
-    if (sl.vals) (void)hipFree(sl.vals);
+    for (void* p : {(void*)sl.rowptr, (void*)sl.order, (void*)sl.colidx, (void*)sl.vals})
+      if (p) (void)hipFree(p);
     sl.rowptr = nullptr;
+    sl.order = nullptr;
     sl.colidx = nullptr;
     sl.vals = nullptr;
     sl.rows_cap = std::max(rows, sl.rows_cap);
     sl.nnz_cap = std::max(nnz, sl.nnz_cap);
     SHIP(hipMalloc(&sl.rowptr, sizeof(int32_t) * (size_t)(sl.rows_cap + 1)));
+    SHIP(hipMalloc(&sl.order, sizeof(int32_t) * (size_t)sl.rows_cap));
     SHIP(hipMalloc(&sl.colidx, sizeof(int32_t) * (size_t)std::max<int64_t>(sl.nnz_cap, 1)));
     if (vals) SHIP(hipMalloc(&sl.vals, sizeof(float) * (size_t)std::max<int64_t>(sl.nnz_cap, 1)));
   }
-  const size_t hb = sizeof(int32_t) * (size_t)(sl.rows_cap + 1) + sizeof(int32_t) * (size_t)sl.nnz_cap +
+  const size_t hb = sizeof(int32_t) * (size_t)(2 * sl.rows_cap + 1) + sizeof(int32_t) * (size_t)sl.nnz_cap +
                     (sl.vals ? sizeof(float) * (size_t)sl.nnz_cap : 0);
   if (hb > sl.host_bytes) {
     SHIP(hipEventSynchronize(sl.copied));
@@ -92,6 +99,19 @@ int ensure_slot(Slot& sl, int64_t rows, int64_t nnz, bool vals) {
     sl.host_bytes = hb;
   }
   return MMSB_OK;
+}
+
+// Pinned (page-locked, device-visible) host memory? A pageable pointer makes the query fail; the
+// failure is cleared so that it does not surface at a later launch check.
+bool is_pinned(const void* p) {
+  if (!p) return true;
+  hipPointerAttribute_t a{};
+  const hipError_t e = hipPointerGetAttributes(&a, p);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    return false;
+  }
+  return a.type == hipMemoryTypeHost;
 }
 
 void par_copy(void* dst, const void* src, size_t n) {
@@ -147,7 +167,7 @@ int spmm_f32_host_csr(hipStream_t s, int32_t m, int32_t n, int64_t nnz, const in
     for (Slot& sl : st.slot) {
       if (sl.used) (void)hipEventSynchronize(sl.used);
       if (sl.copied) (void)hipEventSynchronize(sl.copied);
-      for (void* p : {(void*)sl.rowptr, (void*)sl.colidx, (void*)sl.vals})
+      for (void* p : {(void*)sl.rowptr, (void*)sl.order, (void*)sl.colidx, (void*)sl.vals})
         if (p) (void)hipFree(p);
       if (sl.host) (void)hipHostFree(sl.host);
       if (sl.copied) (void)hipEventDestroy(sl.copied);
@@ -160,6 +180,7 @@ int spmm_f32_host_csr(hipStream_t s, int32_t m, int32_t n, int64_t nnz, const in
     st.device = dev;
   }
   for (Slot& sl : st.slot) STRY(ensure_slot(sl, max_rows, max_nnz, vals != nullptr));
+  const bool direct = is_pinned(colidx) && is_pinned(vals);
   for (size_t i = 0; i < chunks.size(); ++i) {
     Slot& sl = st.slot[i & 1];
     const int32_t r0 = chunks[i].first, r1 = chunks[i].second, rows = r1 - r0;
@@ -167,19 +188,35 @@ int spmm_f32_host_csr(hipStream_t s, int32_t m, int32_t n, int64_t nnz, const in
     // the slot's previous upload has left the staging, its previous product has read the device copy
     SHIP(hipEventSynchronize(sl.copied));
     int32_t* hrp = static_cast<int32_t*>(sl.host);
-    int32_t* hci = hrp + (sl.rows_cap + 1);
+    int32_t* hord = hrp + (sl.rows_cap + 1);
+    int32_t* hci = hord + sl.rows_cap;
     float* hv = reinterpret_cast<float*>(hci + sl.nnz_cap);
+    int32_t maxlen = 0;
     for (int32_t j = 0; j <= rows; ++j) hrp[j] = (int32_t)((int64_t)rowptr[r0 + j] - z0);
-    par_copy(hci, colidx + z0, sizeof(int32_t) * (size_t)cnnz);
-    if (vals) par_copy(hv, vals + z0, sizeof(float) * (size_t)cnnz);
+    for (int32_t j = 0; j < rows; ++j) maxlen = std::max(maxlen, hrp[j + 1] - hrp[j]);
+    {  // rows longest first, ties by index: a stable counting sort on the lengths
+      std::vector<int32_t> start((size_t)maxlen + 2, 0);
+      for (int32_t j = 0; j < rows; ++j) ++start[(size_t)(maxlen - (hrp[j + 1] - hrp[j])) + 1];
+      for (size_t b = 1; b < start.size(); ++b) start[b] += start[b - 1];
+      for (int32_t j = 0; j < rows; ++j) hord[start[(size_t)(maxlen - (hrp[j + 1] - hrp[j]))]++] = j;
+    }
+    const int32_t* src_ci = colidx + z0;
+    const float* src_v = vals ? vals + z0 : nullptr;
+    if (!direct) {
+      par_copy(hci, src_ci, sizeof(int32_t) * (size_t)cnnz);
+      if (vals) par_copy(hv, src_v, sizeof(float) * (size_t)cnnz);
+      src_ci = hci;
+      src_v = hv;
+    }
     SHIP(hipStreamWaitEvent(st.copy, sl.used, 0));
     SHIP(hipMemcpyAsync(sl.rowptr, hrp, sizeof(int32_t) * (size_t)(rows + 1), hipMemcpyHostToDevice, st.copy));
-    if (cnnz) SHIP(hipMemcpyAsync(sl.colidx, hci, sizeof(int32_t) * (size_t)cnnz, hipMemcpyHostToDevice, st.copy));
-    if (vals && cnnz) SHIP(hipMemcpyAsync(sl.vals, hv, sizeof(float) * (size_t)cnnz, hipMemcpyHostToDevice, st.copy));
+    SHIP(hipMemcpyAsync(sl.order, hord, sizeof(int32_t) * (size_t)rows, hipMemcpyHostToDevice, st.copy));
+    if (cnnz) SHIP(hipMemcpyAsync(sl.colidx, src_ci, sizeof(int32_t) * (size_t)cnnz, hipMemcpyHostToDevice, st.copy));
+    if (vals && cnnz) SHIP(hipMemcpyAsync(sl.vals, src_v, sizeof(float) * (size_t)cnnz, hipMemcpyHostToDevice, st.copy));
     SHIP(hipEventRecord(sl.copied, st.copy));
     SHIP(hipStreamWaitEvent(s, sl.copied, 0));
     STRY(spmm_f32(s, rows, n, cnnz, sl.rowptr, sl.colidx, vals ? sl.vals : nullptr, alpha, B, ldb, r,
-                      Y + (int64_t)r0 * ldy, ldy));
+                      Y + (int64_t)r0 * ldy, ldy, sl.order));
     SHIP(hipEventRecord(sl.used, s));
   }
   SHIP(hipStreamSynchronize(s));

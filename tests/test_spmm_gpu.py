@@ -105,15 +105,24 @@ def test_row_order_and_ordered_spmm(mmsb, dev, r):
 
 @pytest.mark.parametrize("chunk_nnz", [1, 50, 700, 1 << 22])
 @pytest.mark.parametrize("weighted", [False, True])
-def test_spmm_with_host_csr_matches_device(mmsb, dev, chunk_nnz, weighted):
+@pytest.mark.parametrize("pinned", [False, True])
+def test_spmm_with_host_csr_matches_device(mmsb, dev, chunk_nnz, weighted, pinned):
     """The CSR block in host memory, streamed in row chunks (mmsb_spmm_f32_host_csr): Y identical to
     the device-resident product, for chunks of one nonzero (every row its own chunk, rows longer than
-    the chunk included), a few hundred, and the whole block; empty rows at both ends."""
+    the chunk included), a few hundred, and the whole block; empty rows at both ends; from pageable
+    arrays (staged) and from pinned ones (copied directly)."""
     import torch
     rng = np.random.default_rng(chunk_nnz + weighted)
     m, n, r = 1500, 900, 130
     rowptr, colidx = random_csr(rng, m, n, 0.05, empty_rows=(0, 1, 700, 1499))
     vals = rng.uniform(0.5, 2.0, colidx.size).astype(np.float32) if weighted else None
+    if pinned:
+        def pin(a):
+            t = torch.empty(a.size, dtype=torch.int32 if a.dtype == np.int32 else torch.float32, pin_memory=True)
+            t.numpy()[:] = a
+            return t.numpy()
+        colidx = pin(colidx)
+        vals = None if vals is None else pin(vals)
     B = torch.from_numpy(rng.standard_normal((n, 132)).astype(np.float32)).to(dev)
     want = torch.full((m, 132), 3.0, dtype=torch.float32, device=dev)
     got = torch.full((m, 132), 3.0, dtype=torch.float32, device=dev)
