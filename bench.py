@@ -83,8 +83,8 @@ def main():
                          "for rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE passes; prints no JSON line")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (see DESIGN.md, "
-              "'How the path would shard'); run with --gpus 1", file=sys.stderr)
+        print("bench.py measures one GPU: the library runs each context on one GPU and has no "
+              "multi-GPU path; run with --gpus 1", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np

@@ -1,7 +1,8 @@
 // Seeded synthetic MMSB graph generator (host, multi-threaded, deterministic).
 //
 // The reference consumes externally produced partition files (stpm.h:63-69) and ships no
-// generator; this produces the BASELINE workloads in the same block layout. The model and every
+// generator; this produces seeded synthetic workloads (1000 to 4M nodes per partition, plain or
+// degree-corrected) in the same block layout. The model and every
 // operation order are those of oracle/gen.py, which this file reproduces bit-for-bit:
 //   theta_i = 1 or Pareto(a) / mean;  pi_i ~ Dirichlet(alpha0/k) by Gamma(1+alpha) U^(1/alpha)
 //   (Marsaglia-Tsang, polar normals), components with U^(1/alpha) < e^-20 set to 0;
