@@ -30,6 +30,9 @@
 // in FP32. In BF16 mode every k x k operand is written by the kernel that produces it both in
 // FP32 and as its (hi, lo) BF16 pair, so the MFMA loops load ready operands and convert nothing.
 // m_A and m_B of the next step are reduced from the tiles of the new Phi where it is written.
+// Y_A, Y_B, Y_C^T either hold all of X (row offset row0) or, for checkpointed runs, only the current
+// minibatch (row offset 0, Y_C^T with leading dimension b), refilled from CSR before each step by a
+// caller-supplied launch (StgdArgs::y_window).
 #include <hip/hip_cooperative_groups.h>
 
 #include <algorithm>
