@@ -75,6 +75,7 @@ struct YRowsJob {
   void* out;
   int64_t ldo;
   int32_t trans;
+  int64_t rows_R;     // rows of R (its n_cols block dimension): sizes the gather for the strip choice
 };
 struct YRowsJobs {
   YRowsJob j[3];

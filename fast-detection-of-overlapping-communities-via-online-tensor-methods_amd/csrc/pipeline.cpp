@@ -381,7 +381,8 @@ static int fill_window(mmsb_ctx* c, bool bf16, int64_t row0, int32_t nb) {
   for (int i = 0; i < 3; ++i) {
     const DevCsr& g = c->blk[i];
     void* out = bf16 ? (void*)c->yw16[i] : (void*)c->yw[i];
-    jobs.j[i] = YRowsJob{g.rowptr + row0, g.colidx, g.vals, R[i], out, i == 2 ? (int64_t)c->b : (int64_t)c->kp, i == 2};
+    jobs.j[i] = YRowsJob{g.rowptr + row0, g.colidx, g.vals, R[i], out, i == 2 ? (int64_t)c->b : (int64_t)c->kp, i == 2,
+                         (int64_t)g.n_cols};
   }
   jobs.count = 3;
   return spmm_yrows(c->s, jobs, nb, c->k, c->kp, bf16);
@@ -1528,7 +1529,7 @@ extern "C" int mmsb_power_method(mmsb_ctx* c, int32_t iters, int32_t orthogonali
       const float* R[3] = {c->W, c->MB, c->MC};
       for (int i = 0; i < 3; ++i) {
         const DevCsr& g = c->blk[i];
-        jobs.j[i] = YRowsJob{g.rowptr + r0, g.colidx, g.vals, R[i], yr[i], c->kp, 0};
+        jobs.j[i] = YRowsJob{g.rowptr + r0, g.colidx, g.vals, R[i], yr[i], c->kp, 0, (int64_t)g.n_cols};
       }
       jobs.count = 3;
       TRY(spmm_yrows(c->s, jobs, nb, c->k, c->kp, false));
@@ -1599,7 +1600,7 @@ static int yrows_host(mmsb_ctx* c, int32_t what, int64_t row0, int64_t nrows, do
   for (int64_t r = 0; r < nrows && st == MMSB_OK; r += chunk) {
     const int32_t m = (int32_t)std::min<int64_t>(chunk, nrows - r);
     YRowsJobs jobs{};
-    jobs.j[0] = YRowsJob{g.rowptr + row0 + r, g.colidx, g.vals, R[i], d, kp, 0};
+    jobs.j[0] = YRowsJob{g.rowptr + row0 + r, g.colidx, g.vals, R[i], d, kp, 0, (int64_t)g.n_cols};
     jobs.count = 1;
     st = spmm_yrows(c->s, jobs, m, k, kp, false);
     if (st == MMSB_OK) st = d2h(c->s, h.data(), d, sizeof(float) * (size_t)m * kp);
