@@ -9,10 +9,58 @@
 // counting-sort transpose element for element.
 #include <hipcub/hipcub.hpp>
 
+#include <mutex>
+
 #include "common.h"
 #include "dense.h"
 
 namespace mmsb {
+
+namespace {
+// Temporaries of the sorts (keys, indices, the radix sort's own storage), kept between calls and
+// grown on demand for the current device: a set of blocks is transposed and ordered without a
+// hipMalloc / hipFree pair per temporary. Calls are serialised, and each waits for its stream
+// before returning, so the next call finds them idle.
+struct SortScratch {
+  std::mutex mu;
+  int device = -1;
+  void* p[4] = {nullptr, nullptr, nullptr, nullptr};
+  size_t bytes[4] = {0, 0, 0, 0};
+};
+
+SortScratch& sort_scratch() {
+  static SortScratch s;
+  return s;
+}
+
+hipError_t scratch_ready(SortScratch& s) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess || dev == s.device) return e;
+  for (int i = 0; i < 4; ++i) {
+    if (s.p[i]) (void)hipFree(s.p[i]);
+    s.p[i] = nullptr;
+    s.bytes[i] = 0;
+  }
+  s.device = dev;
+  return hipSuccess;
+}
+
+template <typename T>
+hipError_t scratch_take(SortScratch& s, int i, size_t bytes, T** out) {
+  bytes = std::max<size_t>(bytes, 256);
+  if (s.bytes[i] < bytes) {
+    if (s.p[i]) (void)hipFree(s.p[i]);
+    s.p[i] = nullptr;
+    s.bytes[i] = 0;
+    const hipError_t e = hipMalloc(&s.p[i], bytes);
+    if (e != hipSuccess) return e;
+    s.bytes[i] = bytes;
+  }
+  *out = static_cast<T*>(s.p[i]);
+  return hipSuccess;
+}
+}  // namespace
 
 __global__ void iota_kernel(int64_t n, int32_t* __restrict__ out) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -69,12 +117,15 @@ int csr_transpose(hipStream_t s, int32_t nr, int32_t nc, int64_t nnz, const int3
   const int end_bit = bits_for(nc);
   int st = MMSB_OK;
   auto fail = [&](hipError_t e) { return set_error(MMSB_ERR_HIP, "csr_transpose: %s", hipGetErrorString(e)); };
-  hipError_t e = hipMalloc(&keys_out, sizeof(int32_t) * nnz);
-  if (e == hipSuccess) e = hipMalloc(&idx, sizeof(int32_t) * nnz);
-  if (e == hipSuccess) e = hipMalloc(&perm, sizeof(int32_t) * nnz);
+  SortScratch& sc = sort_scratch();
+  std::lock_guard<std::mutex> lock(sc.mu);
+  hipError_t e = scratch_ready(sc);
+  if (e == hipSuccess) e = scratch_take(sc, 0, sizeof(int32_t) * nnz, &keys_out);
+  if (e == hipSuccess) e = scratch_take(sc, 1, sizeof(int32_t) * nnz, &idx);
+  if (e == hipSuccess) e = scratch_take(sc, 2, sizeof(int32_t) * nnz, &perm);
   if (e == hipSuccess)
     e = hipcub::DeviceRadixSort::SortPairs(nullptr, temp_bytes, ci, keys_out, idx, perm, (int)nnz, 0, end_bit, s);
-  if (e == hipSuccess) e = hipMalloc(&temp, std::max<size_t>(temp_bytes, 1));
+  if (e == hipSuccess) e = scratch_take(sc, 3, temp_bytes, &temp);
   if (e == hipSuccess) {
     hipLaunchKernelGGL(iota_kernel, dim3((unsigned)ceil_div(nnz, 256)), dim3(256), 0, s, nnz, idx);
     e = hipGetLastError();
@@ -90,12 +141,8 @@ int csr_transpose(hipStream_t s, int32_t nr, int32_t nc, int64_t nnz, const int3
     hipLaunchKernelGGL(gather_rows_kernel, dim3((unsigned)ceil_div(nnz, 256)), dim3(256), 0, s, nr, nnz, rp, perm, v, ciT, vT);
     e = hipGetLastError();
   }
-  if (e == hipSuccess) e = hipStreamSynchronize(s);  // the temporaries are freed below
+  if (e == hipSuccess) e = hipStreamSynchronize(s);  // the temporaries are idle for the next call
   if (e != hipSuccess) st = fail(e);
-  (void)hipFree(keys_out);
-  (void)hipFree(idx);
-  (void)hipFree(perm);
-  (void)hipFree(temp);
   return st;
 }
 
@@ -110,12 +157,15 @@ int csr_row_order(hipStream_t s, int32_t nr, const int32_t* rp, int32_t* order) 
   void* temp = nullptr;
   size_t temp_bytes = 0;
   int st = MMSB_OK;
-  hipError_t e = hipMalloc(&len, sizeof(int32_t) * nr);
-  if (e == hipSuccess) e = hipMalloc(&len_out, sizeof(int32_t) * nr);
-  if (e == hipSuccess) e = hipMalloc(&idx, sizeof(int32_t) * nr);
+  SortScratch& sc = sort_scratch();
+  std::lock_guard<std::mutex> lock(sc.mu);
+  hipError_t e = scratch_ready(sc);
+  if (e == hipSuccess) e = scratch_take(sc, 0, sizeof(int32_t) * nr, &len);
+  if (e == hipSuccess) e = scratch_take(sc, 1, sizeof(int32_t) * nr, &len_out);
+  if (e == hipSuccess) e = scratch_take(sc, 2, sizeof(int32_t) * nr, &idx);
   if (e == hipSuccess)
     e = hipcub::DeviceRadixSort::SortPairsDescending(nullptr, temp_bytes, len, len_out, idx, order, (int)nr, 0, 31, s);
-  if (e == hipSuccess) e = hipMalloc(&temp, std::max<size_t>(temp_bytes, 1));
+  if (e == hipSuccess) e = scratch_take(sc, 3, temp_bytes, &temp);
   if (e == hipSuccess) {
     hipLaunchKernelGGL(row_lengths_kernel, dim3((unsigned)ceil_div((int64_t)nr, 256)), dim3(256), 0, s, nr, rp, len);
     hipLaunchKernelGGL(iota_kernel, dim3((unsigned)ceil_div((int64_t)nr, 256)), dim3(256), 0, s, (int64_t)nr, idx);
@@ -123,12 +173,8 @@ int csr_row_order(hipStream_t s, int32_t nr, const int32_t* rp, int32_t* order) 
   }
   if (e == hipSuccess)
     e = hipcub::DeviceRadixSort::SortPairsDescending(temp, temp_bytes, len, len_out, idx, order, (int)nr, 0, 31, s);
-  if (e == hipSuccess) e = hipStreamSynchronize(s);  // the temporaries are freed below
+  if (e == hipSuccess) e = hipStreamSynchronize(s);  // the temporaries are idle for the next call
   if (e != hipSuccess) st = set_error(MMSB_ERR_HIP, "csr_row_order: %s", hipGetErrorString(e));
-  (void)hipFree(len);
-  (void)hipFree(len_out);
-  (void)hipFree(idx);
-  (void)hipFree(temp);
   return st;
 }
 
