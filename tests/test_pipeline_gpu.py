@@ -537,3 +537,31 @@ def test_batch_in_a_second_process(mmsb, tmp_path):
     assert "child batch ok" in r.stdout
     batch.close()
     p.close()
+
+
+def test_factorisations_at_large_k(mmsb):
+    """k = 1100: the seeded Phi0 (CholeskyQR of a k x k Gaussian: the triangular inverse above 1024
+    columns) and the power method's orthonormalisation (the Jacobi factorisation with rotations in
+    4-column blocks, since 8-column blocks of A and V exceed the LDS at this size), against the
+    oracle. The means are kept small: with mu ~ 0.05 the shared mean terms make the power step's
+    columns nearly collinear (cond(Phi) ~ 1e7, so Phi^T Phi is at 1e-14 of its largest eigenvalue,
+    where the device drops directions as numerically null and the oracle divides by them); with
+    mu ~ 0.005 cond(Phi) ~ 9e3 and both orthonormalisations are determined to FP64 accuracy."""
+    n, k = 1536, 1100
+    rng = np.random.default_rng(7)
+    ys = [(rng.standard_normal((n, k)) / np.sqrt(k)).astype(np.float32).astype(np.float64) for _ in range(3)]
+    mus = [0.005 * rng.standard_normal(k) for _ in range(3)]
+    cfg = mmsb.config(nx=n, na=n, nb=n, nc=n, ncom=k, alpha_0=0.8, oversample=0, minibatch=256, seed=13)
+    p = mmsb.Pipeline(cfg)
+    p.set_yvectors(*ys, *mus)
+    p.stgd(0, "fp32")                                   # the seeded Phi0
+    phi0 = core.init_phi(13, k)
+    assert rel_err(p.get("phi"), phi0) < 1e-5
+    p.set_phi(phi0)
+    lam = p.power_method(1, orthogonalize=True)
+    phi_ref, lam_ref = core.power_method(*ys, *mus, phi0, alpha0=0.8, iters=1, orthogonalize=True)
+    np.testing.assert_allclose(lam, lam_ref, rtol=1e-5)
+    got = p.get("phi")
+    assert rel_err(got, phi_ref) < 1e-5
+    assert np.abs(got.T @ got - np.eye(k)).max() < 1e-6
+    p.close()
