@@ -61,7 +61,10 @@ def test_whitening_matches_oracle(mmsb, k, n, scaling):
     p.close()
 
 
-@pytest.mark.parametrize("k,b,renorm", [(10, 100, 1), (10, 64, 1), (20, 256, 1), (12, 128, 0), (70, 200, 1)])
+# b = 1 is the reference's one-node step (stpm.c:497-790); b = 1500 > n_X makes every step the whole
+# partition (one step per epoch, the rows beyond n_X zero)
+@pytest.mark.parametrize("k,b,renorm", [(10, 100, 1), (10, 64, 1), (20, 256, 1), (12, 128, 0), (70, 200, 1),
+                                        (10, 1, 1), (12, 1500, 1)])
 def test_stgd_steps_match_oracle(mmsb, k, b, renorm):
     n = 1000
     rng = np.random.default_rng(k + b)
