@@ -568,3 +568,46 @@ def test_factorisations_at_large_k(mmsb):
     assert rel_err(got, phi_ref) < 1e-5
     assert np.abs(got.T @ got - np.eye(k)).max() < 1e-6
     p.close()
+
+
+def test_unequal_partitions_match_oracle(mmsb):
+    """Partitions of four different sizes (n_X, n_A, n_B, n_C as the NX..NC macros of stpm.h:47-106
+    allow): whitening, the neighbour vectors, STGD steps and Pi of every partition against the oracle."""
+    nx, na, nb, nc, k = 900, 700, 800, 650, 6
+    g = mmsb.generate(nx, na, nb, nc, k, 1.0, 0.6, 0.02, seed=3)
+    cfg = mmsb.config(nx=nx, na=na, nb=nb, nc=nc, ncom=k, alpha_0=1.0, oversample=10, power_iters=2, seed=4,
+                      minibatch=128, learn_rate=2e-3)
+    bl = {nm: core.Csr(g[nm][0], g[nm][1], g[nm][3], g[nm][4]) for nm in ("XA", "XB", "XC", "BA", "CA")}
+    assert (bl["XA"].n_rows, bl["XA"].n_cols, bl["BA"].n_rows, bl["CA"].n_rows) == (nx, na, nb, nc)
+    wcfg = core.WhitenConfig(k=k, alpha0=1.0, oversample=10, power_iters=2, seed=4)
+    p = mmsb.Pipeline(cfg)
+    p.set_graph(g)
+    p.whiten()
+    ref = core.whiten(bl["XA"], bl["XB"], bl["XC"], wcfg)
+    ys = core.yvectors(bl["XA"], bl["XB"], bl["XC"], ref)
+    for nm, n in (("mu_a", na), ("mu_b", nb), ("mu_c", nc)):
+        got = p.get(nm)
+        assert got.shape == (n,), (nm, got.shape)
+        np.testing.assert_allclose(got, ref["mu_" + nm[-1].upper()], rtol=1e-15)
+    assert rel_err(p.get("W"), ref["W"]) < 1e-4
+    for nm, key in (("YA", "Y_A"), ("YB", "Y_B"), ("YC", "Y_C")):
+        got = p.get(nm)
+        assert got.shape == ys[key].shape == (nx, k), (nm, got.shape)
+        assert rel_err(got, ys[key]) < 1e-4, nm
+    # STGD from the seeded Phi0 on the device's own vectors (the oracle steps the same inputs)
+    ya, yb, yc = (p.get(nm).astype(np.float64) for nm in ("YA", "YB", "YC"))
+    mus = [p.get(nm) for nm in ("muw_a", "muw_b", "muw_c")]
+    phi0 = core.init_phi(4, k)
+    p.set_phi(phi0)
+    p.stgd(10, "fp32")
+    phi_ref, _ = core.stgd(ya, yb, yc, *mus, phi0, alpha0=1.0, eta=2e-3, batch=128, epochs=1, steps=10)
+    phi = p.get("phi")
+    assert rel_err(phi, phi_ref) < 1e-4
+    p.eigen()
+    p.post()
+    pis = core.post_proc(ref, phi, p.get("lambda_p"), bl["XA"], bl["BA"], bl["CA"])
+    for nm, P, n in zip(("pi_x", "pi_a", "pi_b", "pi_c"), pis, (nx, na, nb, nc)):
+        got = p.get(nm)
+        assert got.shape == P.shape and P.shape[0] == n, (nm, got.shape, P.shape)
+        assert rel_err(got, P) < 2e-4, nm
+    p.close()
