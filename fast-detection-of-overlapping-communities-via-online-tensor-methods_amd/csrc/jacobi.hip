@@ -20,6 +20,11 @@
 namespace mmsb {
 
 static constexpr int kMaxSweeps = 40;
+// Widest column block of the rotations (8, 4, 2 or 1); narrower blocks are taken when the wider do
+// not fit in LDS.
+#ifndef MMSB_JACOBI_BMAX
+#define MMSB_JACOBI_BMAX 8
+#endif
 // counters: [0, kMaxSweeps) rotations per sweep, [kMaxSweeps] sweeps run, [kMaxSweeps + 1] converged
 static constexpr int kSweepsRun = kMaxSweeps, kDone = kMaxSweeps + 1;
 
@@ -232,11 +237,11 @@ int jacobi_svd(hipStream_t s, int32_t r, double* A, double* U, double* sigma, do
   MMSB_LAUNCH_CHECK();
   const int arrays = V ? 2 : 1;
   int st;
-  if (fits(8, arrays))
+  if (MMSB_JACOBI_BMAX >= 8 && fits(8, arrays))
     st = V ? run_sweeps<8, true>(s, r, A, Vw, counters) : run_sweeps<8, false>(s, r, A, Vw, counters);
-  else if (fits(4, arrays))
+  else if (MMSB_JACOBI_BMAX >= 4 && fits(4, arrays))
     st = V ? run_sweeps<4, true>(s, r, A, Vw, counters) : run_sweeps<4, false>(s, r, A, Vw, counters);
-  else if (fits(2, arrays))
+  else if (MMSB_JACOBI_BMAX >= 2 && fits(2, arrays))
     st = V ? run_sweeps<2, true>(s, r, A, Vw, counters) : run_sweeps<2, false>(s, r, A, Vw, counters);
   else if (fits(1, arrays))
     st = V ? run_sweeps<1, true>(s, r, A, Vw, counters) : run_sweeps<1, false>(s, r, A, Vw, counters);
