@@ -122,3 +122,32 @@ def test_weighted_blocks_match_oracle_and_recompute(mmsb):
     assert np.array_equal(res.get("phi"), win.get("phi"))
     for p in ps:
         p.close()
+
+
+@pytest.mark.parametrize("prec", ["fp32", "bf16"])
+def test_recompute_with_unequal_partitions(mmsb, prec):
+    """Partitions of four sizes: the recomputed rows gather from blocks of n_A, n_B and n_C columns
+    (G_XA W, G_XB Z_B^T W, G_XC Z_C^T W), so each block's own width must be used; the recompute form
+    must still equal the resident one bit for bit."""
+    nx, na, nb, nc, k, b = 1100, 700, 900, 650, 20, 128
+    g = mmsb.generate(nx, na, nb, nc, k, 1.0, 0.6, 0.02, seed=5)
+    ps = []
+    for mode in ("stored", "recompute"):
+        cfg = mmsb.config(nx=nx, na=na, nb=nb, nc=nc, ncom=k, alpha_0=1.0, oversample=10, power_iters=1, seed=6,
+                          minibatch=b, learn_rate=2e-3, epochs=1, y_mode=mode)
+        p = mmsb.Pipeline(cfg)
+        p.set_graph(g)
+        p.whiten()
+        ps.append(p)
+    res, win = ps
+    for nm in ("YA", "YB", "YC"):
+        assert np.array_equal(res.get(nm), win.get(nm)), nm
+    for p in ps:
+        p.stgd(-1, prec)
+    assert np.array_equal(res.get("phi"), win.get("phi"))
+    for p in ps:
+        p.post()
+    for nm in ("pi_x", "pi_a", "pi_b", "pi_c"):
+        assert np.array_equal(res.get(nm), win.get(nm)), nm
+    for p in ps:
+        p.close()
