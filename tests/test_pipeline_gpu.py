@@ -223,7 +223,7 @@ def test_read_inputs_and_write_outputs(mmsb, tmp_path):
     p.close()
 
 
-@pytest.mark.parametrize("k,b,degree", [(10, 128, 3), (70, 200, 3), (24, 64, 3), (70, 200, 5)])
+@pytest.mark.parametrize("k,b,degree", [(10, 128, 3), (70, 200, 3), (24, 64, 3), (70, 200, 5), (10, 1, 3)])
 def test_stgd_bf16_tracks_oracle(mmsb, k, b, degree):
     """BF16 contractions (FP32 accumulation, split-BF16 k^3 products): Phi within 5e-3 of the FP64
     oracle after 25 steps, and the FP32 polish epoch brings it back to the FP32 trajectory's level."""
