@@ -82,18 +82,22 @@ extern "C" int mmsb_read_coo(const char* path, int32_t n_rows, int32_t n_cols, i
     ents.push_back({row * (int64_t)n_cols + col, weighted ? nums[e * w + 2] : 1.0, e});
   }
   std::sort(ents.begin(), ents.end(), [](const Ent& x, const Ent& y) { return x.key < y.key || (x.key == y.key && x.order < y.order); });
+  // Values are kept for weighted files (unless every entry is forced to 1) and for summed two-column
+  // files, where read_G_sparse adds 1.0 per line (IOfn.cpp:45-46) so that a repeated line -- the
+  // while(!feof) repeat of the last one included -- counts twice.
+  const bool with_vals = dup_policy != MMSB_DUP_ONE && (weighted || dup_policy == MMSB_DUP_SUM);
   std::vector<int64_t> keys;
   std::vector<double> vals;
   for (size_t i = 0; i < ents.size();) {
     size_t j = i;
     double v = 0.0;
     while (j < ents.size() && ents[j].key == ents[i].key) {
-      if (!weighted || dup_policy == MMSB_DUP_ONE) v = 1.0;
+      if (dup_policy == MMSB_DUP_ONE) v = 1.0;
       else if (dup_policy == MMSB_DUP_LAST) v = ents[j].v;
       else v += ents[j].v;
       ++j;
     }
-    if (!weighted || std::fabs(v) > 1e-18) {
+    if (!with_vals || std::fabs(v) > 1e-18) {
       keys.push_back(ents[i].key);
       vals.push_back(v);
     }
@@ -106,8 +110,8 @@ extern "C" int mmsb_read_coo(const char* path, int32_t n_rows, int32_t n_cols, i
   out->nnz = nnz;
   out->rowptr = (int32_t*)std::calloc((size_t)n_rows + 1, sizeof(int32_t));
   out->colidx = (int32_t*)std::malloc(sizeof(int32_t) * (size_t)std::max<int64_t>(nnz, 1));
-  out->vals = (weighted && dup_policy != MMSB_DUP_ONE) ? (float*)std::malloc(sizeof(float) * (size_t)std::max<int64_t>(nnz, 1)) : nullptr;
-  if (!out->rowptr || !out->colidx || (weighted && dup_policy != MMSB_DUP_ONE && !out->vals))
+  out->vals = with_vals ? (float*)std::malloc(sizeof(float) * (size_t)std::max<int64_t>(nnz, 1)) : nullptr;
+  if (!out->rowptr || !out->colidx || (with_vals && !out->vals))
     return set_error(MMSB_ERR_NOMEM, "read_coo: out of host memory");
   for (int64_t e = 0; e < nnz; ++e) {
     out->rowptr[keys[e] / n_cols + 1]++;

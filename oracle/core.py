@@ -242,6 +242,17 @@ def paper_coeffs(alpha0: float):
     return (alpha0 + 1.0) * (alpha0 + 2.0) / 2.0, -alpha0 * (alpha0 + 1.0) / 2.0, alpha0 * alpha0
 
 
+def coeffs_for(alpha0: float, scaling: str = "paper"):
+    """The tensor coefficients that pair with an M2 scaling: the paper / CPU M2 (Util.cpp:81,109-113)
+    goes with the paper T (Util.cpp:289-298), the GPU M2 (pre_proc.c:292-302) with the init_buff T
+    (stpm.c:1026-1038), as each reference path does."""
+    if scaling == "paper":
+        return paper_coeffs(alpha0)
+    if scaling == "gpu":
+        return gpu_coeffs(alpha0)
+    raise ValueError(scaling)
+
+
 def init_phi(seed: int, k: int) -> np.ndarray:
     """Phi0: orthonormalised k x k Gaussian (replaces the time-seeded QR of rand(), stpm.c:1041-1064)."""
     Q, _ = cholqr2(gaussian_matrix(seed, 2, k, k, True))
@@ -286,20 +297,52 @@ def polar_renorm(phi, ns_iters=2, degree=3):
     return phi
 
 
-def stgd_step(phi, YA, YB, YC, muA, muB, muC, alpha0, eta, theta=1.0, renorm=True, ns_iters=2, ns_degree=3):
-    """Phi <- Phi - eta [theta Phi (Phi^T Phi)^o2 - T(Phi, Phi, I)] (stpm.c:649-691), then the
-    orthonormal renormalisation."""
+def qr_renorm(phi):
+    """Q of the QR factorisation with a non-negative diagonal of R (culaDeviceDgeqrfp followed by
+    culaDeviceDorgqr, stpm.c:774-786): numpy's Householder QR with the signs of Q's columns set so
+    that R's diagonal is positive (the factorisation is then unique for a full-rank Phi)."""
+    Q, R = np.linalg.qr(phi)
+    d = np.sign(np.diag(R))
+    d[d == 0] = 1.0
+    return Q * d
+
+
+def stgd_step(phi, YA, YB, YC, muA, muB, muC, alpha0, eta, theta=1.0, renorm=True, ns_iters=2, ns_degree=3,
+              scaling="paper", renorm_mode="ns"):
+    """Phi <- Phi - eta [theta Phi (Phi^T Phi)^o2 - T(Phi, Phi, I)] (stpm.c:649-691) with the
+    coefficients of `scaling`, then the orthonormal renormalisation: Newton-Schulz towards the polar
+    factor (renorm_mode "ns") or the QR of the reference (renorm_mode "qr", stpm.c:774-786).
+    Returns (Phi_next, Phi~) with Phi~ the iterate before the renormalisation."""
     G = phi.T @ phi
-    T = t_hat_c(phi, YA, YB, YC, muA, muB, muC, gpu_coeffs(alpha0))
+    T = t_hat_c(phi, YA, YB, YC, muA, muB, muC, coeffs_for(alpha0, scaling))
     new = phi - eta * (theta * phi @ (G * G) - T)
+    pre = new
     if renorm:
-        new = polar_renorm(new, ns_iters, ns_degree)
-    return new
+        if renorm_mode == "qr":
+            new = qr_renorm(new)
+        elif renorm_mode == "ns":
+            new = polar_renorm(new, ns_iters, ns_degree)
+        else:
+            raise ValueError(renorm_mode)
+    return new, pre
+
+
+def stop_rule(step, frob, thresh, iter_min, iter_max):
+    """The STGD loop's end test after step `step` (0-based, sgd_iter of stpm.c:194,769) with
+    ||dPhi||_F^2 = frob: converged when frob < THRESH and step > SGD_ITER_MIN (stpm.c:747), or the
+    cap when step > SGD_ITER_MAX (stpm.c:761). A bound <= 0 switches its rule off."""
+    if iter_min > 0 and frob < thresh and step > iter_min:
+        return True
+    return iter_max > 0 and step > iter_max
 
 
 def stgd(Y_A, Y_B, Y_C, mu_A, mu_B, mu_C, phi0, alpha0, eta, batch, epochs, theta=1.0,
-         renorm_every=1, steps=None, ns_iters=2, ns_degree=3):
-    """Sequential minibatches over X (stpm.c:499-502,768); returns Phi and ||dPhi||_F^2 per step."""
+         renorm_every=1, steps=None, ns_iters=2, ns_degree=3, scaling="paper", renorm_mode="ns",
+         thresh=None, iter_min=0, iter_max=0):
+    """Sequential minibatches over X (stpm.c:499-502,768); returns Phi and ||dPhi||_F^2 per step.
+    The log holds ||Phi_{t+1} - Phi_t||^2 after the renormalisation, except in renorm_mode "qr",
+    which logs the change before the QR as the reference does (stpm.c:705-743, QR at :774-786).
+    With thresh given, the loop ends early by stop_rule (checked after every step)."""
     phi = phi0.copy()
     n = Y_A.shape[0]
     log = []
@@ -309,10 +352,13 @@ def stgd(Y_A, Y_B, Y_C, mu_A, mu_B, mu_C, phi0, alpha0, eta, batch, epochs, thet
         lo = (s % ((n + batch - 1) // batch)) * batch
         hi = min(lo + batch, n)
         renorm = renorm_every > 0 and ((step + 1) % renorm_every == 0)
-        new = stgd_step(phi, Y_A[lo:hi], Y_B[lo:hi], Y_C[lo:hi], mu_A, mu_B, mu_C, alpha0, eta, theta, renorm, ns_iters,
-                        ns_degree)
-        log.append(float(((new - phi) ** 2).sum()))
+        new, pre = stgd_step(phi, Y_A[lo:hi], Y_B[lo:hi], Y_C[lo:hi], mu_A, mu_B, mu_C, alpha0, eta, theta, renorm,
+                             ns_iters, ns_degree, scaling, renorm_mode)
+        ref = pre if renorm_mode == "qr" else new
+        log.append(float(((ref - phi) ** 2).sum()))
         phi = new
+        if thresh is not None and stop_rule(step, log[-1], thresh, iter_min, iter_max):
+            break
         step += 1
     return phi, np.array(log)
 
@@ -419,7 +465,9 @@ def power_method(Y_A, Y_B, Y_C, mu_A, mu_B, mu_C, phi0, alpha0, iters, orthogona
     """phi_i <- T(I, phi_i, phi_i) over all of X with the paper coefficients (A-mode free, as
     tensor_form_main/shift1/shift0 Util.cpp:187-206), then column normalisation (Util.cpp:166),
     plus a symmetric orthonormalisation when requested (the reference has none, so its columns
-    collapse). Returns Phi and lambda = ||T(I,phi,phi)|| per column."""
+    collapse). Returns Phi and lambda = ||T(I,phi,phi)||^3 per column, the reference's
+    (sum phi^2)^(3/2) of the unnormalised iterate (Util.cpp:164-165). The orthonormalisation is the
+    polar factor U V^T of Phi's own SVD, so it does not square Phi's condition number."""
     c3, c1, c0 = paper_coeffs(alpha0)
     phi = phi0.copy()
     n = Y_A.shape[0]
@@ -430,10 +478,11 @@ def power_method(Y_A, Y_B, Y_C, mu_A, mu_B, mu_C, phi0, alpha0, iters, orthogona
         KA = c3 * b * c + c1 * (b * mc + mb * c)
         kA = c0 * mb * mc + c1 * (b * c).mean(0)
         new = Y_A.T @ KA / n + np.outer(mu_A, kA)
-        lam = np.sqrt((new * new).sum(0))
-        new = new / np.where(lam > 0, lam, 1.0)
+        nrm = np.sqrt((new * new).sum(0))
+        lam = nrm ** 3
+        new = new / np.where(nrm > 0, nrm, 1.0)
         if orthogonalize:
-            w, V = np.linalg.eigh(new.T @ new)
-            new = new @ (V / np.sqrt(np.maximum(w, 1e-300))) @ V.T
+            U, _, Vt = np.linalg.svd(new)
+            new = U @ Vt
         phi = new
     return phi, lam

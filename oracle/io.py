@@ -35,8 +35,9 @@ def read_coo(path: str, n_rows: int, n_cols: int, first_is_row: bool = True, wei
     """Read a 1-based COO text file into CSR (rowptr, colidx, vals or None, n_rows, n_cols).
 
     duplicates: "one" (BINARY: entry = 1), "last" (EXPECTED on the GPU path: last value wins),
-    "sum" (CPU path: setFromTriplets sums). feof_repeat applies the last entry once more, as the
-    reference's while(!feof) loop does when the file ends with a newline.
+    "sum" (CPU path: setFromTriplets sums, also for two-column files, whose lines count 1 each).
+    feof_repeat applies the last entry once more, as the reference's while(!feof) loop does when
+    the file ends with a newline.
     """
     nums = parse_numbers(path)
     w = 3 if weighted else 2
@@ -52,10 +53,12 @@ def read_coo(path: str, n_rows: int, n_cols: int, first_is_row: bool = True, wei
     rows, cols = (a, b) if first_is_row else (b, a)
     if rows.size and (rows.min() < 0 or rows.max() >= n_rows or cols.min() < 0 or cols.max() >= n_cols):
         raise ValueError(f"{path}: index out of range for {n_rows} x {n_cols}")
-    if not weighted or duplicates == "one":
+    if duplicates == "one" or (not weighted and duplicates == "last"):
         rp, ci, nr, nc = coo_to_csr(rows, cols, n_rows, n_cols)
         return rp, ci, None, nr, nc
-    vals = nums[:, 2]
+    # weighted, or summed binary triplets (read_G_sparse pushes 1.0 per line, IOfn.cpp:45-46, so the
+    # feof repeat of the last line gives that entry the value 2)
+    vals = nums[:, 2] if weighted else np.ones(nums.shape[0])
     key = rows * n_cols + cols
     if duplicates == "last":
         order = np.argsort(key, kind="stable")
@@ -101,3 +104,42 @@ def format_pi_triplets(pi_kn: np.ndarray) -> str:
             if v != 0.0:
                 out.append("%d\t%d\t%g\n" % (row + 1, col + 1, v))
     return "".join(out)
+
+
+SIM_INFO_LINES = (
+    ("Total number of nodes          = %d", "n_total"),
+    ("Number of nodes in partition X = %d", "nx"),
+    ("Number of nodes in partition A = %d", "na"),
+    ("Number of nodes in partition B = %d", "nb"),
+    ("Number of nodes in partition C = %d", "nc"),
+    ("Number of nodes communities    = %d", "ncom"),
+    ("Alpha_0 (Dirichlet parameter)  = %5.25e", "alpha_0"),
+    ("Learning rate = %5.25e", "learn_rate"),
+    ("Threshold for STPM convergence    = %5.25e", "thresh"),
+    ("Pseudoinverse numerical tolerance = %5.25e", "pinv_tol"),
+    ("Total number of Frob Norm values written to file = %d", "n_frob"),
+    ("Number of iterations between convergence test    = %d", "iters_between_tests"),
+    ("Maximum number of stochastic iterations allowed  = %d", "sgd_iter_max"),
+    ("Exec Time svd1  = %5.25e (Seconds)", "t_svd1"),
+    ("Exec Time svd2  = %5.25e (Seconds)", "t_svd2"),
+    ("Exec Time Pre Proc  = %5.25e (Seconds)", "t_pre"),
+    ("Exec Time STPM      = %5.25e (Seconds)", "t_stpm"),
+    ("Exec Time Post Proc = %5.25e (Seconds)", "t_post"),
+    ("Total Exec Time     = %5.25e (Seconds)", "t_total"),
+)
+
+
+def format_sim_info(v: dict) -> str:
+    """The sim_info file, line for line as stpm.c:957-975 prints it (the total time is the sum of
+    the three phase times, stpm.c:975)."""
+    return "".join((fmt % v[key]) + "\n" for fmt, key in SIM_INFO_LINES)
+
+
+def parse_sim_info(text: str) -> dict:
+    """Inverse of format_sim_info: the value of every line, keyed as SIM_INFO_LINES."""
+    lines = text.split("\n")
+    out = {}
+    for (fmt, key), line in zip(SIM_INFO_LINES, lines):
+        val = line.split("=", 1)[1].split("(")[0].strip()
+        out[key] = int(val) if fmt.endswith("%d") else float(val)
+    return out
