@@ -10,6 +10,7 @@ torch ships its own libamdhip64.so.7 and the library's DT_NEEDED entry then reso
 """
 from __future__ import annotations
 
+import atexit
 import ctypes
 import os
 
@@ -78,12 +79,13 @@ class StpmConfigC(ctypes.Structure):
                 ("m2_scaling", ctypes.c_int32), ("member_tau", ctypes.c_double), ("seed", ctypes.c_uint64),
                 ("ba_first_is_row", ctypes.c_int32), ("feof_repeat", ctypes.c_int32), ("device", ctypes.c_int32),
                 ("verbose", ctypes.c_int32), ("use_graphs", ctypes.c_int32),
-                ("y_mode", ctypes.c_int32)]
+                ("y_mode", ctypes.c_int32), ("renorm_mode", ctypes.c_int32)]
 
 
 PREC = {"fp32": 0, "bf16": 1}
 M2_SCALING = {"paper": 0, "gpu": 1}
 Y_MODE = {"auto": 0, "stored": 1, "recompute": 2}
+RENORM = {"ns": 0, "qr": 1}
 BLOCKS = {"XA": 0, "XB": 1, "XC": 2, "BA": 3, "CA": 4}
 QUANTITIES = ["mu_a", "mu_b", "mu_c", "W", "U", "V", "sigma", "LB", "LC", "lam_m2", "YA", "YB", "YC",
               "muw_a", "muw_b", "muw_c", "phi", "frob", "lambda", "lambda_p", "alpha_hat",
@@ -148,6 +150,9 @@ SIGNATURES: dict = {
     "stpm_force_exit": (None, []),
     "mmsb_grad": (ctypes.c_int, [_vp, ctypes.c_int64, ctypes.c_int32, _vp, ctypes.c_int32, ctypes.c_double, ctypes.c_int32,
                                  _vp, _vp]),
+    "mmsb_write_triplets": (ctypes.c_int, [_vp, ctypes.c_char_p, ctypes.c_char_p]),
+    "mmsb_release": (None, []),
+    "mmsb_set_strip_limit": (ctypes.c_int64, [ctypes.c_int64]),
     "pre_proc": (ctypes.c_int, []),
     "GPU_Task": (_vp, [_vp]),
     "post_proc": (ctypes.c_int, []),
@@ -166,6 +171,9 @@ def _load():
 
 
 lib = _load()
+# Free the library's process-wide pinned staging, copy stream and sort temporaries while the HIP
+# runtime is still alive (Python's exit handlers run before the runtime's own teardown).
+atexit.register(lib.mmsb_release)
 
 
 def last_error() -> str:
@@ -271,6 +279,8 @@ def config(**kw) -> StpmConfigC:
             val = M2_SCALING[val]
         if key == "y_mode" and isinstance(val, str):
             val = Y_MODE[val]
+        if key == "renorm_mode" and isinstance(val, str):
+            val = RENORM[val]
         if isinstance(val, str):
             val = val.encode()
         if not hasattr(c, key):
@@ -374,6 +384,10 @@ class Pipeline:
 
     def write_outputs(self):
         check(lib.mmsb_write_outputs(self.ctx))
+
+    def write_triplets(self, pi_path=None, white_path=None):
+        check(lib.mmsb_write_triplets(self.ctx, None if pi_path is None else str(pi_path).encode(),
+                                      None if white_path is None else str(white_path).encode()))
 
     def get(self, name):
         import numpy as np

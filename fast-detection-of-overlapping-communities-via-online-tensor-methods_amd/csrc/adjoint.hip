@@ -110,8 +110,9 @@ __global__ void pm_elem_kernel(int32_t nb, int32_t k, const double* __restrict__
   colsum[j] += sbc;
 }
 
-// T[:, j] = T[:, j] / n + mu_A kappa_j, kappa = c0 m_B o m_C + c1 colsum / n; lam_j = ||T[:, j]||;
-// Phi[:, j] = T[:, j] / lam_j. One workgroup.
+// T[:, j] = T[:, j] / n + mu_A kappa_j, kappa = c0 m_B o m_C + c1 colsum / n;
+// lam_j = ||T[:, j]||^3, the reference's (sum phi^2)^(3/2) (Util.cpp:164-165); Phi[:, j] = T[:, j] / ||T[:, j]||
+// (normc, Util.cpp:166). One workgroup.
 __global__ __launch_bounds__(256) void pm_finish_kernel(int32_t k, double inv_n, const double* __restrict__ T,
                                                         const double* __restrict__ mua, const double* __restrict__ m,
                                                         const double* __restrict__ colsum, double c1, double c0,
@@ -124,7 +125,7 @@ __global__ __launch_bounds__(256) void pm_finish_kernel(int32_t k, double inv_n,
       nn += v * v;
     }
     const double l = sqrt(nn);
-    lam[j] = l;
+    lam[j] = l * l * l;
     const double sc = l > 0.0 ? 1.0 / l : 0.0;
     for (int i = 0; i < k; ++i) phi[(int64_t)i * k + j] = (T[(int64_t)i * k + j] * inv_n + mua[i] * kap) * sc;
   }
@@ -135,19 +136,18 @@ __global__ void zero_d_kernel(int64_t n, double* p) {
   if (t < n) p[t] = 0.0;
 }
 
-// S = V diag(1/sqrt(sigma)) V^T (column-major V from the Jacobi SVD of Phi^T Phi; row-major S)
-__global__ void inv_sqrt_sym_kernel(int32_t k, const double* __restrict__ V, const double* __restrict__ sig,
-                                    double* __restrict__ S) {
+// P = U V^T over the singular triplets with sigma_l > 0 (column-major U, V of the one-sided Jacobi
+// SVD; out column-major). Applied to the SVD of Phi^T (the row-major Phi buffer read column-major),
+// out read row-major is the polar factor of Phi, the orthonormal matrix nearest to it.
+__global__ void polar_kernel(int32_t k, const double* __restrict__ U, const double* __restrict__ V,
+                             const double* __restrict__ sig, double* __restrict__ out) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (int64_t)k * k) return;
-  const int i = (int)(t / k), j = (int)(t % k);
-  const double smax = sig[0];
+  const int i = (int)(t % k), j = (int)(t / k);
   double acc = 0.0;
-  for (int l = 0; l < k; ++l) {
-    const double sl = sig[l];
-    if (sl > 1e-14 * smax) acc += V[(int64_t)l * k + i] * V[(int64_t)l * k + j] / sqrt(sl);
-  }
-  S[t] = acc;
+  for (int l = 0; l < k; ++l)
+    if (sig[l] > 0.0) acc += U[(int64_t)l * k + i] * V[(int64_t)l * k + j];
+  out[t] = acc;
 }
 
 static size_t align256(size_t b) { return (b + 255) / 256 * 256; }
@@ -156,13 +156,15 @@ size_t power_workspace_bytes(int32_t k, int32_t chunk) {
   const size_t ck = sizeof(double) * (size_t)chunk * k, kk = sizeof(double) * (size_t)k * k;
   size_t tn = std::max(tn_workspace_bytes(chunk, k, k), tn_workspace_bytes(k, k, k));
   tn = std::max(tn, tn_workspace_bytes(k, k, 1));
-  return 3 * align256(ck) + 6 * align256(kk) + 3 * align256(sizeof(double) * 4 * k) + align256(tn) +
+  return 3 * align256(ck) + 4 * align256(kk) + 3 * align256(sizeof(double) * 4 * k) + align256(tn) +
          align256(jacobi_workspace_bytes(k)) + align256(64);
 }
 
 // One iteration of the batch implicit power method with the A mode free (CPU tensorDecom_alpha0,
-// Util.cpp:131-206): Phi_j <- T(I, phi_j, phi_j) over all n rows, lambda_j = ||.||, columns
-// normalised, then (optionally) the symmetric orthonormalisation Phi (Phi^T Phi)^-1/2.
+// Util.cpp:131-206): Phi_j <- T(I, phi_j, phi_j) over all n rows, lambda_j = ||.||^3, columns
+// normalised, then (optionally) the orthonormalisation Phi <- U V^T from the one-sided Jacobi SVD of
+// Phi itself (the polar factor Phi (Phi^T Phi)^-1/2, without forming Phi^T Phi, whose condition
+// number is cond(Phi)^2).
 // Y_A, Y_B row-major [n x ldy]; Y_C given transposed [k x ldyc] (node-contiguous).
 int power_iteration(hipStream_t s, int32_t k, int64_t n, const PowerRows& rows, int32_t ldy, const double* mua,
                     const double* mub, const double* muc, double c3, double c1, double c0, double* phi, double* lam,
@@ -171,8 +173,7 @@ int power_iteration(hipStream_t s, int32_t k, int64_t n, const PowerRows& rows, 
   auto take = [&](size_t bytes) { void* p = w; w += align256(bytes); return p; };
   const size_t ck = sizeof(double) * (size_t)chunk * k, kk = sizeof(double) * (size_t)k * k;
   double *B = (double*)take(ck), *C = (double*)take(ck), *KA = (double*)take(ck);
-  double *T = (double*)take(kk), *G = (double*)take(kk), *Vj = (double*)take(kk), *Uj = (double*)take(kk),
-         *Sj = (double*)take(kk), *Ptmp = (double*)take(kk);
+  double *T = (double*)take(kk), *G = (double*)take(kk), *Vj = (double*)take(kk), *Uj = (double*)take(kk);
   double *m = (double*)take(sizeof(double) * 4 * k), *colsum = (double*)take(sizeof(double) * 4 * k),
          *sig = (double*)take(sizeof(double) * 4 * k);
   double* tn = (double*)take(std::max(tn_workspace_bytes(chunk, k, k), std::max(tn_workspace_bytes(k, k, k), tn_workspace_bytes(k, k, 1))));
@@ -194,14 +195,12 @@ int power_iteration(hipStream_t s, int32_t k, int64_t n, const PowerRows& rows, 
   }
   hipLaunchKernelGGL(pm_finish_kernel, dim3(1), dim3(256), 0, s, k, 1.0 / (double)n, T, mua, m, colsum, c1, c0, phi, lam);
   MMSB_LAUNCH_CHECK();
-  if (orthogonalize) {  // Phi <- Phi (Phi^T Phi)^-1/2
-    MMSB_TRY((gemm_tn<double, double>(s, k, k, k, 1.0, phi, k, phi, k, 0.0, G, tn)));
-    MMSB_TRY(jacobi_svd(s, k, G, Uj, sig, Vj, jw, sweeps));
-    hipLaunchKernelGGL(inv_sqrt_sym_kernel, dim3((unsigned)ceil_div((int64_t)k * k, 256)), dim3(256), 0, s, k, Vj, sig, Sj);
-    MMSB_LAUNCH_CHECK();
-    MMSB_TRY((gemm_nn<double, double>(s, k, k, k, 1.0, phi, k, Sj, k, 0.0, Ptmp, k)));
-    hipError_t e = hipMemcpyAsync(phi, Ptmp, kk, hipMemcpyDeviceToDevice, s);
+  if (orthogonalize) {  // Phi <- U V^T of Phi's own SVD (G holds a copy: the Jacobi SVD overwrites its input)
+    hipError_t e = hipMemcpyAsync(G, phi, kk, hipMemcpyDeviceToDevice, s);
     if (e != hipSuccess) return set_error(MMSB_ERR_HIP, "power: %s", hipGetErrorString(e));
+    MMSB_TRY(jacobi_svd(s, k, G, Uj, sig, Vj, jw, sweeps));
+    hipLaunchKernelGGL(polar_kernel, dim3((unsigned)ceil_div((int64_t)k * k, 256)), dim3(256), 0, s, k, Uj, Vj, sig, phi);
+    MMSB_LAUNCH_CHECK();
   }
   return MMSB_OK;
 }

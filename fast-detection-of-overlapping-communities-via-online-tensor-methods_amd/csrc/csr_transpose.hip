@@ -13,6 +13,7 @@
 
 #include "common.h"
 #include "dense.h"
+#include "xfer.h"
 
 namespace mmsb {
 
@@ -29,19 +30,24 @@ struct SortScratch {
 };
 
 SortScratch& sort_scratch() {
-  static SortScratch s;
+  static SortScratch s;  // kept between calls; freed by release_sort_scratch (mmsb_release)
   return s;
+}
+
+void drop(SortScratch& s) {
+  for (int i = 0; i < 4; ++i) {
+    if (s.p[i]) (void)hipFree(s.p[i]);
+    s.p[i] = nullptr;
+    s.bytes[i] = 0;
+  }
+  s.device = -1;
 }
 
 hipError_t scratch_ready(SortScratch& s) {
   int dev = 0;
   hipError_t e = hipGetDevice(&dev);
   if (e != hipSuccess || dev == s.device) return e;
-  for (int i = 0; i < 4; ++i) {
-    if (s.p[i]) (void)hipFree(s.p[i]);
-    s.p[i] = nullptr;
-    s.bytes[i] = 0;
-  }
+  drop(s);
   s.device = dev;
   return hipSuccess;
 }
@@ -61,6 +67,12 @@ hipError_t scratch_take(SortScratch& s, int i, size_t bytes, T** out) {
   return hipSuccess;
 }
 }  // namespace
+
+void release_sort_scratch() {
+  SortScratch& s = sort_scratch();
+  std::lock_guard<std::mutex> lock(s.mu);
+  drop(s);
+}
 
 __global__ void iota_kernel(int64_t n, int32_t* __restrict__ out) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;

@@ -220,6 +220,8 @@ static int run_sweeps(hipStream_t s, int32_t r, double* A, double* Vw, int32_t* 
     MMSB_HIP(hipMemcpyAsync(&done, counters + kDone, sizeof(int32_t), hipMemcpyDeviceToHost, s));
     MMSB_HIP(hipStreamSynchronize(s));
   }
+  if (!done)  // every sweep still rotated: the factors are not converged, so say so instead of using them
+    return set_error(MMSB_ERR_NUMERIC, "jacobi: no convergence after %d sweeps (r = %d)", kMaxSweeps, r);
   return MMSB_OK;
 }
 

@@ -113,7 +113,9 @@ struct mmsb_ctx {
   int64_t frob_cap = 0, steps_done = 0;
   bf16_bits *YA16 = nullptr, *YB16 = nullptr, *YCt16 = nullptr;
   double* frob_epoch = nullptr;            // per-epoch log written by captured epochs
-  hipGraphExec_t epoch_graph[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};  // [precision][cur]
+  // captured epochs, [precision][cur][end test on]
+  hipGraphExec_t epoch_graph[2][2][2] = {{{nullptr, nullptr}, {nullptr, nullptr}}, {{nullptr, nullptr}, {nullptr, nullptr}}};
+  StgdStop* stopctl = nullptr;             // device state of the per-step end test (stpm.c:747-765)
   float *phiT = nullptr, *phitT = nullptr, *phiaT = nullptr;
   bool y16_valid = false;
   bool have_phi = false;
@@ -127,6 +129,9 @@ struct mmsb_ctx {
   int32_t* cnt[4] = {nullptr, nullptr, nullptr, nullptr};
   bool eig_done = false, post_done = false;
   double times[4] = {0, 0, 0, 0};
+  // seconds of the two factorisations of the last whitening: the randomized SVD of P_BC and the
+  // M2 eigenproblem (the svd1 / svd2 timers of sim_info, stpm.c:884-893,970-971)
+  double t_svd[2] = {0, 0};
 
   void log(const char* fmt, ...) const {
     if (!cfg.verbose) return;
@@ -167,12 +172,13 @@ static void dfree(mmsb_ctx* c, T*& p) {
 // Captured epoch graphs hold the step parameters and buffer addresses in their kernel arguments.
 static int drop_epoch_graphs(mmsb_ctx* c) {
   HCHK(hipStreamSynchronize(c->s));
-  for (auto& row : c->epoch_graph)
-    for (hipGraphExec_t& g : row)
-      if (g) {
-        HCHK(hipGraphExecDestroy(g));
-        g = nullptr;
-      }
+  for (auto& plane : c->epoch_graph)
+    for (auto& row : plane)
+      for (hipGraphExec_t& g : row)
+        if (g) {
+          HCHK(hipGraphExecDestroy(g));
+          g = nullptr;
+        }
   return MMSB_OK;
 }
 
@@ -421,6 +427,8 @@ static int whiten_impl(mmsb_ctx* c) {
   TRY(csr_rowsum(s, XBt.n_rows, XBt.rowptr, XBt.vals, 1.0 / (double)nx, c->mu[1]));
   TRY(csr_rowsum(s, XCt.n_rows, XCt.rowptr, XCt.vals, 1.0 / (double)nx, c->mu[2]));
   c->log("whiten: randomized SVD of P_BC (n_B=%lld n_C=%lld r=%d q=%d)\n", (long long)nb, (long long)ncn, r, cf.power_iters);
+  HCHK(hipStreamSynchronize(s));
+  double t_mark = now_s();
   // range of P_BC
   TRY(gaussian_fill(s, cf.seed, 0, ncn, r, T[0], ld));
   TRY(spmm(c, XC, 1.f, T[0], ld, r, T[1], ld));
@@ -446,6 +454,8 @@ static int whiten_impl(mmsb_ctx* c) {
   TRY(scale_cols<float>(s, nb, k, c->U, kp, c->sgn));
   TRY(scale_cols<float>(s, ncn, k, c->V, kp, c->sgn));
   TRY(pinv_scale(s, k, c->sig, 0.0, cf.pinv_tol, 0, c->sinv, nullptr));
+  HCHK(hipStreamSynchronize(s));
+  c->t_svd[0] = now_s() - t_mark;
   // L_B, L_C
   c->log("whiten: Z_B / Z_C factors\n");
   TRY(spmm(c, XC, 1.f, c->V, kp, k, T[1], ld));
@@ -456,6 +466,8 @@ static int whiten_impl(mmsb_ctx* c) {
   TRY(scale_cols<float>(s, na, k, c->LC, kp, c->sinv));
   // M2: range finder, Rayleigh-Ritz, W
   c->log("whiten: M2 eigenpairs (n_A=%lld)\n", (long long)na);
+  HCHK(hipStreamSynchronize(s));
+  t_mark = now_s();
   TRY(gaussian_fill(s, cf.seed, 1, na, r, T[0], ld));
   TRY(m2_apply(c, T[0], T[2]));
   TRY(orth(c, T[2], na, r, ld, T[2], T[3], nullptr, 4, cf.power_iters > 0));
@@ -478,6 +490,8 @@ static int whiten_impl(mmsb_ctx* c) {
   TRY(scale_cols<float>(s, na, k, c->W, kp, c->lamis));
   TRY(col_sign<float>(s, na, k, c->W, kp, c->best, c->sgn));
   TRY(scale_cols<float>(s, na, k, c->W, kp, c->sgn));
+  HCHK(hipStreamSynchronize(s));
+  c->t_svd[1] = now_s() - t_mark;
   // neighbour vectors (pre_proc.c:339-366)
   // y rows are G_X* rows times W, Z_B^T W = U (L_B^T W) and Z_C^T W = V (L_C^T W)
   c->log(window ? "whiten: neighbour-vector factors (rows recomputed per STGD minibatch)\n" : "whiten: neighbour vectors\n");
@@ -580,15 +594,19 @@ static int ensure_frob(mmsb_ctx* c, int64_t more) {
 
 static StgdArgs stgd_args(const mmsb_ctx* c) {
   StgdArgs a{};
+  a.k = c->k;
   a.kp = c->kp;
   a.b = c->b;
   a.bl = c->bl;
   a.n_x = c->n[0];
   a.n_pad = c->n_pad;
-  const double a0 = c->cfg.alpha_0;
-  a.c3 = 1.0f;
-  a.c1 = (float)(-a0 / (a0 + 2.0));
-  a.c0 = (float)(2.0 * a0 * a0 / ((a0 + 1.0) * (a0 + 2.0)));
+  // the T of the same reference path as the M2 that whitened the vectors: paper M2 with the paper
+  // coefficients (CPU Util.cpp:289-298), GPU M2 with init_buff's (stpm.c:1026-1038)
+  double co[3];
+  (void)mmsb_tensor_coeffs(c->cfg.alpha_0, c->cfg.m2_scaling, co);
+  a.c3 = (float)co[0];
+  a.c1 = (float)co[1];
+  a.c0 = (float)co[2];
   a.eta = (float)c->cfg.learn_rate;
   a.theta = (float)c->cfg.theta;
   a.mu_a = c->muw[0];
@@ -596,6 +614,7 @@ static StgdArgs stgd_args(const mmsb_ctx* c) {
   a.mu_c = c->muw[2];
   a.cs_part = c->cs_part;
   a.mab = c->mab;
+  a.frob_pre = c->cfg.renorm_mode == MMSB_RENORM_QR;  // the reference logs before its QR
   return a;
 }
 
@@ -634,13 +653,32 @@ static void y_ptrs(const mmsb_ctx* c, int32_t precision, const void*& ya, const 
   yc = bf ? (const void*)c->YCt16 : (const void*)c->YCt;
 }
 
-static int run_stgd_steps(mmsb_ctx* c, int64_t nsteps, int32_t precision, StgdTiming* timing = nullptr) {
+// The end test of the reference loop (stpm.c:747-765) after global step t with log value f.
+static bool stop_rule(const stpm_config& cf, int64_t t, double f) {
+  return (cf.sgd_iter_min > 0 && f < cf.thresh && t > cf.sgd_iter_min) || (cf.sgd_iter_max > 0 && t > cf.sgd_iter_max);
+}
+
+// Runs nsteps steps. With `rule`, the end test of the reference loop is applied after every step
+// (on the device within a launch, on the host at the end of each launch) and the run stops after the
+// first step that meets it; *stopped reports that.
+static int run_stgd_steps(mmsb_ctx* c, int64_t nsteps, int32_t precision, StgdTiming* timing = nullptr,
+                          bool rule = false, bool* stopped = nullptr) {
+  if (stopped) *stopped = false;
   if (nsteps <= 0) return MMSB_OK;
   if (precision != MMSB_PREC_FP32 && precision != MMSB_PREC_BF16)
     return set_error(MMSB_ERR_ARG, "stgd: unknown precision %d", precision);
   if (precision == MMSB_PREC_BF16) TRY(prepare_bf16(c));
   TRY(ensure_frob(c, nsteps));
+  const bool qr = c->cfg.renorm_mode == MMSB_RENORM_QR;
+  if (qr) TRY(alloc_small(c));
   StgdArgs a = stgd_args(c);
+  if (rule) {
+    TRY(dalloc(c, c->stopctl, 1));
+    a.stop = c->stopctl;
+    a.thresh = c->cfg.thresh;
+    a.iter_min = c->cfg.sgd_iter_min;
+    a.iter_max = c->cfg.sgd_iter_max;
+  }
   const StgdBuffers bufs = stgd_bufs(c);
   const void *ya, *yb, *yc;
   y_ptrs(c, precision, ya, yb, yc);
@@ -654,22 +692,32 @@ static int run_stgd_steps(mmsb_ctx* c, int64_t nsteps, int32_t precision, StgdTi
     yc = bf ? (const void*)c->yw16[2] : (const void*)c->yw[2];
     fill = [c, bf](int64_t row0, int32_t nb) { return fill_window(c, bf, row0, nb); };
   }
+  StgdRenorm renorm;
+  if (qr)  // CholeskyQR2 of Phi~ (k x k, FP64 Gram and Cholesky): R with a positive diagonal, as Dgeqrfp
+    renorm = [c](const float* src, float* dst) {
+      return orth(c, const_cast<float*>(src), c->k, c->k, c->kp, dst, c->nsq, nullptr, 0);
+    };
   const int64_t spe = (c->n[0] + c->bl - 1) / c->bl;
   auto launch = [&](int64_t step0, int64_t ns, double* frob_out, int& cur, StgdTiming* tm) -> int {
     return stgd_run(c->s, precision, a, ya, yb, yc, step0, ns, c->cfg.renorm_every, c->cfg.ns_iters, c->cfg.ns_degree, bufs,
-                    cur, frob_out, tm, fill);
+                    cur, frob_out, tm, fill, renorm);
   };
   // Whole epochs that start at an epoch boundary replay a captured HIP graph of the epoch's
-  // launches (one graph per precision and Phi ping-pong parity); anything else launches directly.
+  // launches (one graph per precision, Phi ping-pong parity and end-test setting); anything else
+  // launches directly.
   const bool renorm_aligned = c->cfg.renorm_every <= 1 || spe % c->cfg.renorm_every == 0;
+  if (rule) TRY(stgd_set_i64(c->s, &c->stopctl->stop_at, INT64_MAX));
   int64_t done = 0;
   while (done < nsteps) {
     const int64_t step0 = c->steps_done + done;
+    const int cur0 = c->cur;
+    int64_t ns = 0;
+    if (rule) TRY(stgd_set_i64(c->s, &c->stopctl->base, step0));
     const bool epoch_aligned =
         (step0 % spe == 0) && (nsteps - done >= spe) && renorm_aligned && c->cfg.use_graphs && !timing;
     if (epoch_aligned) {
       TRY(dalloc(c, c->frob_epoch, (size_t)spe));
-      hipGraphExec_t& ge = c->epoch_graph[precision][c->cur];
+      hipGraphExec_t& ge = c->epoch_graph[precision][c->cur][rule ? 1 : 0];
       if (!ge) {
         hipGraph_t g = nullptr;
         int cur = c->cur;
@@ -687,10 +735,33 @@ static int run_stgd_steps(mmsb_ctx* c, int64_t nsteps, int32_t precision, StgdTi
       HCHK(hipGraphLaunch(ge, c->s));
       HCHK(hipMemcpyAsync(c->frob + step0, c->frob_epoch, sizeof(double) * spe, hipMemcpyDeviceToDevice, c->s));
       if (spe & 1) c->cur ^= 1;
-      done += spe;
+      ns = spe;
     } else {
-      const int64_t ns = std::min<int64_t>(nsteps - done, spe - step0 % spe);
+      ns = std::min<int64_t>(nsteps - done, spe - step0 % spe);
       TRY(launch(step0, ns, c->frob + step0, c->cur, timing));
+    }
+    if (rule) {  // where did the launch stop, and does its last step end the loop?
+      StgdStop hs{};
+      HCHK(hipMemcpyAsync(&hs, c->stopctl, sizeof(hs), hipMemcpyDeviceToHost, c->s));
+      HCHK(hipStreamSynchronize(c->s));
+      int64_t ran = ns;
+      if (hs.stop_at < step0 + ns) {
+        ran = std::max<int64_t>(0, hs.stop_at - step0);
+        c->cur = cur0 ^ (int)(ran & 1);
+      }
+      done += ran;
+      if (ran < ns) {
+        if (stopped) *stopped = true;
+        break;
+      }
+      double last = 0.0;
+      HCHK(hipMemcpyAsync(&last, c->frob + step0 + ns - 1, sizeof(double), hipMemcpyDeviceToHost, c->s));
+      HCHK(hipStreamSynchronize(c->s));
+      if (stop_rule(c->cfg, step0 + ns - 1, last)) {
+        if (stopped) *stopped = true;
+        break;
+      }
+    } else {
       done += ns;
     }
     if (done < nsteps) {  // a stop request ends the run at this epoch boundary
@@ -699,6 +770,7 @@ static int run_stgd_steps(mmsb_ctx* c, int64_t nsteps, int32_t precision, StgdTi
     }
   }
   c->steps_done += done;
+  if (qr) TRY(check_status(c, "QR renormalisation"));
   return MMSB_OK;
 }
 
@@ -1108,6 +1180,59 @@ static std::string out_name(const stpm_config& cf, const char* stem) {
   return std::string(cf.folder_name) + stem + "_" + std::to_string(cf.ncom) + "_" + cf.alpha_0_str + "_" + cf.learn_rate_str + ".txt";
 }
 
+// The CPU program's two result files (CommunityDetection.cpp:121-144, write_pi IOfn.cpp:57-67):
+// pi_path gets the k x (n_X + n_A + n_B + n_C) matrix of normalised memberships (the partitions
+// concatenated in the order X, A, B, C, each column a node's normProbVector) and white_path the
+// n_A x k whitening matrix W, both as "row+1 \t col+1 \t value" triplets of the nonzeros in
+// column-major order with 6 significant digits. A null path skips that file. The normalised Pi is
+// fetched in slabs of rows, so the host holds one slab, not the whole matrix.
+static int write_triplets_impl(mmsb_ctx* c, const char* pi_path, const char* white_path) {
+  const int32_t k = c->k;
+  if (pi_path) {
+    if (!c->post_done) return set_error(MMSB_ERR_STATE, "write_triplets: Pi not computed (call mmsb_post)");
+    FILE* f = std::fopen(pi_path, "w");
+    if (!f) return set_error(MMSB_ERR_IO, "writing %s failed", pi_path);
+    int64_t col0 = 0;
+    int st = MMSB_OK;
+    for (int p = 0; p < 4 && st == MMSB_OK; ++p) {
+      View v = view_of(c, MMSB_Q_PIN_X + p);
+      if (!v.p) continue;
+      const int64_t rows = v.rows, slab = std::max<int64_t>(1, ((int64_t)64 << 20) / ((int64_t)k * 8));
+      std::vector<double> h;
+      for (int64_t r0 = 0; r0 < rows && st == MMSB_OK; r0 += slab) {
+        View w = v;
+        w.rows = std::min(slab, rows - r0);
+        w.p = static_cast<const float*>(v.p) + r0 * v.ld;
+        h.resize((size_t)w.rows * k);
+        st = copy_out(c, w, h.data());
+        for (int64_t i = 0; i < w.rows && st == MMSB_OK; ++i)
+          for (int32_t j = 0; j < k; ++j) {
+            const double x = h[(size_t)i * k + j];
+            if (x != 0.0) std::fprintf(f, "%d\t%lld\t%g\n", j + 1, (long long)(col0 + r0 + i + 1), x);
+          }
+      }
+      col0 += rows;
+    }
+    if (std::fclose(f) != 0 && st == MMSB_OK) st = set_error(MMSB_ERR_IO, "closing %s failed", pi_path);
+    TRY(st);
+  }
+  if (white_path) {
+    if (!c->whitened) return set_error(MMSB_ERR_STATE, "write_triplets: W not computed (call mmsb_whiten)");
+    const View v = view_of(c, MMSB_Q_W);
+    std::vector<float> h((size_t)v.rows * k);
+    TRY(copy_raw(c, v, h.data()));
+    FILE* f = std::fopen(white_path, "w");
+    if (!f) return set_error(MMSB_ERR_IO, "writing %s failed", white_path);
+    for (int32_t j = 0; j < k; ++j)
+      for (int64_t a = 0; a < v.rows; ++a) {
+        const double x = h[(size_t)a * k + j];
+        if (x != 0.0) std::fprintf(f, "%lld\t%d\t%g\n", (long long)a + 1, j + 1, x);
+      }
+    if (std::fclose(f) != 0) return set_error(MMSB_ERR_IO, "closing %s failed", white_path);
+  }
+  return MMSB_OK;
+}
+
 static int set_sizes(mmsb_ctx* c) {
   const stpm_config& cf = c->cfg;
   c->k = cf.ncom;
@@ -1168,6 +1293,15 @@ extern "C" void stpm_config_default(stpm_config* c) {
   c->use_graphs = 1;
 }
 
+// Live contexts: the process-wide staging and sort temporaries are released with the last one.
+static std::atomic<int> g_live_contexts{0};
+
+static void release_process_resources() {
+  release_staging();
+  release_streamer();
+  release_sort_scratch();
+}
+
 extern "C" int mmsb_create(const stpm_config* cfg, mmsb_ctx** out) {
   clear_error();
   MMSB_CHECK_ARG(cfg && out, "create: null argument");
@@ -1190,6 +1324,7 @@ extern "C" int mmsb_create(const stpm_config* cfg, mmsb_ctx** out) {
     return set_error(MMSB_ERR_HIP, "create: %s", hipGetErrorString(e));
   }
   *out = c;
+  g_live_contexts.fetch_add(1);
   return MMSB_OK;
 }
 
@@ -1207,12 +1342,14 @@ extern "C" void mmsb_destroy(mmsb_ctx* c) {
   if (!c) return;
   (void)hipSetDevice(c->cfg.device);
   if (c->s) (void)hipStreamSynchronize(c->s);
-  for (auto& row : c->epoch_graph)
-    for (hipGraphExec_t& g : row)
-      if (g) (void)hipGraphExecDestroy(g);
+  for (auto& plane : c->epoch_graph)
+    for (auto& row : plane)
+      for (hipGraphExec_t& g : row)
+        if (g) (void)hipGraphExecDestroy(g);
   for (void* p : c->allocs) (void)hipFree(p);
   if (c->s) (void)hipStreamDestroy(c->s);
   delete c;
+  if (g_live_contexts.fetch_sub(1) == 1) release_process_resources();
 }
 
 extern "C" int mmsb_set_block(mmsb_ctx* c, int32_t which, const mmsb_csr* b) {
@@ -1373,25 +1510,26 @@ extern "C" int mmsb_stgd(mmsb_ctx* c, int64_t steps, int32_t precision) {
   if (steps >= 0) {
     TRY(run_stgd_steps(c, steps, precision));
   } else {
+    // The configured epochs, each ended early by the reference's test after any step (stpm.c:747-765:
+    // ||dPhi||^2 < THRESH after more than SGD_ITER_MIN steps, or more than SGD_ITER_MAX steps) or by a
+    // stop request at an epoch boundary (stpm.c:754).
     const stpm_config& cf = c->cfg;
     const int64_t spe = (c->n[0] + c->bl - 1) / c->bl;
-    const int64_t min_steps = cf.sgd_iter_min > 0 ? cf.sgd_iter_min : (int64_t)cf.epochs * spe;
+    const bool rule = cf.sgd_iter_min > 0 || cf.sgd_iter_max > 0;
     for (int32_t e = 0; e < cf.epochs; ++e) {
       const bool low = precision == MMSB_PREC_BF16 && e < cf.epochs - cf.polish_epochs;
-      TRY(run_stgd_steps(c, spe, low ? MMSB_PREC_BF16 : MMSB_PREC_FP32));
-      if (e + 1 < cf.epochs) {  // stpm.c:754: a stop request ends the loop like convergence
+      bool stopped = false;
+      TRY(run_stgd_steps(c, spe, low ? MMSB_PREC_BF16 : MMSB_PREC_FP32, nullptr, rule, &stopped));
+      if (stopped) {
+        c->log("STGD loop ended after step %lld\n", (long long)c->steps_done);
+        break;
+      }
+      if (e + 1 < cf.epochs) {
         HCHK(hipStreamSynchronize(c->s));
         if (c->stop_req.load()) {
           c->log("Exiting STPM loop now\n");
           break;
         }
-      }
-      if (cf.sgd_iter_max > 0 && c->steps_done >= cf.sgd_iter_max) break;          // stpm.c:761
-      if (c->steps_done >= min_steps) {
-        double last = 0.0;
-        HCHK(hipMemcpyAsync(&last, c->frob + c->steps_done - 1, sizeof(double), hipMemcpyDeviceToHost, c->s));
-        HCHK(hipStreamSynchronize(c->s));
-        if (last < cf.thresh) break;                                                     // stpm.c:747
       }
     }
   }
@@ -1689,6 +1827,13 @@ extern "C" int mmsb_get_f32(mmsb_ctx* c, int32_t what, float* host, int64_t coun
   return copy_raw(c, v, host);
 }
 
+extern "C" int mmsb_write_triplets(mmsb_ctx* c, const char* pi_path, const char* white_path) {
+  clear_error();
+  MMSB_CHECK_ARG(c, "write_triplets: null context");
+  HCHK(hipSetDevice(c->cfg.device));
+  return write_triplets_impl(c, pi_path, white_path);
+}
+
 extern "C" int mmsb_sync(mmsb_ctx* c) {
   clear_error();
   MMSB_CHECK_ARG(c, "sync: null context");
@@ -1725,6 +1870,9 @@ extern "C" int mmsb_write_outputs(mmsb_ctx* c) {
     TRY(copy_out(c, view_of(c, MMSB_Q_FROB), h.data()));
     TRY(mmsb_write_values(out_name(cf, "frob_norm").c_str(), h.data(), c->steps_done));
   }
+  // the CPU program's normalised-Pi and W triplet files (CommunityDetection.cpp:141-142)
+  TRY(write_triplets_impl(c, c->post_done ? out_name(cf, "pi").c_str() : nullptr,
+                          c->whitened ? out_name(cf, "white").c_str() : nullptr));
   const std::string path = out_name(cf, "sim_info");  // stpm.c:957-975
   FILE* f = std::fopen(path.c_str(), "w");
   if (!f) return set_error(MMSB_ERR_IO, "writing %s failed", path.c_str());
@@ -1741,7 +1889,8 @@ extern "C" int mmsb_write_outputs(mmsb_ctx* c) {
   std::fprintf(f, "Total number of Frob Norm values written to file = %d\n", (int)c->steps_done);
   std::fprintf(f, "Number of iterations between convergence test    = %d\n", 1);
   std::fprintf(f, "Maximum number of stochastic iterations allowed  = %d\n", (int)cf.sgd_iter_max);
-  std::fprintf(f, "Minibatch size = %d\n", c->bl);
+  std::fprintf(f, "Exec Time svd1  = %5.25e (Seconds)\n", c->t_svd[0]);
+  std::fprintf(f, "Exec Time svd2  = %5.25e (Seconds)\n", c->t_svd[1]);
   std::fprintf(f, "Exec Time Pre Proc  = %5.25e (Seconds)\n", c->times[0]);
   std::fprintf(f, "Exec Time STPM      = %5.25e (Seconds)\n", c->times[1]);
   std::fprintf(f, "Exec Time Post Proc = %5.25e (Seconds)\n", c->times[2]);
@@ -1787,6 +1936,14 @@ extern "C" int post_proc(void) {
 extern "C" void stpm_force_exit(void) {
   mmsb_ctx* c = g_ctx;
   if (c) c->stop_req.store(1);
+}
+
+extern "C" void mmsb_release(void) {
+  if (g_ctx) {
+    mmsb_destroy(g_ctx);
+    g_ctx = nullptr;
+  }
+  release_process_resources();
 }
 
 extern "C" void* GPU_Task(void*) {

@@ -17,6 +17,8 @@
 // loop is unrolled so that several independent B-row gathers are in flight per wave -- the
 // kernel is HBM-bound (each nonzero moves 4r bytes of B) and needs that memory-level
 // parallelism to approach the bandwidth roofline.
+#include <atomic>
+
 #include "common.h"
 #include "dense.h"
 
@@ -24,8 +26,9 @@
 namespace mmsb {
 
 // Largest gathered matrix B (bytes) for which the whole-matrix and minibatch-rows SpMMs cut wide
-// rows into 256-column strips.
-static constexpr int64_t kStripBytes = (int64_t)8 << 30;
+// rows into 256-column strips: 8 GiB unless mmsb_set_strip_limit sets another (process-wide).
+static constexpr int64_t kStripBytesDefault = (int64_t)8 << 30;
+static std::atomic<int64_t> g_strip_bytes{kStripBytesDefault};
 
 // Gathers one output row: acc[p] (on lanes of group 0) = sum over the row's nonzeros of v * B[c, :],
 // float4 chunk q = gl + p * LPG. Shared by the whole-matrix kernel and the minibatch-rows kernel, so
@@ -171,13 +174,13 @@ int spmm_f32(hipStream_t s, int32_t m, int32_t n, int64_t nnz, const int32_t* ro
   if (r4 <= 8) return launch_spmm<8, 1>(s, m, rowptr, colidx, vals, alpha, B, ldb, r, Y, ldy, order);
   if (r4 <= 16) return launch_spmm<16, 1>(s, m, rowptr, colidx, vals, alpha, B, ldb, r, Y, ldy, order);
   if (r4 <= 32) return launch_spmm<32, 1>(s, m, rowptr, colidx, vals, alpha, B, ldb, r, Y, ldy, order);
-  // Wider than 256 columns and B within kStripBytes: 256-column strips, one wave per (row, strip)
+  // Wider than 256 columns and B within the strip limit: 256-column strips, one wave per (row, strip)
   // (grid.y = strips), twice or four times the waves of one wave per row with 2-4 float4 chunks per
   // lane (1M x 1M blocks: r = 510 7.7 -> 6.9 ms, r = 1010 16.5 -> 15.7 ms). On a larger B each strip
   // pass touches every row's pages again, and the strips lose (4M rows, r = 1010: 129.5 -> 137.4
   // ms), so there a wave keeps the whole row. Every form sums a row's nonzeros in the same order
   // (LPG = 64: one nonzero after another), so Y does not depend on the choice.
-  if ((int64_t)n * ldb * (int64_t)sizeof(float) <= kStripBytes)
+  if ((int64_t)n * ldb * (int64_t)sizeof(float) <= g_strip_bytes.load(std::memory_order_relaxed))
     return launch_spmm<64, 1>(s, m, rowptr, colidx, vals, alpha, B, ldb, r, Y, ldy, order);
   if (r4 <= 128) return launch_spmm<64, 2>(s, m, rowptr, colidx, vals, alpha, B, ldb, r, Y, ldy, order);
   if (r4 <= 192) return launch_spmm<64, 3>(s, m, rowptr, colidx, vals, alpha, B, ldb, r, Y, ldy, order);
@@ -274,7 +277,7 @@ int spmm_yrows(hipStream_t s, const YRowsJobs& jobs, int32_t nb, int32_t r, int3
   if (r4 <= 32) return launch_yrows<32, 1>(s, jobs, nb, r, ldr, bf16);
   int64_t rmax = 0;  // the largest gathered matrix decides, as in spmm_f32
   for (int32_t i = 0; i < jobs.count; ++i) rmax = std::max<int64_t>(rmax, jobs.j[i].rows_R);
-  if (rmax * ldr * (int64_t)sizeof(float) <= kStripBytes) return launch_yrows<64, 1>(s, jobs, nb, r, ldr, bf16);
+  if (rmax * ldr * (int64_t)sizeof(float) <= g_strip_bytes.load(std::memory_order_relaxed)) return launch_yrows<64, 1>(s, jobs, nb, r, ldr, bf16);
   if (r4 <= 128) return launch_yrows<64, 2>(s, jobs, nb, r, ldr, bf16);
   if (r4 <= 192) return launch_yrows<64, 3>(s, jobs, nb, r, ldr, bf16);
   if (r4 <= 256) return launch_yrows<64, 4>(s, jobs, nb, r, ldr, bf16);
@@ -299,4 +302,9 @@ extern "C" int mmsb_spmm_f32_ordered(void* stream, int32_t m, int32_t n, int64_t
   MMSB_CHECK_ARG(row_order, "spmm_ordered: null row order");
   return mmsb::spmm_f32(static_cast<hipStream_t>(stream), m, n, nnz, rowptr, colidx, vals, alpha, B,
                         ldb, r, Y, ldy, row_order);
+}
+
+extern "C" int64_t mmsb_set_strip_limit(int64_t bytes) {
+  const int64_t v = bytes < 0 ? mmsb::kStripBytesDefault : bytes;
+  return mmsb::g_strip_bytes.exchange(v);
 }

@@ -23,6 +23,7 @@
 #include "dense.h"
 #include "host_util.h"
 #include "mmsb_stpm.h"
+#include "xfer.h"
 
 namespace mmsb {
 
@@ -63,8 +64,25 @@ struct Streamer {
 };
 
 Streamer& streamer() {
-  static Streamer st;
+  static Streamer st;  // kept between calls; freed by release_streamer (mmsb_release)
   return st;
+}
+
+// Waits for both slots and frees everything the streamer holds (caller holds st.mu).
+void drop(Streamer& st) {
+  for (Slot& sl : st.slot) {
+    if (sl.used) (void)hipEventSynchronize(sl.used);
+    if (sl.copied) (void)hipEventSynchronize(sl.copied);
+    for (void* p : {(void*)sl.rowptr, (void*)sl.order, (void*)sl.colidx, (void*)sl.vals})
+      if (p) (void)hipFree(p);
+    if (sl.host) (void)hipHostFree(sl.host);
+    if (sl.copied) (void)hipEventDestroy(sl.copied);
+    if (sl.used) (void)hipEventDestroy(sl.used);
+    sl = Slot();
+  }
+  if (st.copy) (void)hipStreamDestroy(st.copy);
+  st.copy = nullptr;
+  st.device = -1;
 }
 
 int ensure_slot(Slot& sl, int64_t rows, int64_t nnz, bool vals) {
@@ -164,18 +182,7 @@ int spmm_f32_host_csr(hipStream_t s, int32_t m, int32_t n, int64_t nnz, const in
   int dev = 0;
   SHIP(hipGetDevice(&dev));
   if (st.device != dev) {  // one set of staging per process, for the current device
-    for (Slot& sl : st.slot) {
-      if (sl.used) (void)hipEventSynchronize(sl.used);
-      if (sl.copied) (void)hipEventSynchronize(sl.copied);
-      for (void* p : {(void*)sl.rowptr, (void*)sl.order, (void*)sl.colidx, (void*)sl.vals})
-        if (p) (void)hipFree(p);
-      if (sl.host) (void)hipHostFree(sl.host);
-      if (sl.copied) (void)hipEventDestroy(sl.copied);
-      if (sl.used) (void)hipEventDestroy(sl.used);
-      sl = Slot();
-    }
-    if (st.copy) (void)hipStreamDestroy(st.copy);
-    st.copy = nullptr;
+    drop(st);
     SHIP(hipStreamCreateWithFlags(&st.copy, hipStreamNonBlocking));
     st.device = dev;
   }
@@ -221,6 +228,12 @@ int spmm_f32_host_csr(hipStream_t s, int32_t m, int32_t n, int64_t nnz, const in
   }
   SHIP(hipStreamSynchronize(s));
   return MMSB_OK;
+}
+
+void release_streamer() {
+  Streamer& st = streamer();
+  std::lock_guard<std::mutex> lock(st.mu);
+  drop(st);
 }
 
 }  // namespace mmsb

@@ -9,14 +9,22 @@
 
 namespace mmsb {
 
+// Device state of the STGD loop's end test (stpm.c:747-765): steps with a global index >= stop_at are
+// skipped; base is the global index of the current run's first step (written before each run).
+struct StgdStop {
+  int64_t stop_at;
+  int64_t base;
+};
+
 // Arguments shared by the phase kernels of one STGD step; passed by value to every launch.
 struct StgdArgs {
+  int32_t k;            // communities
   int32_t kp;           // k padded to a multiple of 64
   int32_t b;            // minibatch tile rows (multiple of 64, >= bl)
   int32_t bl;           // logical minibatch: step s covers rows [s*bl, s*bl + bl) of X
   int64_t n_x;          // nodes in X
   int64_t n_pad;        // rows of Y_A/Y_B and columns of Y_C^T (>= (ceil(n_x / bl) - 1) * bl + b; >= b with y_window)
-  float c3, c1, c0;     // tensor coefficients (GPU scaling of init_buff)
+  float c3, c1, c0;     // tensor coefficients (of the reference path whose M2 scaling whitened Y)
   float eta, theta;     // LEARN_RATE and the weight of the orthogonality term
   const double* mu_a;   // [kp] whitened means
   const double* mu_b;
@@ -27,6 +35,12 @@ struct StgdArgs {
   float* mab;           // [2 x kp] m_A, m_B
   int32_t y_window;     // 0: Y holds all of X; 1: Y holds the current minibatch only (rows from 0), refilled
                         // before each step by the caller's fill callback (checkpointed STGD)
+  int32_t frob_pre;     // 1: log ||Phi~ - Phi||^2 before the renormalisation (the reference's order,
+                        // stpm.c:705-743 then the QR at :774-786); 0: ||Phi_next - Phi||^2 after it
+  StgdStop* stop;       // null: every step runs; else the end test below, checked per step on the device
+  double thresh;        // THRESH: converged when ||dPhi||^2 < thresh and the step index > iter_min
+  int64_t iter_min;     // SGD_ITER_MIN (<= 0: the convergence test is off)
+  int64_t iter_max;     // SGD_ITER_MAX: the last step is the first with index > iter_max (<= 0: off)
 };
 
 // Device buffers of a run (all [kp x kp] FP32 row-major unless noted). The *T buffers hold the
@@ -75,11 +89,17 @@ struct StgdTiming {
 // launch is capturable in a HIP graph (no memset nodes, no host synchronisation).
 // With args.y_window, fill(row0, nb) is called before each step's first launch to enqueue (on s) the
 // kernels that write rows [row0, row0 + nb) of X into the Y buffers.
+// With a renorm callback (the reference's QR mode), a renormalising step runs P1, P2, then
+// renorm(src = bufs.phit, dst = bufs.phia) -- launches that leave the orthonormal factor of Phi~ in
+// phia -- and a QRFIN kernel that makes it the next Phi (with its derived copies).
 using StgdFill = std::function<int(int64_t row0, int32_t nb)>;
+using StgdRenorm = std::function<int(const float* src, float* dst)>;
 int stgd_run(hipStream_t s, int32_t precision, const StgdArgs& args, const void* YA, const void* YB, const void* YCt,
              int64_t step0, int64_t nsteps, int32_t renorm_every, int32_t ns_iters, int32_t ns_degree,
              const StgdBuffers& bufs, int& cur, double* frob_log, StgdTiming* timing = nullptr,
-             const StgdFill& fill = StgdFill());
+             const StgdFill& fill = StgdFill(), const StgdRenorm& renorm = StgdRenorm());
+// Sets *p = v on stream s (one thread; capturable, unlike a memset node).
+int stgd_set_i64(hipStream_t s, int64_t* p, int64_t v);
 
 // ---- many instances in one persistent launch ----------------------------------------------------
 // One member of a batch: its step arguments, buffers and neighbour vectors (FP32 or BF16 as the

@@ -426,10 +426,10 @@ __device__ inline void store4(T* p, const float v[4]) {
 // ---- the phases of one step, as per-tile device functions ----------------------------------------
 // A step is the phase sequence P1, P2, then ns_iters x (GRAM, UPDATE) -- (GRAM, QPOLY, UPDATE) for
 // degree-5 iterations -- or COPY when the step does not renormalise; SETUP precedes the first step of
-// a run. The same tile functions serve the per-phase
+// a run. In the reference's QR mode a renormalising step is P1, P2, the host's QR launches and QRFIN. The same tile functions serve the per-phase
 // launches of one instance (stgd_phase_kernel, blockIdx.x = tile) and the persistent launch over many
 // instances (stgd_batch_kernel, tiles from a work queue).
-enum StgdPhase { PH_SETUP = 0, PH_P1 = 1, PH_P2 = 2, PH_GRAM = 3, PH_UPDATE = 4, PH_COPY = 5, PH_QPOLY = 6 };
+enum StgdPhase { PH_SETUP = 0, PH_P1 = 1, PH_P2 = 2, PH_GRAM = 3, PH_UPDATE = 4, PH_COPY = 5, PH_QPOLY = 6, PH_QRFIN = 7 };
 
 // Everything the phases of one step of one instance read or write.
 struct StepCtx {
@@ -444,6 +444,8 @@ struct StepCtx {
   int64_t row0;        // first X row of the minibatch
   int64_t yrow0;       // its first row in the Y buffers (row0, or 0 when they hold only the minibatch)
   int32_t nb;          // rows in the minibatch
+  int64_t st;          // step index within the run
+  const double* frob_prev;  // the previous step's log entry within the run (null for the run's first)
   double* frob;        // this step's ||Phi_{t+1} - Phi_t||_F^2
   double* frob_zero;   // SETUP: the run's frob entries, cleared by tile 0
   int64_t nfrob;
@@ -691,8 +693,11 @@ __device__ void p2_tile(const StepCtx& P, int tile, StgdSmem& sm) {
     const float t = red_get(red, 2, r, c) + red_get(red, 3, r, c);
     v[e] = p0[e] - eta * a.theta * phh + eta * (t * inv_b + muc * sm.p2.cmu[c]);
   }
+  TilePre pre{};  // the reference's log: ||Phi~ - Phi||^2 before the renormalisation (frob_pre)
+#pragma unroll
+  for (int e = 0; e < 4; ++e) pre.pv[e] = p0[e];
   finish_tile<SPLIT>(sm.p2.S, kp, ti, i0, j0, v, a.phit, sp<SPLIT>(P.B.phit16), P.B.phitT, sp<SPLIT>(P.B.phitT16),
-                     TilePre{}, nullptr, nullptr);
+                     pre, a.frob_pre ? P.frob : nullptr, nullptr);
 }
 
 // The Newton-Schulz scale guard c = min(1, sqrt(2 / ||G||_inf)) from the row sums of |G|, computed
@@ -821,17 +826,23 @@ __device__ void update_tile(const StepCtx& P, int it, int tile, StgdSmem& sm) {
                      last ? P.frob : nullptr, last ? P.B.mpart : nullptr);
 }
 
-// COPY: Phi_next = Phi~ with the outputs of a last update. SETUP: the derived copies of the
-// current Phi (split pairs, Phi^T, the m partial sums), and tile 0 clears the run's frob entries.
+// COPY: Phi_next = Phi~ with the outputs of a last update; QRFIN: Phi_next = the QR factor the host
+// launches left in phia (rows and columns from k on set to zero), without a log entry (the step logged
+// before its QR). SETUP: the derived copies of the current Phi (split pairs, Phi^T, the m partial
+// sums), and tile 0 clears the run's frob entries.
 template <bool SPLIT>
-__device__ void copy_tile(const StepCtx& P, bool setup, int tile, StgdSmem& sm) {
+__device__ void copy_tile(const StepCtx& P, bool setup, int tile, StgdSmem& sm, bool qrfin = false) {
   const int kp = P.a.kp, ntk = kp / 32;
   const int ti = tile / ntk, tj = tile % ntk;
   const int64_t i0 = (int64_t)ti * 32, j0 = (int64_t)tj * 32;
   const int r = threadIdx.x >> 3, c4 = 4 * (threadIdx.x & 7);
-  const float* X = setup ? P.B.phi[P.cur] : P.B.phit;
+  const float* X = setup ? P.B.phi[P.cur] : (qrfin ? P.B.phia : P.B.phit);
   const float4 xv = *reinterpret_cast<const float4*>(X + (i0 + r) * kp + j0 + c4);
-  const float v[4] = {xv.x, xv.y, xv.z, xv.w};
+  float v[4] = {xv.x, xv.y, xv.z, xv.w};
+  if (qrfin)
+#pragma unroll
+    for (int e = 0; e < 4; ++e)
+      if (i0 + r >= P.a.k || j0 + c4 + e >= P.a.k) v[e] = 0.f;
   if (setup && tile == 0)
     for (int64_t t = threadIdx.x; t < P.nfrob; t += blockDim.x) P.frob_zero[t] = 0.0;
   TilePre pre;
@@ -841,7 +852,7 @@ __device__ void copy_tile(const StepCtx& P, bool setup, int tile, StgdSmem& sm) 
                        pre, nullptr, P.B.mpart);
   else
     finish_tile<SPLIT>(sm.S, kp, ti, i0, j0, v, P.B.phi[P.cur ^ 1], sp<SPLIT>(P.B.phi16[P.cur ^ 1]), P.B.phiT,
-                       sp<SPLIT>(P.B.phiT16), pre, P.frob, P.B.mpart);
+                       sp<SPLIT>(P.B.phiT16), pre, (qrfin || P.a.frob_pre) ? nullptr : P.frob, P.B.mpart);
 }
 
 template <int DMODE, int KMODE, typename TY, typename TPH>
@@ -854,24 +865,49 @@ __device__ void run_tile(const StepCtx& P, int phase, int it, int tile, StgdSmem
     case PH_GRAM: gram_tile<KMODE>(P, it, tile, sm); break;
     case PH_UPDATE: update_tile<KMODE>(P, it, tile, sm); break;
     case PH_QPOLY: qpoly_tile<KMODE>(P, it, tile, sm); break;
+    case PH_QRFIN: copy_tile<SPLIT>(P, false, tile, sm, true); break;
     default: copy_tile<SPLIT>(P, false, tile, sm); break;
   }
+}
+
+// The end test of the reference's STGD loop after step t (sgd_iter, 0-based) with log value f
+// (stpm.c:747-765): converged, or past the cap.
+__device__ inline bool stop_test(const StgdArgs& a, int64_t t, double f) {
+  return (a.iter_min > 0 && f < a.thresh && t > a.iter_min) || (a.iter_max > 0 && t > a.iter_max);
 }
 
 // One phase of one step, one kernel per phase (each with its own register allocation):
 // blockIdx.x = tile. (An XCD-aware order that gave each XCD a compact group of tiles cut the HBM
 // fetches of P1 from 24 to 9 MiB per launch, but the step took 2.6 % longer: the waves of a group
 // then request the same lines at the same moment.)
+// With the end test on (P.a.stop), every phase of step g = base + st first checks whether the loop
+// has ended; P1 of each step but a run's first applies the test to the previous step's log entry
+// (complete, since its last kernel has finished) and marks the end -- every workgroup reaches the same
+// decision from the same value, so the whole launch returns together. The skipped kernels write
+// nothing, and the host reads stop_at after the run.
 template <int DMODE, int KMODE, typename TY, typename TPH, int PHASE>
 __global__ __launch_bounds__(256) void stgd_phase_kernel(StepCtx P, int32_t it) {
   __shared__ StgdSmem sm;
   constexpr bool SPLIT = KMODE == MM_BF16X3;
+  if constexpr (PHASE != PH_SETUP) {
+    if (P.a.stop) {
+      const int64_t g = P.a.stop->base + P.st;
+      if constexpr (PHASE == PH_P1) {
+        if (P.frob_prev && stop_test(P.a, g - 1, *P.frob_prev)) {
+          if (threadIdx.x == 0) P.a.stop->stop_at = g;
+          return;
+        }
+      }
+      if (P.a.stop->stop_at <= g) return;
+    }
+  }
   if constexpr (PHASE == PH_SETUP) copy_tile<SPLIT>(P, true, blockIdx.x, sm);
   else if constexpr (PHASE == PH_P1) p1_tile<DMODE, KMODE, TY, TPH>(P, blockIdx.x, sm);
   else if constexpr (PHASE == PH_P2) p2_tile<DMODE, KMODE, TY, TPH>(P, blockIdx.x, sm);
   else if constexpr (PHASE == PH_GRAM) gram_tile<KMODE>(P, it, blockIdx.x, sm);
   else if constexpr (PHASE == PH_UPDATE) update_tile<KMODE>(P, it, blockIdx.x, sm);
   else if constexpr (PHASE == PH_QPOLY) qpoly_tile<KMODE>(P, it, blockIdx.x, sm);
+  else if constexpr (PHASE == PH_QRFIN) copy_tile<SPLIT>(P, false, blockIdx.x, sm, true);
   else copy_tile<SPLIT>(P, false, blockIdx.x, sm);
 }
 
@@ -918,6 +954,8 @@ __device__ inline void make_step(const BatchInst& I, int64_t st, StepCtx& P) {
   P.row0 = (g % I.spe) * I.a.bl;
   P.yrow0 = P.row0;
   P.nb = (int32_t)min((int64_t)I.a.bl, I.a.n_x - P.row0);
+  P.st = st;
+  P.frob_prev = nullptr;
   P.a.phi = I.B.phi[P.cur];
   P.a.phit = I.B.phit;
   P.frob = I.frob_log + st;
@@ -979,6 +1017,16 @@ __global__ __launch_bounds__(256) void stgd_slot_kernel(const BatchInst* insts, 
   run_tile<DMODE, KMODE, TY, TPH>(P, sg.phase, sg.it, (int)(t - sg.first), sm);
 }
 
+__global__ void set_i64_kernel(int64_t* p, int64_t v) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *p = v;
+}
+
+int stgd_set_i64(hipStream_t s, int64_t* p, int64_t v) {
+  hipLaunchKernelGGL(set_i64_kernel, dim3(1), dim3(64), 0, s, p, v);
+  MMSB_LAUNCH_CHECK();
+  return MMSB_OK;
+}
+
 __global__ void zero_f64_kernel(int64_t n, double* p) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t < n) p[t] = 0.0;
@@ -1005,7 +1053,7 @@ static bool step_renorms(int64_t gstep, int32_t renorm_every, int32_t ns_iters) 
 template <int DMODE, int KMODE, typename TY, typename TPH>
 static int run_steps(hipStream_t s, const StgdArgs& args, const void* YA, const void* YB, const void* YCt, int64_t step0,
                      int64_t nsteps, int32_t renorm_every, int32_t ns_iters, int32_t ns_degree, const StgdBuffers& B,
-                     int& cur, double* frob_log, StgdTiming* timing, const StgdFill& fill) {
+                     int& cur, double* frob_log, StgdTiming* timing, const StgdFill& fill, const StgdRenorm& renorm) {
   const int kp = args.kp;
   const int64_t spe = ceil_div(args.n_x, args.bl);
   StepCtx P{};
@@ -1025,6 +1073,7 @@ static int run_steps(hipStream_t s, const StgdArgs& args, const void* YA, const 
       case PH_GRAM: hipLaunchKernelGGL((stgd_phase_kernel<DMODE, KMODE, TY, TPH, PH_GRAM>), grid, block, 0, s, P, it); break;
       case PH_UPDATE: hipLaunchKernelGGL((stgd_phase_kernel<DMODE, KMODE, TY, TPH, PH_UPDATE>), grid, block, 0, s, P, it); break;
       case PH_QPOLY: hipLaunchKernelGGL((stgd_phase_kernel<DMODE, KMODE, TY, TPH, PH_QPOLY>), grid, block, 0, s, P, it); break;
+      case PH_QRFIN: hipLaunchKernelGGL((stgd_phase_kernel<DMODE, KMODE, TY, TPH, PH_QRFIN>), grid, block, 0, s, P, it); break;
       default: hipLaunchKernelGGL((stgd_phase_kernel<DMODE, KMODE, TY, TPH, PH_COPY>), grid, block, 0, s, P, it); break;
     }
     MMSB_LAUNCH_CHECK();
@@ -1043,6 +1092,8 @@ static int run_steps(hipStream_t s, const StgdArgs& args, const void* YA, const 
     P.nb = (int32_t)std::min<int64_t>(args.bl, args.n_x - P.row0);
     P.yrow0 = args.y_window ? 0 : P.row0;
     P.a.phi = B.phi[cur];
+    P.st = st;
+    P.frob_prev = st > 0 ? frob_log + st - 1 : nullptr;
     P.frob = frob_log + st;
     if (timing) MMSB_HIP(hipEventRecord(timing->ev[0], s));
     if (args.y_window) MMSB_TRY(fill(P.row0, P.nb));
@@ -1050,7 +1101,10 @@ static int run_steps(hipStream_t s, const StgdArgs& args, const void* YA, const 
     if (timing) MMSB_HIP(hipEventRecord(timing->ev[1], s));
     MMSB_TRY(launch(PH_P2, 0));
     if (timing) MMSB_HIP(hipEventRecord(timing->ev[2], s));
-    if (step_renorms(gstep, renorm_every, ns_iters)) {
+    if (renorm && renorm_every > 0 && (gstep + 1) % renorm_every == 0) {
+      MMSB_TRY(renorm(B.phit, B.phia));
+      MMSB_TRY(launch(PH_QRFIN, 0));
+    } else if (!renorm && step_renorms(gstep, renorm_every, ns_iters)) {
       for (int it = 0; it < ns_iters; ++it) {
         MMSB_TRY(launch(PH_GRAM, it));
         if (ns_degree == 5) MMSB_TRY(launch(PH_QPOLY, it));
@@ -1094,7 +1148,8 @@ static int check_split(const StgdBuffers& bufs) {
 
 int stgd_run(hipStream_t s, int32_t precision, const StgdArgs& args, const void* YA, const void* YB, const void* YCt,
              int64_t step0, int64_t nsteps, int32_t renorm_every, int32_t ns_iters, int32_t ns_degree,
-             const StgdBuffers& bufs, int& cur, double* frob_log, StgdTiming* timing, const StgdFill& fill) {
+             const StgdBuffers& bufs, int& cur, double* frob_log, StgdTiming* timing, const StgdFill& fill,
+             const StgdRenorm& renorm) {
   MMSB_TRY(check_args(args, ns_iters, ns_degree));
   MMSB_CHECK_ARG(nsteps >= 0, "stgd: negative step count");
   MMSB_CHECK_ARG(!args.y_window || fill, "stgd: a minibatch window needs a fill callback");
@@ -1102,11 +1157,11 @@ int stgd_run(hipStream_t s, int32_t precision, const StgdArgs& args, const void*
   if (precision == MMSB_PREC_BF16) {
     MMSB_TRY(check_split(bufs));
     return run_steps<MM_BF16, MM_BF16X3, __bf16, __bf16>(s, args, YA, YB, YCt, step0, nsteps, renorm_every, ns_iters,
-                                                         ns_degree, bufs, cur, frob_log, timing, fill);
+                                                         ns_degree, bufs, cur, frob_log, timing, fill, renorm);
   }
   MMSB_CHECK_ARG(precision == MMSB_PREC_FP32, "stgd: unknown precision %d", precision);
   return run_steps<MM_F32, MM_F32, float, float>(s, args, YA, YB, YCt, step0, nsteps, renorm_every, ns_iters, ns_degree,
-                                                 bufs, cur, frob_log, timing, fill);
+                                                 bufs, cur, frob_log, timing, fill, renorm);
 }
 
 // ---- batch ------------------------------------------------------------------------------------------
@@ -1144,6 +1199,7 @@ int stgd_batch_plan(StgdBatch*& out, int32_t precision, const std::vector<StgdBa
     const StgdBatchMember& m = members[i];
     MMSB_TRY(check_args(m.args, ns_iters, ns_degree));
     MMSB_CHECK_ARG(!m.args.y_window, "stgd batch: instance %zu recomputes its minibatches (not supported in a batch)", i);
+    MMSB_CHECK_ARG(!m.args.stop && !m.args.frob_pre, "stgd batch: instance %zu uses the per-step end test or QR mode", i);
     if (precision == MMSB_PREC_BF16) MMSB_TRY(check_split(m.bufs));
     MMSB_CHECK_ARG(m.nsteps >= 1, "stgd batch: instance %zu has no steps", i);
     BatchInst& I = insts[i];

@@ -23,14 +23,11 @@ struct Stager {
 };
 
 Stager& stager() {
-  static Stager st;  // process lifetime; released by the runtime at exit
+  static Stager st;  // kept between copies; freed by release_staging (mmsb_release)
   return st;
 }
 
-int ensure(Stager& st) {
-  int dev = 0;
-  if (hipGetDevice(&dev) != hipSuccess) return set_error(MMSB_ERR_HIP, "staging: hipGetDevice failed");
-  if (st.buf[0] && st.device == dev) return MMSB_OK;
+void drop(Stager& st) {
   for (int i = 0; i < 2; ++i) {
     if (st.ev[i]) (void)hipEventSynchronize(st.ev[i]);
     if (st.buf[i]) (void)hipHostFree(st.buf[i]);
@@ -38,6 +35,14 @@ int ensure(Stager& st) {
     st.buf[i] = nullptr;
     st.ev[i] = nullptr;
   }
+  st.device = -1;
+}
+
+int ensure(Stager& st) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return set_error(MMSB_ERR_HIP, "staging: hipGetDevice failed");
+  if (st.buf[0] && st.device == dev) return MMSB_OK;
+  drop(st);
   for (int i = 0; i < 2; ++i) {
     if (hipHostMalloc(&st.buf[i], kChunk, hipHostMallocDefault) != hipSuccess)
       return set_error(MMSB_ERR_NOMEM, "staging: hipHostMalloc(%zu) failed", kChunk);
@@ -56,6 +61,12 @@ void par_copy(void* dst, const void* src, size_t n) {
 }
 
 }  // namespace
+
+void release_staging() {
+  Stager& st = stager();
+  std::lock_guard<std::mutex> lock(st.mu);
+  drop(st);
+}
 
 int h2d(hipStream_t s, void* dst, const void* src, size_t bytes) {
   if (bytes == 0) return MMSB_OK;

@@ -190,16 +190,23 @@ enum { MMSB_M2_PAPER = 0, MMSB_M2_GPU = 1 };
  * whitening); MMSB_Y_AUTO recomputes when the stored form would not fit in 90% of the free
  * device memory. Both forms give identical STGD results. */
 enum { MMSB_Y_AUTO = 0, MMSB_Y_STORED = 1, MMSB_Y_RECOMPUTE = 2 };
+/* Orthonormal renormalisation of Phi after a step: MMSB_RENORM_NS = ns_iters Newton-Schulz
+ * iterations towards the polar factor (GEMMs only), logging ||Phi_next - Phi||^2; MMSB_RENORM_QR =
+ * the reference's QR (culaDeviceDgeqrfp + Dorgqr, stpm.c:774-786) as a k x k CholeskyQR2 (FP64 Gram and
+ * Cholesky, R with a positive diagonal), logging ||Phi~ - Phi||^2 before it as stpm.c:705-743 does. */
+enum { MMSB_RENORM_NS = 0, MMSB_RENORM_QR = 1 };
 
 typedef struct stpm_config {
   int32_t nx, na, nb, nc;     /* NX, NA, NB, NC (stpm.h:72-75)                                   */
   int32_t ncom;               /* NCOM = k (stpm.h:76)                                            */
   double alpha_0;             /* ALPHA_0 (stpm.h:78)                                             */
   double learn_rate;          /* LEARN_RATE = eta (stpm.h:79)                                    */
-  double thresh;              /* THRESH: stop when ||dPhi||_F^2 < thresh ... (stpm.h:94)          */
+  double thresh;              /* THRESH: the loop ends after the step t (from 0) whose ||dPhi||_F^2 < thresh ... (stpm.h:94) */
   double pinv_tol;            /* PINV_TOL (stpm.h:95), relative to the largest value here        */
-  int64_t sgd_iter_min;       /* ... after at least this many steps (SGD_ITER_MIN, stpm.h:96)    */
-  int64_t sgd_iter_max;       /* hard cap on steps (SGD_ITER_MAX, stpm.h:97); <= 0: epochs only  */
+  int64_t sgd_iter_min;       /* ... when t > SGD_ITER_MIN (stpm.h:96, stpm.c:747); <= 0: no convergence test */
+  int64_t sgd_iter_max;       /* or after the first step t > SGD_ITER_MAX (stpm.h:97, stpm.c:761), i.e. at most
+                                 SGD_ITER_MAX + 2 steps; <= 0: no cap. Both are checked after every minibatch
+                                 step of mmsb_stgd(ctx, -1, ...), within the configured epochs              */
   int32_t binary;             /* 1 = BINARY two-column files, 0 = EXPECTED three-column (stpm.h:56-57) */
   int32_t synthetic;          /* SYNTHETIC: G_BA/G_CA present, Pi_A/B/C produced (stpm.h:55)    */
   char folder_name[512];      /* FOLDER_NAME (stpm.h:47), with trailing '/'                       */
@@ -226,6 +233,7 @@ typedef struct stpm_config {
   int32_t verbose;            /* progress lines on stdout                                        */
   int32_t use_graphs;         /* replay whole STGD epochs as captured HIP graphs                 */
   int32_t y_mode;             /* MMSB_Y_*: neighbour vectors stored or recomputed per minibatch  */
+  int32_t renorm_mode;        /* MMSB_RENORM_*: Newton-Schulz (default) or the reference's QR    */
 } stpm_config;
 
 /* Fills the reference defaults of stpm.h:47-106 (DBLP sizes, NCOM 100, ALPHA_0 1, LEARN_RATE 1e-3,
@@ -276,7 +284,10 @@ int mmsb_set_yvectors(mmsb_ctx* ctx, int64_t n_x, int32_t k, const double* ya, c
                       const double* mua, const double* mub, const double* muc);
 /* STGD (the loop of GPU_Task, stpm.c:497-790): `steps` minibatch steps in `precision`; steps < 0
  * runs the configured epochs (BF16 epochs followed by polish_epochs in FP32 when precision is
- * BF16) with the THRESH / SGD_ITER_MIN / SGD_ITER_MAX stopping rules checked once per epoch. */
+ * BF16), ended after the first step that meets the THRESH / SGD_ITER_MIN / SGD_ITER_MAX test of
+ * stpm.c:747-765 (evaluated on the device after every step, so captured epochs keep running
+ * without host round trips). The tensor coefficients follow m2_scaling: the paper M2 with the
+ * paper T (CPU Util.cpp:289-298), the GPU M2 with init_buff's T (stpm.c:1026-1038). */
 int mmsb_stgd(mmsb_ctx* ctx, int64_t steps, int32_t precision);
 /* Changes LEARN_RATE (stpm.h:79, a compile-time constant in the reference) for the following
  * mmsb_stgd calls of this context, e.g. a different rate for polishing epochs. */
@@ -326,8 +337,15 @@ int mmsb_power_method(mmsb_ctx* ctx, int32_t iters, int32_t orthogonalize, doubl
 int mmsb_eigen(mmsb_ctx* ctx);
 /* Pi_X, Pi_A, Pi_B, Pi_C, normalisation and membership counts (post_proc.c:62-119). */
 int mmsb_post(mmsb_ctx* ctx);
-/* Writes Pi_*, evecs_, evals_, frob_norm_ and sim_info_ files named as stpm.h:82-91. */
+/* Writes Pi_*, evecs_, evals_, frob_norm_ and sim_info_ files named as stpm.h:82-91 (sim_info line for
+ * line as stpm.c:957-975, svd1 / svd2 timing the randomized SVD of P_BC and the M2 eigenproblem), plus
+ * the CPU program's pi_ and white_ triplet files of mmsb_write_triplets under the same naming. */
 int mmsb_write_outputs(mmsb_ctx* ctx);
+/* The CPU program's result files (CommunityDetection.cpp:121-144, write_pi IOfn.cpp:57-67): pi_path
+ * receives the normalised memberships of all nodes, partitions concatenated X, A, B, C (k x N, each
+ * node's normProbVector), white_path the n_A x k whitening matrix W; both as "row\tcol\tvalue"
+ * triplets (1-based) of the nonzeros in column-major order, 6 significant digits. NULL skips a file. */
+int mmsb_write_triplets(mmsb_ctx* ctx, const char* pi_path, const char* white_path);
 /* Copies a quantity to host memory as double; count is the number of doubles available. */
 int mmsb_get(mmsb_ctx* ctx, int32_t what, double* host, int64_t count);
 /* Rows [row0, row0 + nrows) of a row-indexed quantity (tall matrices, vectors, counts). Y_A, Y_B
@@ -340,6 +358,17 @@ int mmsb_get_f32(mmsb_ctx* ctx, int32_t what, float* host, int64_t count);
 int64_t mmsb_get_size(mmsb_ctx* ctx, int32_t what);
 /* Blocks until all work of the context has finished. */
 int mmsb_sync(mmsb_ctx* ctx);
+/* Releases the process-wide resources the library keeps between calls -- the pinned staging of large
+ * copies, the host-CSR SpMM's staging, slots and copy stream, the CSR sort temporaries -- and the
+ * context of the program-level entry points (pre_proc / GPU_Task / post_proc). No counterpart in the
+ * reference. Call it before the HIP runtime is torn down (the Python binding does so at interpreter
+ * exit); mmsb_destroy of the last live context releases the same resources. Later calls re-create
+ * them on demand. */
+void mmsb_release(void);
+/* Largest gathered matrix (bytes, n x ldb FP32) for which the SpMM kernels cut rows wider than 256
+ * columns into 256-column strips (default 8 GiB; bytes < 0 restores it); returns the previous limit.
+ * Process-wide; the results do not depend on it. No counterpart in the reference. */
+int64_t mmsb_set_strip_limit(int64_t bytes);
 
 /* ---- program-level drop-in (stpm.c / pre_proc.c / post_proc.c) -------------------------- */
 /* Sets the configuration that pre_proc / GPU_Task / post_proc use (the stpm.h macros). */

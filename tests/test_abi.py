@@ -79,3 +79,17 @@ def test_bench_refuses_more_than_one_gpu():
                        timeout=300)
     assert r.returncode == 2
     assert "one GPU" in (r.stdout + r.stderr)
+
+
+def test_release_and_strip_limit_without_a_gpu(mmsb):
+    """mmsb_release with nothing allocated touches no device and may be called repeatedly; the strip
+    limit setter returns the previous value and restores the 8 GiB default for a negative argument."""
+    mmsb.lib.mmsb_release()
+    mmsb.lib.mmsb_release()
+    prev = mmsb.lib.mmsb_set_strip_limit(12345)
+    assert mmsb.lib.mmsb_set_strip_limit(-1) == 12345
+    assert mmsb.lib.mmsb_set_strip_limit(prev) == 8 << 30
+
+
+def test_renorm_mode_field_round_trips(mmsb):
+    assert mmsb.config().renorm_mode == 0 and mmsb.config(renorm_mode="qr").renorm_mode == 1

@@ -376,11 +376,18 @@ def test_batch_matches_single_instances(mmsb, prec, degree):
     grid, ms = batch.autotune()
     assert grid >= 0 and ms > 0          # 0: grouped per-slot launches won
     batch.run(1)
-    for (n, k, b, a0, eta), p_single, p_batch in zip(specs, singles, members):
+    for idx, ((n, k, b, a0, eta), p_single, p_batch) in enumerate(zip(specs, singles, members)):
         spe = -(-n // b)
         p_single.stgd(3 * spe, prec)
         assert np.array_equal(p_single.get("phi"), p_batch.get("phi")), (n, k, b)
         np.testing.assert_allclose(p_batch.get("frob"), p_single.get("frob"), rtol=1e-9)
+        # and every member against the FP64 oracle's three epochs on the same inputs
+        rng = np.random.default_rng(idx)
+        ys = [rng.standard_normal((n, k)) for _ in range(3)]
+        mus = [0.2 * rng.standard_normal(k) for _ in range(3)]
+        phi_ref, log_ref = core.stgd(*ys, *mus, core.init_phi(idx + 1, k), alpha0=a0, eta=eta, batch=b, epochs=3,
+                                     ns_iters=2 if degree == 3 else 1, ns_degree=degree)
+        assert rel_err(p_batch.get("phi"), phi_ref) < (1e-4 if prec == "fp32" else 2e-2), (n, k, b)
     batch.close()
     for p in singles + members:
         p.close()
@@ -542,18 +549,18 @@ def test_batch_in_a_second_process(mmsb, tmp_path):
     p.close()
 
 
-def test_factorisations_at_large_k(mmsb):
+@pytest.mark.parametrize("mu_scale", [0.005, 0.05])
+def test_factorisations_at_large_k(mmsb, mu_scale):
     """k = 1100: the seeded Phi0 (CholeskyQR of a k x k Gaussian: the triangular inverse above 1024
-    columns) and the power method's orthonormalisation (the Jacobi factorisation with rotations in
-    4-column blocks, since 8-column blocks of A and V exceed the LDS at this size), against the
-    oracle. The means are kept small: with mu ~ 0.05 the shared mean terms make the power step's
-    columns nearly collinear (cond(Phi) ~ 1e7, so Phi^T Phi is at 1e-14 of its largest eigenvalue,
-    where the device drops directions as numerically null and the oracle divides by them); with
-    mu ~ 0.005 cond(Phi) ~ 9e3 and both orthonormalisations are determined to FP64 accuracy."""
+    columns) and the power method's orthonormalisation (the one-sided Jacobi SVD of Phi itself, with
+    rotations in 4-column blocks, since 8-column blocks of A and V exceed the LDS at this size),
+    against the oracle. With means ~ 0.05 the shared mean terms make the power step's columns nearly
+    collinear (cond(Phi) ~ 1e7): the polar factor from Phi's own SVD stays accurate, where one formed
+    from Phi^T Phi (cond ~ 1e14) would not."""
     n, k = 1536, 1100
     rng = np.random.default_rng(7)
     ys = [(rng.standard_normal((n, k)) / np.sqrt(k)).astype(np.float32).astype(np.float64) for _ in range(3)]
-    mus = [0.005 * rng.standard_normal(k) for _ in range(3)]
+    mus = [mu_scale * rng.standard_normal(k) for _ in range(3)]
     cfg = mmsb.config(nx=n, na=n, nb=n, nc=n, ncom=k, alpha_0=0.8, oversample=0, minibatch=256, seed=13)
     p = mmsb.Pipeline(cfg)
     p.set_yvectors(*ys, *mus)

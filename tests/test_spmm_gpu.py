@@ -131,6 +131,9 @@ def test_spmm_with_host_csr_matches_device(mmsb, dev, chunk_nnz, weighted, pinne
     mmsb.spmm_f32_host_csr(rowptr, colidx, vals, B, got, r=r, n=n, alpha=0.25, chunk_nnz=chunk_nnz)
     torch.cuda.synchronize()
     assert torch.equal(got, want)
+    ref = csr_spmm(rowptr, colidx, vals, B[:, :r].double().cpu().numpy(), m, alpha=0.25)
+    np.testing.assert_allclose(got[:, :r].cpu().numpy(), ref, rtol=1e-4, atol=1e-4)
+    assert (got[:, r:] == 3.0).all()            # columns beyond r untouched
 
 
 def test_spmm_host_csr_rejects_bad_blocks(mmsb, dev):
