@@ -106,7 +106,8 @@ struct mmsb_ctx {
   float* phi[2] = {nullptr, nullptr};
   int cur = 0;
   float *phit = nullptr, *gt = nullptr, *phia = nullptr, *h = nullptr, *cgt = nullptr, *cs_part = nullptr, *mab = nullptr;
-  float *ns_rowsum = nullptr, *mpart = nullptr, *nsq = nullptr;
+  unsigned long long* ns_rowsum = nullptr;
+  float *mpart = nullptr, *nsq = nullptr;
   bf16_bits *phi16[2] = {nullptr, nullptr}, *phit16 = nullptr, *phia16 = nullptr, *phiT16 = nullptr, *phitT16 = nullptr,
             *phiaT16 = nullptr, *gt16 = nullptr, *h16 = nullptr, *nsq16 = nullptr;
   double* frob = nullptr;

@@ -67,7 +67,7 @@ struct StgdBuffers {
   uint16_t* h16;
   uint16_t* q16;
   void* cgt;            // [kp x b] C_g^T: FP32, or BF16 in BF16 mode
-  float* rowsum;        // [2 x kp] row sums of |X^T X| (Newton-Schulz scale guard)
+  unsigned long long* rowsum;  // [2 x kp] row sums of |X^T X| in 2^-32 fixed point (Newton-Schulz scale guard)
   float* mpart;         // [kp/32 x 2 x kp] per-row-block partial sums of mu_A^T Phi, mu_B^T Phi
 };
 

@@ -281,17 +281,46 @@ __device__ inline void wave_mm_pair(floatx16& acc_a, floatx16& acc_b, const op_t
   }
 }
 
-// The four waves' 32 x 32 partial accumulators. Register i of lane l holds element
+// Waves per workgroup. Every tile's reduction is split over all of them (kWaves parts of the k or
+// minibatch range), which keeps kWaves waves of loads in flight per tile -- with one tile per compute
+// unit at k ~ 500, the only source of memory-level parallelism. The epilogues run on the first four
+// waves (threads 0..255), which own one element group of four each.
+#ifndef MMSB_STGD_WAVES
+#define MMSB_STGD_WAVES 4
+#endif
+constexpr int kWaves = MMSB_STGD_WAVES;
+constexpr int kThreads = 64 * kWaves;
+static_assert(kWaves == 4 || kWaves == 8, "4 or 8 waves per workgroup");
+
+__device__ inline bool epi_thread() { return threadIdx.x < 256; }
+
+// Four 32 x 32 partial accumulators. Register i of lane l holds element
 // (8 (i / 4) + 4 (l / 32) + i % 4, l % 32) of the block; rows of 66 words keep the column-wise
 // read-back of the epilogues to two-way bank conflicts.
 struct Red4 {
   float v[4][16][66];
 };
 
-__device__ inline void red_put(Red4& R, const floatx16& acc) {
-  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+__device__ inline void red_slot_put(Red4& R, int slot, const floatx16& acc) {
+  const int l = threadIdx.x & 63;
 #pragma unroll
-  for (int i = 0; i < 16; ++i) R.v[w][i][l] = acc[i];
+  for (int i = 0; i < 16; ++i) R.v[slot][i][l] = acc[i];
+}
+
+// Leaves the wave partials in the four slots of R: with eight waves, wave w + 4 adds its accumulator
+// to wave w's first (fixed order), so slot w holds acc_w + acc_{w+4}. The caller synchronises before
+// reading R. Every thread of the block calls it.
+__device__ inline void red_put(Red4& R, floatx16 acc) {
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  if constexpr (kWaves == 8) {
+    if (w >= 4) red_slot_put(R, w - 4, acc);
+    __syncthreads();
+    if (w < 4) {
+#pragma unroll
+      for (int i = 0; i < 16; ++i) acc[i] += R.v[w][i][l];
+    }
+  }
+  if (w < 4) red_slot_put(R, w, acc);
 }
 
 __device__ inline float red_get(const Red4& R, int w, int r, int c) {
@@ -332,6 +361,7 @@ struct TilePre {
 
 __device__ inline void tile_pre(TilePre& t, int kp, int64_t i0, int64_t j0, const float* prev, const double* mu_a,
                                 const double* mu_b) {
+  if (!epi_thread()) return;
   const int r = threadIdx.x >> 3, c4 = 4 * (threadIdx.x & 7);
   if (prev) {
     const float4 p = *reinterpret_cast<const float4*>(prev + (i0 + r) * kp + j0 + c4);
@@ -366,6 +396,8 @@ __device__ inline void finish_tile(TileShm& S, int kp, int ti, int64_t i0, int64
   const int r = threadIdx.x >> 3, c4 = 4 * (threadIdx.x & 7);
   const int64_t nel = (int64_t)kp * kp;
   const int64_t off = (i0 + r) * kp + j0 + c4;
+  const bool act = epi_thread();  // waves 4.. hold no elements; they only meet the barrier
+  if (act) {
   put4<SPLIT>(Out, Out16, nel, off, v);
 #pragma unroll
   for (int e = 0; e < 4; ++e) S.t[c4 + e][r] = v[e];
@@ -398,8 +430,9 @@ __device__ inline void finish_tile(TileShm& S, int kp, int ti, int64_t i0, int64
       }
     }
   }
+  }
   __syncthreads();
-  {
+  if (act) {
     const int c = threadIdx.x >> 3, r4 = 4 * (threadIdx.x & 7);
     const float t[4] = {S.t[c][r4], S.t[c][r4 + 1], S.t[c][r4 + 2], S.t[c][r4 + 3]};
     put4<SPLIT>(OutT, OutT16, nel, (j0 + c) * kp + i0 + r4, t);
@@ -584,14 +617,14 @@ __device__ void p1_tile(const StepCtx& P, int tile, StgdSmem& sm) {
     const int tb = tile / ntk, tj = tile % ntk;
     const int64_t x0 = (int64_t)tb * 32, j0 = (int64_t)tj * 32;
     PartSum ps;
-    mpart_load(ps, P.B.mpart, kp, ntk, j0);  // m_A, m_B of the block's columns
-    // every wave: P_A and P_B over its quarter of the k reduction, sharing the Phi^T fragments
+    if (epi_thread()) mpart_load(ps, P.B.mpart, kp, ntk, j0);  // m_A, m_B of the block's columns
+    // every wave: P_A and P_B over its part of the k reduction, sharing the Phi^T fragments
     floatx16 acc_b = {};
-    kslice<DMODE>(kp, 4, w, k0, kl);
+    kslice<DMODE>(kp, kWaves, w, k0, kl);
     wave_mm_pair<DMODE>(acc, acc_b, static_cast<const TY*>(P.YA) + P.yrow0 * (int64_t)kp,
                         static_cast<const TY*>(P.YB) + P.yrow0 * (int64_t)kp, kp, pick<DMODE>(P.B.phiT, P.B.phiT16, nel, kp).hi,
                         kp, x0, j0, k0, kl);
-    {
+    if (epi_thread()) {
       const float m = mpart_reduce(ps, P.B.mpart, kp, ntk, j0);
       const int o = threadIdx.x >> 2;
       if ((threadIdx.x & 3) == 0) sm.p1.mvec[o >> 5][o & 31] = m;
@@ -599,6 +632,7 @@ __device__ void p1_tile(const StepCtx& P, int tile, StgdSmem& sm) {
     red_put(sm.p1.red[0], acc);
     red_put(sm.p1.red[1], acc_b);
     __syncthreads();
+    if (!epi_thread()) return;
     // C_g^T rows are columns j of the block: thread -> column c, rows r4 .. r4 + 3
     const int c = threadIdx.x >> 3, r4 = 4 * (threadIdx.x & 7);
     const float mA = sm.p1.mvec[0][c], mB = sm.p1.mvec[1][c];
@@ -626,11 +660,12 @@ __device__ void p1_tile(const StepCtx& P, int tile, StgdSmem& sm) {
     const int ti = tile / ntk, tj = tile % ntk;
     const int64_t i0 = (int64_t)ti * 32, j0 = (int64_t)tj * 32;
     const Opnd<op_t<KMODE>> phiT_k = pick<KMODE>(P.B.phiT, P.B.phiT16, nel, kp);
-    kslice<KMODE>(kp, 4, w, k0, kl);
+    kslice<KMODE>(kp, kWaves, w, k0, kl);
     wave_mm<KMODE, true>(acc, phiT_k, phiT_k, i0, j0, k0, kl);  // G = Phi^T Phi
     Red4& red = sm.p1.red[0];
     red_put(red, acc);
     __syncthreads();
+    if (!epi_thread()) return;
     const int r = threadIdx.x >> 3, c4 = 4 * (threadIdx.x & 7);
     float g[4];
 #pragma unroll
@@ -656,17 +691,26 @@ __device__ void p2_tile(const StepCtx& P, int tile, StgdSmem& sm) {
   floatx16 acc = {};
   int64_t k0, kl;
   PartSum ps;  // column sums of P_A o P_B and m_A, m_B for c_mu, gathered during the MFMA loop
-  cspart_load(ps, a.cs_part, kp, a.b / 32, j0);
-  const float mab_a = a.mab[j0 + (threadIdx.x >> 3)], mab_b = a.mab[kp + j0 + (threadIdx.x >> 3)];
+  const bool epi = epi_thread();
   const int r = threadIdx.x >> 3, c4 = 4 * (threadIdx.x & 7);
-  const float muc = (float)a.mu_c[i0 + r];  // epilogue inputs, loaded ahead of the MFMA loop
-  const float4 ph = *reinterpret_cast<const float4*>(a.phi + (i0 + r) * kp + j0 + c4);
-  if (w < 2) {  // Phi H over two halves of k (H symmetric: row j of H is column j)
-    kslice<KMODE>(kp, 2, w, k0, kl);
+  float mab_a = 0.f, mab_b = 0.f, muc = 0.f;
+  float4 ph = make_float4(0.f, 0.f, 0.f, 0.f);
+  if (epi) {  // epilogue inputs, loaded ahead of the MFMA loop
+    cspart_load(ps, a.cs_part, kp, a.b / 32, j0);
+    mab_a = a.mab[j0 + (threadIdx.x >> 3)];
+    mab_b = a.mab[kp + j0 + (threadIdx.x >> 3)];
+    muc = (float)a.mu_c[i0 + r];
+    ph = *reinterpret_cast<const float4*>(a.phi + (i0 + r) * kp + j0 + c4);
+  }
+  // waves with (w & 2) == 0 take Phi H, the others Y_C^T C_g, each over kWaves / 2 parts of its range;
+  // after red_put, slots 0, 1 hold Phi H and slots 2, 3 Y_C^T C_g
+  const int part = (w & 1) + 2 * (w >> 2);
+  if ((w & 2) == 0) {  // Phi H (H symmetric: row j of H is column j)
+    kslice<KMODE>(kp, kWaves / 2, part, k0, kl);
     wave_mm<KMODE, true>(acc, pick<KMODE>(a.phi, P.B.phi16[P.cur], nel, kp), pick<KMODE>(P.B.h, P.B.h16, nel, kp), i0, j0,
                          k0, kl);
-  } else {      // Y_C^T C_g over two halves of the minibatch tile
-    kslice<DMODE>(a.b, 2, w & 1, k0, kl);
+  } else {             // Y_C^T C_g over parts of the minibatch tile
+    kslice<DMODE>(a.b, kWaves / 2, part, k0, kl);
     const Opnd<TY> yc = opnd<TY>(static_cast<const TY*>(P.YCt) + P.yrow0, nullptr, a.n_pad);
     const Opnd<TPH> cg = opnd<TPH>(static_cast<const TPH*>(P.B.cgt), nullptr, a.b);
     if (P.yrow0 % VEC == 0)
@@ -674,24 +718,26 @@ __device__ void p2_tile(const StepCtx& P, int tile, StgdSmem& sm) {
     else
       wave_mm<DMODE, false>(acc, yc, cg, i0, j0, k0, kl);
   }
-  {
+  if (epi) {
     const int c = threadIdx.x >> 3;
     const float ssum = cspart_reduce(ps, a.cs_part, kp, a.b / 32, j0);
     if ((threadIdx.x & 7) == 0) sm.p2.cmu[c] = a.c0 * mab_a * mab_b + a.c1 * ssum / (float)P.nb;
   }
   if (tile == 0)
-    for (int t = threadIdx.x; t < kp; t += blockDim.x) P.B.rowsum[t] = 0.f;
+    for (int t = threadIdx.x; t < kp; t += blockDim.x) P.B.rowsum[t] = 0ull;
   red_put(red, acc);
   __syncthreads();
   const float eta = a.eta, inv_b = 1.f / (float)P.nb;
   const float p0[4] = {ph.x, ph.y, ph.z, ph.w};
-  float v[4];
+  float v[4] = {0.f, 0.f, 0.f, 0.f};
+  if (epi) {
 #pragma unroll
-  for (int e = 0; e < 4; ++e) {
-    const int c = c4 + e;
-    const float phh = red_get(red, 0, r, c) + red_get(red, 1, r, c);
-    const float t = red_get(red, 2, r, c) + red_get(red, 3, r, c);
-    v[e] = p0[e] - eta * a.theta * phh + eta * (t * inv_b + muc * sm.p2.cmu[c]);
+    for (int e = 0; e < 4; ++e) {
+      const int c = c4 + e;
+      const float phh = red_get(red, 0, r, c) + red_get(red, 1, r, c);
+      const float t = red_get(red, 2, r, c) + red_get(red, 3, r, c);
+      v[e] = p0[e] - eta * a.theta * phh + eta * (t * inv_b + muc * sm.p2.cmu[c]);
+    }
   }
   TilePre pre{};  // the reference's log: ||Phi~ - Phi||^2 before the renormalisation (frob_pre)
 #pragma unroll
@@ -703,18 +749,28 @@ __device__ void p2_tile(const StepCtx& P, int tile, StgdSmem& sm) {
 // The Newton-Schulz scale guard c = min(1, sqrt(2 / ||G||_inf)) from the row sums of |G|, computed
 // by wave 3 of a block: guard_load issues the loads before the block's MFMA loop, guard_scale
 // reduces them after it and leaves c in *cscale (shared memory; read after the next barrier).
-__device__ inline void guard_load(float rs[8], const float* rowsum, int kp) {
-  const int lane = threadIdx.x & 63;
-#pragma unroll
-  for (int u = 0; u < 8; ++u) rs[u] = lane + 64 * u < kp ? rowsum[lane + 64 * u] : 0.f;
+// The row sums are accumulated across tiles as fixed-point integers (units of 2^-32): integer
+// addition is associative, so the sums -- and the guard they decide -- do not depend on the order in
+// which the tiles' atomics land, and Phi is bit-reproducible from run to run. Partial sums are
+// clamped at 2^30, far beyond the point where the guard acts.
+__device__ inline unsigned long long fix32(float s) {
+  return (unsigned long long)(long long)llrintf(fminf(s, 1073741824.f) * 4294967296.f);
 }
 
-__device__ inline void guard_scale(const float rs[8], const float* rowsum, int kp, float* cscale) {
+__device__ inline float unfix32(unsigned long long v) { return (float)((double)v * 2.3283064365386963e-10); }
+
+__device__ inline void guard_load(float rs[8], const unsigned long long* rowsum, int kp) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int u = 0; u < 8; ++u) rs[u] = lane + 64 * u < kp ? unfix32(rowsum[lane + 64 * u]) : 0.f;
+}
+
+__device__ inline void guard_scale(const float rs[8], const unsigned long long* rowsum, int kp, float* cscale) {
   const int lane = threadIdx.x & 63;
   float mx = 0.f;
 #pragma unroll
   for (int u = 0; u < 8; ++u) mx = fmaxf(mx, rs[u]);
-  for (int t = lane + 512; t < kp; t += 64) mx = fmaxf(mx, rowsum[t]);
+  for (int t = lane + 512; t < kp; t += 64) mx = fmaxf(mx, unfix32(rowsum[t]));
   for (int off = 32; off > 0; off >>= 1) mx = fmaxf(mx, __shfl_xor(mx, off));
   if (lane == 0) *cscale = mx > 2.f ? sqrtf(2.f / mx) : 1.f;
 }
@@ -729,15 +785,16 @@ __device__ void gram_tile(const StepCtx& P, int it, int tile, StgdSmem& sm) {
   const int ti = tile / ntk, tj = tile % ntk;
   const int64_t i0 = (int64_t)ti * 32, j0 = (int64_t)tj * 32;
   const NsBufs nb = ns_bufs(P.B, P.cur, it, P.ns_iters);
-  float* rowsum = P.B.rowsum + (it & 1) * kp;
+  unsigned long long* rowsum = P.B.rowsum + (it & 1) * kp;
   Red4& red = sm.p2.red;
   floatx16 acc = {};
   int64_t k0, kl;
   const Opnd<op_t<KMODE>> xT = pick<KMODE>(nb.xT, nb.xT16, nel, kp);
-  kslice<KMODE>(kp, 4, w, k0, kl);
+  kslice<KMODE>(kp, kWaves, w, k0, kl);
   wave_mm<KMODE, true>(acc, xT, xT, i0, j0, k0, kl);
   red_put(red, acc);
   __syncthreads();
+  if (!epi_thread()) return;
   const int r = threadIdx.x >> 3, c4 = 4 * (threadIdx.x & 7);
   float g[4], s = 0.f;
 #pragma unroll
@@ -747,7 +804,7 @@ __device__ void gram_tile(const StepCtx& P, int it, int tile, StgdSmem& sm) {
   }
   put4<SPLIT>(P.B.gt, sp<SPLIT>(P.B.gt16), nel, (i0 + r) * kp + j0 + c4, g);
   for (int off = 1; off < 8; off <<= 1) s += __shfl_xor(s, off);
-  if ((threadIdx.x & 7) == 0) atomicAdd(rowsum + i0 + r, s);
+  if ((threadIdx.x & 7) == 0) atomicAdd(rowsum + i0 + r, fix32(s));
 }
 
 // Degree-5 iteration, second stage: Q = -(5/4) c^2 G + (3/8) c^4 G^2 (symmetric, as G), so that the
@@ -760,20 +817,22 @@ __device__ void qpoly_tile(const StepCtx& P, int it, int tile, StgdSmem& sm) {
   const int w = threadIdx.x >> 6;
   const int ti = tile / ntk, tj = tile % ntk;
   const int64_t i0 = (int64_t)ti * 32, j0 = (int64_t)tj * 32;
-  const float* rowsum = P.B.rowsum + (it & 1) * kp;
+  const unsigned long long* rowsum = P.B.rowsum + (it & 1) * kp;
   Red4& red = sm.p2.red;
   floatx16 acc = {};
   int64_t k0, kl;
-  kslice<KMODE>(kp, 4, w, k0, kl);
+  kslice<KMODE>(kp, kWaves, w, k0, kl);
   float rs[8];
   if (w == 3) guard_load(rs, rowsum, kp);
   const int r = threadIdx.x >> 3, c4 = 4 * (threadIdx.x & 7);
-  const float4 gv = *reinterpret_cast<const float4*>(P.B.gt + (i0 + r) * kp + j0 + c4);
+  float4 gv = make_float4(0.f, 0.f, 0.f, 0.f);
+  if (epi_thread()) gv = *reinterpret_cast<const float4*>(P.B.gt + (i0 + r) * kp + j0 + c4);
   const Opnd<op_t<KMODE>> g = pick<KMODE>(P.B.gt, P.B.gt16, nel, kp);
   wave_mm<KMODE, true>(acc, g, g, i0, j0, k0, kl);  // (G G)_ij = G[i,:] . G[j,:] (G symmetric)
   if (w == 3) guard_scale(rs, rowsum, kp, &sm.p2.cscale);
   red_put(red, acc);
   __syncthreads();
+  if (!epi_thread()) return;
   const float c2 = sm.p2.cscale * sm.p2.cscale;
   const float a1 = -1.25f * c2, a2 = 0.375f * c2 * c2;
   const float gg[4] = {gv.x, gv.y, gv.z, gv.w};
@@ -795,17 +854,18 @@ __device__ void update_tile(const StepCtx& P, int it, int tile, StgdSmem& sm) {
   const int64_t i0 = (int64_t)ti * 32, j0 = (int64_t)tj * 32;
   const NsBufs nb = ns_bufs(P.B, P.cur, it, P.ns_iters);
   const bool last = it == P.ns_iters - 1;
-  const float* rowsum = P.B.rowsum + (it & 1) * kp;
-  float* rowsum_next = P.B.rowsum + ((it + 1) & 1) * kp;
+  const unsigned long long* rowsum = P.B.rowsum + (it & 1) * kp;
+  unsigned long long* rowsum_next = P.B.rowsum + ((it + 1) & 1) * kp;
   Red4& red = sm.p2.red;
   floatx16 acc = {};
   int64_t k0, kl;
-  kslice<KMODE>(kp, 4, w, k0, kl);
+  kslice<KMODE>(kp, kWaves, w, k0, kl);
   float rs[8];
   if (w == 3) guard_load(rs, rowsum, kp);
   const bool quintic = P.ns_degree == 5;
   const int r = threadIdx.x >> 3, c4 = 4 * (threadIdx.x & 7);
-  const float4 xv = *reinterpret_cast<const float4*>(nb.x + (i0 + r) * kp + j0 + c4);
+  float4 xv = make_float4(0.f, 0.f, 0.f, 0.f);
+  if (epi_thread()) xv = *reinterpret_cast<const float4*>(nb.x + (i0 + r) * kp + j0 + c4);
   TilePre pre;
   tile_pre(pre, kp, i0, j0, last ? P.a.phi : nullptr, last ? P.a.mu_a : nullptr, P.a.mu_b);
   wave_mm<KMODE, true>(acc, pick<KMODE>(nb.x, nb.x16, nel, kp),
@@ -813,15 +873,16 @@ __device__ void update_tile(const StepCtx& P, int it, int tile, StgdSmem& sm) {
                        k0, kl);  // X G or X Q (both symmetric)
   if (w == 3) guard_scale(rs, rowsum, kp, &sm.p2.cscale);
   if (tile == 0)
-    for (int t = threadIdx.x; t < kp; t += blockDim.x) rowsum_next[t] = 0.f;
+    for (int t = threadIdx.x; t < kp; t += blockDim.x) rowsum_next[t] = 0ull;
   red_put(red, acc);
   __syncthreads();
   const float cs = sm.p2.cscale;
   const float c1 = (quintic ? 1.875f : 1.5f) * cs, c3 = quintic ? cs : -0.5f * cs * cs * cs;
   const float x[4] = {xv.x, xv.y, xv.z, xv.w};
-  float v[4];
+  float v[4] = {0.f, 0.f, 0.f, 0.f};
+  if (epi_thread())
 #pragma unroll
-  for (int e = 0; e < 4; ++e) v[e] = c1 * x[e] + c3 * red_sum4(red, r, c4 + e);
+    for (int e = 0; e < 4; ++e) v[e] = c1 * x[e] + c3 * red_sum4(red, r, c4 + e);
   finish_tile<SPLIT>(sm.p2.S, kp, ti, i0, j0, v, nb.dst, sp<SPLIT>(nb.dst16), nb.dstT, sp<SPLIT>(nb.dstT16), pre,
                      last ? P.frob : nullptr, last ? P.B.mpart : nullptr);
 }
@@ -837,7 +898,8 @@ __device__ void copy_tile(const StepCtx& P, bool setup, int tile, StgdSmem& sm, 
   const int64_t i0 = (int64_t)ti * 32, j0 = (int64_t)tj * 32;
   const int r = threadIdx.x >> 3, c4 = 4 * (threadIdx.x & 7);
   const float* X = setup ? P.B.phi[P.cur] : (qrfin ? P.B.phia : P.B.phit);
-  const float4 xv = *reinterpret_cast<const float4*>(X + (i0 + r) * kp + j0 + c4);
+  float4 xv = make_float4(0.f, 0.f, 0.f, 0.f);
+  if (epi_thread()) xv = *reinterpret_cast<const float4*>(X + (i0 + r) * kp + j0 + c4);
   float v[4] = {xv.x, xv.y, xv.z, xv.w};
   if (qrfin)
 #pragma unroll
@@ -886,19 +948,20 @@ __device__ inline bool stop_test(const StgdArgs& a, int64_t t, double f) {
 // decision from the same value, so the whole launch returns together. The skipped kernels write
 // nothing, and the host reads stop_at after the run.
 template <int DMODE, int KMODE, typename TY, typename TPH, int PHASE>
-__global__ __launch_bounds__(256) void stgd_phase_kernel(StepCtx P, int32_t it) {
+__global__ __launch_bounds__(kThreads) void stgd_phase_kernel(StepCtx P, int32_t it) {
   __shared__ StgdSmem sm;
   constexpr bool SPLIT = KMODE == MM_BF16X3;
   if constexpr (PHASE != PH_SETUP) {
     if (P.a.stop) {
       const int64_t g = P.a.stop->base + P.st;
+      if (P.a.stop->stop_at <= g) return;  // an earlier step ended the loop (checked first: the log of
+                                           // a skipped step is never written)
       if constexpr (PHASE == PH_P1) {
         if (P.frob_prev && stop_test(P.a, g - 1, *P.frob_prev)) {
           if (threadIdx.x == 0) P.a.stop->stop_at = g;
           return;
         }
       }
-      if (P.a.stop->stop_at <= g) return;
     }
   }
   if constexpr (PHASE == PH_SETUP) copy_tile<SPLIT>(P, true, blockIdx.x, sm);
@@ -964,7 +1027,7 @@ __device__ inline void make_step(const BatchInst& I, int64_t st, StepCtx& P) {
 }
 
 template <int DMODE, int KMODE, typename TY, typename TPH>
-__global__ __launch_bounds__(256) void stgd_batch_kernel(const BatchInst* insts, const BatchSeg* segs,
+__global__ __launch_bounds__(kThreads) void stgd_batch_kernel(const BatchInst* insts, const BatchSeg* segs,
                                                          const int32_t* slot_seg, const int64_t* slot_tiles,
                                                          int32_t nslots) {
   cg::grid_group grid = cg::this_grid();
@@ -997,7 +1060,7 @@ __global__ __launch_bounds__(256) void stgd_batch_kernel(const BatchInst* insts,
 // One phase slot of the plan as its own launch (blockIdx.x = tile of the slot): the grouped
 // alternative to the persistent kernel, where the kernel boundary orders consecutive slots.
 template <int DMODE, int KMODE, typename TY, typename TPH>
-__global__ __launch_bounds__(256) void stgd_slot_kernel(const BatchInst* insts, const BatchSeg* segs, int32_t s0, int32_t s1) {
+__global__ __launch_bounds__(kThreads) void stgd_slot_kernel(const BatchInst* insts, const BatchSeg* segs, int32_t s0, int32_t s1) {
   __shared__ StgdSmem sm;
   __shared__ StepCtx P;
   __shared__ int32_t sidx;
@@ -1065,7 +1128,7 @@ static int run_steps(hipStream_t s, const StgdArgs& args, const void* YA, const 
   P.ns_iters = ns_iters;
   P.ns_degree = ns_degree;
   auto launch = [&](int phase, int it) -> int {
-    const dim3 grid(phase_tiles(phase, kp, args.b)), block(256);
+    const dim3 grid(phase_tiles(phase, kp, args.b)), block(kThreads);
     switch (phase) {
       case PH_SETUP: hipLaunchKernelGGL((stgd_phase_kernel<DMODE, KMODE, TY, TPH, PH_SETUP>), grid, block, 0, s, P, it); break;
       case PH_P1: hipLaunchKernelGGL((stgd_phase_kernel<DMODE, KMODE, TY, TPH, PH_P1>), grid, block, 0, s, P, it); break;
@@ -1284,7 +1347,7 @@ int stgd_batch_plan(StgdBatch*& out, int32_t precision, const std::vector<StgdBa
     return fail(e);
   int dev = 0, per_cu = 0, cus = 0;
   if ((e = hipGetDevice(&dev)) != hipSuccess) return fail(e);
-  if ((e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, batch_kernel(precision), 256, 0)) != hipSuccess) return fail(e);
+  if ((e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, batch_kernel(precision), kThreads, 0)) != hipSuccess) return fail(e);
   if ((e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev)) != hipSuccess) return fail(e);
   if (per_cu < 1) {
     stgd_batch_free(b);
@@ -1306,7 +1369,7 @@ int32_t stgd_batch_grid_max(const StgdBatch* b) { return b ? b->grid_max : 0; }
 int stgd_batch_launch(hipStream_t s, StgdBatch* b) {
   MMSB_CHECK_ARG(b, "stgd batch: null plan");
   void* args[] = {&b->d_inst, &b->d_segs, &b->d_slot_seg, &b->d_slot_tiles, &b->nslots};
-  MMSB_HIP(hipLaunchCooperativeKernel(batch_kernel(b->precision), dim3(b->grid), dim3(256), args, 0, s));
+  MMSB_HIP(hipLaunchCooperativeKernel(batch_kernel(b->precision), dim3(b->grid), dim3(kThreads), args, 0, s));
   return MMSB_OK;
 }
 
@@ -1316,10 +1379,10 @@ int stgd_batch_launch_slots(hipStream_t s, StgdBatch* b) {
     const dim3 grid((unsigned)b->h_slot_tiles[sl]);
     const int32_t s0 = b->h_slot_seg[sl], s1 = b->h_slot_seg[sl + 1];
     if (b->precision == MMSB_PREC_BF16)
-      hipLaunchKernelGGL((stgd_slot_kernel<MM_BF16, MM_BF16X3, __bf16, __bf16>), grid, dim3(256), 0, s, b->d_inst, b->d_segs,
+      hipLaunchKernelGGL((stgd_slot_kernel<MM_BF16, MM_BF16X3, __bf16, __bf16>), grid, dim3(kThreads), 0, s, b->d_inst, b->d_segs,
                          s0, s1);
     else
-      hipLaunchKernelGGL((stgd_slot_kernel<MM_F32, MM_F32, float, float>), grid, dim3(256), 0, s, b->d_inst, b->d_segs, s0,
+      hipLaunchKernelGGL((stgd_slot_kernel<MM_F32, MM_F32, float, float>), grid, dim3(kThreads), 0, s, b->d_inst, b->d_segs, s0,
                          s1);
     MMSB_LAUNCH_CHECK();
   }

@@ -133,6 +133,18 @@ def test_stgd_guarded_renorm_matches_oracle(mmsb, degree, iters):
                                  ns_degree=degree)
     assert rel_err(p.get("phi"), phi_ref) < 1e-4
     np.testing.assert_allclose(p.get("frob"), log_ref, rtol=2e-3, atol=1e-12)
+    # the guard's row sums are order-independent (fixed-point atomics): repeated runs give Phi bit for
+    # bit, in both precisions, though the tiles' atomics land in a different order each time
+    for prec in ("fp32", "bf16"):
+        outs = []
+        for _ in range(3):
+            q = mmsb.Pipeline(cfg)
+            q.set_yvectors(*ys, *mus)
+            q.set_phi(phi0)
+            q.stgd(12, prec)
+            outs.append(q.get("phi"))
+            q.close()
+        assert all(np.array_equal(outs[0], o) for o in outs[1:]), prec
     p.close()
 
 
