@@ -150,6 +150,8 @@ SIGNATURES: dict = {
     "stpm_force_exit": (None, []),
     "mmsb_grad": (ctypes.c_int, [_vp, ctypes.c_int64, ctypes.c_int32, _vp, ctypes.c_int32, ctypes.c_double, ctypes.c_int32,
                                  _vp, _vp]),
+    "mmsb_stgd_grad_f32_noalloc": (ctypes.c_int, [_vp, ctypes.c_int32, ctypes.c_int32, _vp, _vp, _vp, ctypes.c_int32, _vp,
+                                                   _vp, _vp, _vp, _vp, ctypes.c_double, _vp, _vp]),
     "mmsb_write_triplets": (ctypes.c_int, [_vp, ctypes.c_char_p, ctypes.c_char_p]),
     "mmsb_release": (None, []),
     "mmsb_set_strip_limit": (ctypes.c_int64, [ctypes.c_int64]),

@@ -12,6 +12,8 @@
 //   T(I,phi,phi) = Y_A^T K_A / b + mu_A kappa_A^T,  K_A = c3 B o C + c1 (B o m_C + m_B o C),
 //                                                   kappa_A = c0 m_B o m_C + c1 mean_x(B o C)
 // and likewise for the B and C modes. Everything accumulates in FP64; Y may be FP32 or FP64.
+#include <mutex>
+
 #include "common.h"
 #include "dense.h"
 
@@ -288,6 +290,53 @@ extern "C" int mmsb_stgd_grad_f64(void* stream, int32_t k, int32_t nb, const dou
   MMSB_CHECK_ARG(coeffs, "grad: null coefficients");
   return mmsb::stgd_grad<double>(static_cast<hipStream_t>(stream), k, nb, ya, yb, yc, ldy, mu_a, mu_b, mu_c, phi, coeffs[0],
                                  coeffs[1], coeffs[2], theta, grad, loss, work);
+}
+
+namespace mmsb {
+// Scratch of the allocation-free adjoint entry point (mmsb_stgd_grad_f32_noalloc): a device array of
+// fixed size, one copy per GPU, held until the process ends. Host code never passes the array's name
+// to a kernel (on the host it is only a stand-in): the device address of each GPU's copy comes from
+// hipGetSymbolAddress, called once per device with that device current, and is kept per device.
+static constexpr size_t kGradScratchBytes = (size_t)32 << 20;
+static __device__ __align__(256) unsigned char scratch[kGradScratchBytes];
+static constexpr int kMaxScratchDevices = 64;
+static std::mutex g_scratch_mu;
+static void* g_scratch_dev[kMaxScratchDevices] = {};
+
+static int grad_scratch(void** out) {
+  int dev = 0;
+  MMSB_HIP(hipGetDevice(&dev));
+  MMSB_CHECK_ARG(dev >= 0 && dev < kMaxScratchDevices, "grad_noalloc: device %d beyond the supported %d", dev,
+                 kMaxScratchDevices);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_dev[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return set_error(MMSB_ERR_HIP, "grad_noalloc: hipGetSymbolAddress(scratch) on device %d: %s", dev, hipGetErrorString(e));
+    g_scratch_dev[dev] = p;
+  }
+  *out = g_scratch_dev[dev];
+  return MMSB_OK;
+}
+}  // namespace mmsb
+
+extern "C" int mmsb_stgd_grad_f32_noalloc(void* stream, int32_t k, int32_t nb, const float* ya, const float* yb,
+                                          const float* yc, int32_t ldy, const double* mu_a, const double* mu_b,
+                                          const double* mu_c, const double* phi, const double* coeffs, double theta,
+                                          double* grad, double* loss) {
+  mmsb::clear_error();
+  MMSB_CHECK_ARG(coeffs, "grad: null coefficients");
+  MMSB_CHECK_ARG(k >= 1 && nb >= 1, "grad_noalloc: bad shape k=%d nb=%d", k, nb);
+  const size_t need = mmsb::grad_workspace_bytes(k, nb);
+  MMSB_CHECK_ARG(need <= mmsb::kGradScratchBytes,
+                 "grad_noalloc: k=%d, nb=%d needs %zu bytes of scratch, above the fixed %zu (use mmsb_stgd_grad_f32 with a "
+                 "workspace)", k, nb, need, mmsb::kGradScratchBytes);
+  void* work = nullptr;
+  MMSB_TRY(mmsb::grad_scratch(&work));
+  // every region of the workspace is written before it is read (no clearing needed)
+  return mmsb::stgd_grad<float>(static_cast<hipStream_t>(stream), k, nb, ya, yb, yc, ldy, mu_a, mu_b, mu_c, phi, coeffs[0],
+                                coeffs[1], coeffs[2], theta, grad, loss, work);
 }
 
 extern "C" int mmsb_stgd_grad_f32(void* stream, int32_t k, int32_t nb, const float* ya, const float* yb, const float* yc,

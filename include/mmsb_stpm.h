@@ -107,6 +107,16 @@ int mmsb_stgd_grad_f64(void* stream, int32_t k, int32_t nb, const double* ya, co
 int mmsb_stgd_grad_f32(void* stream, int32_t k, int32_t nb, const float* ya, const float* yb, const float* yc,
                        int32_t ldy, const double* mu_a, const double* mu_b, const double* mu_c, const double* phi,
                        const double* coeffs, double theta, double* grad, double* loss, void* work);
+/* mmsb_stgd_grad_f32 without the workspace argument: its scratch is a fixed 32 MiB device array of
+ * the library (one per GPU, kept until the process ends), so the call allocates nothing -- for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc). No counterpart in the reference. k, nb whose mmsb_stgd_grad_workspace
+ * exceeds 32 MiB are rejected with MMSB_ERR_ARG (e.g. k = 100 allows nb up to 6504, k = 256 up to
+ * 2216). Calls on one GPU share its array: calls that overlap on one GPU (other streams or threads)
+ * race and must be serialised by the caller. */
+int mmsb_stgd_grad_f32_noalloc(void* stream, int32_t k, int32_t nb, const float* ya, const float* yb, const float* yc,
+                               int32_t ldy, const double* mu_a, const double* mu_b, const double* mu_c, const double* phi,
+                               const double* coeffs, double theta, double* grad, double* loss);
 
 /* ------------------------------------------------------------------------------------------
  * Host graph containers and the seeded MMSB generator
