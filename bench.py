@@ -30,16 +30,22 @@ PEAK = {"bf16": 2.5e15, "fp32": 157.3e12, "hbm": 8.0e12}
 
 # n per partition, k, alpha0, degree-corrected, target mean degree (or p_in/p_out), minibatch, eta.
 # The degree-corrected graph's whitened vectors are heavy-tailed (row norms: median ~4, 99.9th
-# percentile ~120, max ~600 at 1M nodes), and the cubic step at the reference's eta = 1e-3 moves Phi
-# by O(1) per minibatch (median ||dPhi||_F^2 ~ 7): a chaotic regime in which no two precisions, or
-# two summation orders, can agree. eta = 1e-5 keeps the steps small (median ||dPhi||^2 ~ 7e-5). The
-# renormalisation is the library default (two degree-3 Newton-Schulz iterations per step): a single
-# degree-5 iteration is one kernel shorter but, on this graph, ends two epochs 8.5e-3 away from it.
+# percentile ~120, max ~600 at 1M nodes), and with the paper coefficients that go with the default
+# paper M2 scaling ((a0+1)(a0+2)/2 = 6 times the init_buff ones at a0 = 2) a step of eta >= 5e-6 moves
+# Phi by O(1) per minibatch (median ||dPhi||_F^2 ~ 0.06-30): a chaotic regime in which no two
+# precisions, or two summation orders, can agree. eta = 1e-6 keeps the steps small (median
+# ||dPhi||^2 ~ 3e-5). The renormalisation is the library default (two degree-3 Newton-Schulz
+# iterations per step): a single degree-5 iteration is one kernel shorter but, on this graph, ends two
+# epochs 8.5e-3 away from it. The end-to-end schedule is one BF16 epoch and one FP32 polishing epoch.
 WORKLOADS = {
     "c1": dict(n=1000, k=10, alpha0=1.0, dc=False, p_in=0.6, p_out=0.02, b=100, eta=1e-3),
     "c2": dict(n=100000, k=100, alpha0=1.0, dc=False, p_in=0.01, p_out=2e-4, b=1024, eta=1e-3),
-    "c3": dict(n=1000000, k=500, alpha0=2.0, dc=True, mean_degree=80.0, b=1024, eta=1e-5),
+    "c3": dict(n=1000000, k=500, alpha0=2.0, dc=True, mean_degree=80.0, b=1024, eta=1e-6),
 }
+# A graph of the same size class on which the method recovers the planted communities (60 expected
+# neighbours per partition inside a node's community, 20 outside): scored end to end with the
+# schedule that recovers them.
+RECOVERY = dict(n=250000, k=50, alpha0=1.0, d_in=60.0, d_out=20.0, b=1024, eta=3e-2, epochs=8, polish=1)
 
 
 def log(msg):
@@ -78,6 +84,7 @@ def main():
                     help="Newton-Schulz degree of the renormalisation (default: the workload's, else the library's)")
     ap.add_argument("--ns-iters", type=int, default=None,
                     help="Newton-Schulz iterations per step (default: the workload's, else the library's)")
+    ap.add_argument("--no-recovery", action="store_true", help="skip the recovery run on the recoverable graph")
     ap.add_argument("--traffic-probe", action="store_true",
                     help="after whitening, only direct (ungraphed) STGD steps and a streaming copy of known size, "
                          "for rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE passes; prints no JSON line")
@@ -105,8 +112,16 @@ def main():
     else:
         p_in, p_out = w["p_in"], w["p_out"]
     t0 = time.time()
-    g = m.generate(n, n, n, n, k, w["alpha0"], p_in, p_out, seed=0, degree_corrected=w["dc"], threads=16)
+    g = m.generate(n, n, n, n, k, w["alpha0"], p_in, p_out, seed=0, degree_corrected=w["dc"], threads=16,
+                   keep_truth=not args.no_oracle)
     t_gen = time.time() - t0
+    truth_x = None
+    if "pi" in g:  # the X partition's Dirichlet memberships, in the order of its local indices
+        x_nodes = np.nonzero(g["part"] == 0)[0]
+        truth_x = np.zeros((n, k), np.float32)
+        truth_x[g["local"][x_nodes]] = g["pi"][x_nodes]
+        for key in ("pi", "theta", "part", "local"):
+            del g[key]
     nnz = {nm: int(g[nm][1].size) for nm in ("XA", "XB", "XC", "BA", "CA")}
     mean_deg = 2.0 * g["n_edges"] / (4 * n)
     log(f"generated {args.workload}: {g['n_edges']} edges, mean degree {mean_deg:.1f}, {t_gen:.1f}s")
@@ -114,9 +129,10 @@ def main():
     ns = {key: val for key, val in (("ns_degree", args.ns_degree), ("ns_iters", args.ns_iters)) if val is not None}
     ns = {**{key: w[key] for key in ("ns_degree", "ns_iters") if key in w}, **ns}
     cfg = m.config(nx=n, na=n, nb=n, nc=n, ncom=k, alpha_0=w["alpha0"], learn_rate=w["eta"], minibatch=b,
-                   epochs=1, polish_epochs=0, oversample=10, power_iters=1, seed=1, precision=args.precision, **ns)
+                   epochs=2 if args.precision == "bf16" else 1, polish_epochs=1 if args.precision == "bf16" else 0,
+                   oversample=10, power_iters=1, seed=1, precision=args.precision, **ns)
     ns = {"ns_degree": cfg.ns_degree, "ns_iters": cfg.ns_iters}
-    # ---- end-to-end: CSR on host -> whitening -> 1 STGD epoch -> eigen -> Pi on host ----------
+    # ---- end-to-end: CSR on host -> whitening -> STGD schedule -> eigen -> Pi on host ----------
     t0 = time.time()
     p = m.Pipeline(cfg)
     p.set_graph(g)
@@ -139,6 +155,11 @@ def main():
     t_dl = time.time() - t2 - t_post
     log(f"end-to-end {t_e2e:.2f}s (upload {t_up:.2f}, whiten {t_wh:.2f}, stgd {t_st:.2f}, post {t_post:.2f}, "
         f"download {t_dl:.2f})")
+    recovery = None
+    if truth_x is not None:
+        recovery = {"workload": matched_recovery(pin[0], truth_x)}
+        del truth_x
+        log(f"recovery of the planted communities: {recovery}")
     diag = {"nonfinite_pin": int(sum((~np.isfinite(a)).sum() for a in pin)),
             "nonfinite_phi": int((~np.isfinite(p.get("phi"))).sum()),
             "nonfinite_W_rows0_1000": int((~np.isfinite(p.get_rows("W", 0, 1000))).sum()),
@@ -189,8 +210,11 @@ def main():
 
     cpu_e2e = None
     if not args.no_oracle and args.cpu_e2e_sample > 0 and args.cpu_e2e_sample < n:
-        cpu_e2e = cpu_e2e_estimate(m, w, n, args.cpu_e2e_sample, t_e2e)
+        cpu_e2e = cpu_e2e_sample(m, w, args.cpu_e2e_sample, t_e2e, n)
         log(f"cpu e2e {cpu_e2e}")
+    if recovery is not None and args.workload == "c3" and not args.no_recovery:
+        recovery["recoverable_1M_nodes"] = recovery_run(m)
+        log(f"recovery at scale: {recovery['recoverable_1M_nodes']}")
 
     # ---- SpMM roofline on G_XA (n x n, nnz) times a tall n x r FP32 matrix ----------------------
     spmm = spmm_roofline(m, g["XA"], r=k + 10)
@@ -211,17 +235,20 @@ def main():
         "data": "synthetic (seeded MMSB generator)",
         "config": {"workload": f"mmsb-{args.workload}: {n} nodes/partition, k={k}, alpha0={w['alpha0']}, "
                                f"{'degree-corrected Pareto(2.5), ' if w['dc'] else ''}mean degree {mean_deg:.1f}, "
-                               f"minibatch {b}, eta {w['eta']:g}, renormalisation {ns['ns_iters']} Newton-Schulz "
-                               f"iteration(s) of degree {ns['ns_degree']}, one step = one STGD epoch over X",
+                               f"minibatch {b}, eta {w['eta']:g}, paper M2 and T scaling, renormalisation "
+                               f"{ns['ns_iters']} Newton-Schulz iteration(s) of degree {ns['ns_degree']}, "
+                               f"one step = one STGD epoch over X",
                    "n_per_partition": n, "k": k, "minibatch": b, "learn_rate": w["eta"], "nnz": nnz, **ns,
-                   "parallelism": "none"},
+                   "m2_scaling": "paper", "end_to_end_schedule": f"{cfg.epochs - cfg.polish_epochs} {args.precision} "
+                   f"epoch(s) + {cfg.polish_epochs} FP32 polishing epoch(s)", "parallelism": "none"},
         "end_to_end_s": t_e2e,
-        "stages_s": {"upload_and_transpose": t_up, "whiten": t_wh, "stgd_1_epoch": t_st, "eigen_post": t_post,
+        "stages_s": {"upload_and_transpose": t_up, "whiten": t_wh, "stgd_schedule": t_st, "eigen_post": t_post,
                      "download_pi": t_dl, "generate": t_gen},
         "diagnostics": diag,
         "stgd_nodes_per_s": {pr: v[0] for pr, v in rates.items()},
         "bf16_speedup_vs_fp32": rates["bf16"][0] / rates["fp32"][0],
         "accuracy": accuracy,
+        "recovery": recovery,
         "roofline": {"bound": "mfma", "achieved": p1_achieved and p1_achieved / 1e12, "peak": peak / 1e12,
                      "unit": "TFLOP/s", "frac": p1_achieved and p1_achieved / peak,
                      "traffic": measured_traffic(args.workload, prec_used),
@@ -332,13 +359,18 @@ def accuracy_vs_oracle(m, p, phi0, n, k, b, w, spe, epochs):
            "reference": "oracle FP64 STGD (numpy) on the same whitened vectors, renormalised by two degree-3 "
                         "Newton-Schulz iterations per step",
            "phi_rel_err": {}, "pi_x_rel_err": {}, "pin_x_rel_err": {}, "lambda_max_abs_err": {},
-           "oracle_phi_moved": rel(phi64, phi0)}
+           "membership_agreement": {}, "oracle_phi_moved": rel(phi64, phi0)}
     for name, phi in runs.items():
         pi, pin, lam = post_of(phi)
         acc["phi_rel_err"][name] = rel(phi, phi64)
         acc["pi_x_rel_err"][name] = rel(pi, pi64)
         acc["pin_x_rel_err"][name] = rel(pin, pin64)
         acc["lambda_max_abs_err"][name] = float(np.abs(lam - lam64).max())
+        acc["membership_agreement"][name] = membership_agreement(pin, pi64)
+    acc["membership_rule"] = ("memberships {c : Pi-hat[x, c] > 0} of every X node (normProbVector's sign flip and "
+                              "clamp, Probability.cpp:33-57, tau = 0) against those of the oracle's Phi; "
+                              "'guarded' leaves out nodes whose positive and negative mass tie within eps = 1e-3 "
+                              "(relative) and entries within eps of zero, where the decision is set by rounding")
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     cpu = {"value": epochs * n / t_oracle, "unit": "nodes/s", "cores": threads, "kind": "port",
            "sample": f"{epochs} full STGD epochs (n={n}, b={b}, k={k}) of the oracle's FP64 numpy step on the "
@@ -346,17 +378,100 @@ def accuracy_vs_oracle(m, p, phi0, n, k, b, w, spe, epochs):
     return acc, cpu
 
 
-def cpu_e2e_estimate(m, w, n, ns, gpu_e2e_s):
-    """The oracle's whole pipeline (FP64 numpy / scipy: whitening, y vectors, one STGD epoch,
-    eigenvalues, Pi and its normalisation) timed on a sample of the workload -- ns nodes per partition,
-    same k, alpha0, degree distribution and mean degree -- and scaled by n / ns. Every stage is linear in
-    n at fixed k and degree (SpMMs in nnz * r, tall products in n r^2; the r x r factorisations are a
-    fixed cost), and the full size only adds cache misses, so the scaled figure underestimates the
-    CPU's time at n. The graph generation is not timed."""
+def membership_agreement(pin, pi64, eps=1e-3):
+    """Membership sets of the normalised Pi-hat against those the oracle's raw Pi implies, per node:
+    the fraction of nodes whose sets are identical, over all nodes and over the nodes outside the
+    guard band (a sign tie within eps of the node's mass, or an entry within eps of its largest)."""
+    import numpy as np
+    from oracle import core
+    pin64 = core.norm_prob_rows(pi64)
+    a, b = pin > 0, pin64 > 0
+    same = (a == b).all(1)
+    pos = np.where(pi64 > 0, pi64, 0).sum(1)
+    neg = -np.where(pi64 > 0, 0, pi64).sum(1)
+    clear = np.abs(pos - neg) > eps * (pos + neg)
+    big = np.abs(pi64) > eps * np.abs(pi64).max(1, keepdims=True)
+    same_g = ((a == b) | ~big).all(1)
+    return {"nodes_identical": float(same.mean()), "guarded_nodes": float(clear.mean()),
+            "guarded_identical": float(same_g[clear].mean()) if clear.any() else None}
+
+
+def matched_recovery(pin, truth):
+    """The normalised memberships Pi-hat of the X nodes against the generator's Dirichlet memberships:
+    the correlations (evaluate.py, after CPU/Pvalue.cpp) of every estimated with every true community,
+    matched one to one (Hungarian assignment); the mean and the 10th percentile of the matched
+    correlations, and the recovery ratio of evaluate.py's p-value test."""
+    import importlib
+    import numpy as np
+    from scipy.optimize import linear_sum_assignment
+    ev = importlib.import_module("mmsb_amd.evaluate")
+    n = pin.shape[0]
+    rows = np.random.default_rng(0).choice(n, min(n, 200000), replace=False)
+    est, tru = pin[rows].astype(np.float64), truth[rows].astype(np.float64)
+    r = np.nan_to_num(ev.correlations(est, tru))
+    ri, ci = linear_sum_assignment(-r)
+    mc = r[ri, ci]
+    p = ev.pvalues(r, est.shape[0])
+    return {"matched_corr_mean": float(mc.mean()), "matched_corr_p10": float(np.percentile(mc, 10)),
+            "recovery_ratio_p01": float((p < 0.01).any(0).mean()), "nodes_scored": int(est.shape[0])}
+
+
+def recovery_run(m):
+    """End to end on a graph of the same size class (1M nodes) on which the method recovers the planted
+    communities, with the schedule that recovers them, scored by matched_recovery."""
+    import numpy as np
+    R = RECOVERY
+    n, k = R["n"], R["k"]
+    g = m.generate(n, n, n, n, k, R["alpha0"], R["d_in"] / (n / k), R["d_out"] / n, seed=0, threads=16,
+                   keep_truth=True)
+    x_nodes = np.nonzero(g["part"] == 0)[0]
+    truth = np.zeros((n, k))
+    truth[g["local"][x_nodes]] = g["pi"][x_nodes]
+    for key in ("pi", "theta", "part", "local"):
+        del g[key]
+    cfg = m.config(nx=n, na=n, nb=n, nc=n, ncom=k, alpha_0=R["alpha0"], learn_rate=R["eta"], minibatch=R["b"],
+                   epochs=R["epochs"] + R["polish"], polish_epochs=R["polish"], oversample=10, power_iters=1, seed=1,
+                   precision="bf16")
+    t0 = time.time()
+    p = m.Pipeline(cfg)
+    p.set_graph(g)
+    p.whiten()
+    p.stgd(-1, "bf16")
+    p.eigen()
+    p.post()
+    pin = p.get_f32("pin_x")
+    t = time.time() - t0
+    p.close()
+    out = matched_recovery(pin, truth)
+    out.update({"end_to_end_s": t, "edges": int(g["n_edges"]),
+                "graph": f"{n} nodes/partition, k={k}, alpha0={R['alpha0']}, {R['d_in']:g} / {R['d_out']:g} expected "
+                         f"neighbours per partition inside / outside the community",
+                "schedule": f"{R['epochs']} BF16 + {R['polish']} FP32 epochs, minibatch {R['b']}, eta {R['eta']:g}"})
+    return out
+
+
+def cpu_e2e_sample(m, w, ns, gpu_e2e_s, n):
+    """The oracle's whole pipeline (FP64 numpy / scipy) and the library's, on the same sample graph of the
+    workload family (ns nodes per partition, same k, alpha0, degree distribution and mean degree): a
+    measured CPU / GPU ratio at the sample size, each stage timed. The ratio at the full size is not
+    extrapolated from it."""
     import numpy as np
     from oracle import core
     p_in, p_out = c3_probs(ns, w["k"], w["mean_degree"]) if "mean_degree" in w else (w["p_in"], w["p_out"])
     g = m.generate(ns, ns, ns, ns, w["k"], w["alpha0"], p_in, p_out, seed=0, degree_corrected=w["dc"], threads=16)
+    # the library on the sample: CSR on host -> whitening -> one STGD epoch -> eigen -> Pi-hat on host
+    cfg = m.config(nx=ns, na=ns, nb=ns, nc=ns, ncom=w["k"], alpha_0=w["alpha0"], learn_rate=w["eta"], minibatch=w["b"],
+                   epochs=1, polish_epochs=0, oversample=10, power_iters=1, seed=1)
+    t0 = time.time()
+    p = m.Pipeline(cfg)
+    p.set_graph(g)
+    p.whiten()
+    p.stgd(-1, "fp32")
+    p.eigen()
+    p.post()
+    _ = [p.get_f32("pin_" + s_) for s_ in "xabc"]
+    gpu_s = time.time() - t0
+    p.close()
     bl = {nm: core.Csr(g[nm][0], g[nm][1], g[nm][3], g[nm][4]) for nm in ("XA", "XB", "XC", "BA", "CA")}
     del g
     wcfg = core.WhitenConfig(k=w["k"], alpha0=w["alpha0"], oversample=10, power_iters=1, seed=1)
@@ -376,13 +491,13 @@ def cpu_e2e_estimate(m, w, n, ns, gpu_e2e_s):
     pis = core.post_proc(wh, phi, lam_p, bl["XA"], bl["BA"], bl["CA"])
     _ = [core.norm_prob_rows(P) for P in pis]
     t["eigen_post"] = time.time() - t0
-    scale = n / ns
-    est = scale * sum(t.values())
+    cpu_s = sum(t.values())
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    return {"estimate_s": est, "gpu_end_to_end_s": gpu_e2e_s, "speedup": est / gpu_e2e_s, "cores": threads,
-            "kind": "port", "sample_nodes_per_partition": ns, "scale": scale,
-            "sample_s": {key: round(val, 3) for key, val in t.items()},
-            "sample": f"oracle pipeline on {ns} nodes/partition of the same workload family, scaled by {scale:g}"}
+    return {"cpu_s": cpu_s, "gpu_s": gpu_s, "speedup": cpu_s / gpu_s, "cores": threads, "kind": "port",
+            "sample_nodes_per_partition": ns, "cpu_stages_s": {key: round(val, 3) for key, val in t.items()},
+            "gpu_end_to_end_full_size_s": gpu_e2e_s, "full_nodes_per_partition": n,
+            "sample": f"oracle pipeline (FP64 numpy/scipy) and the library, each end to end with one FP32 STGD epoch, "
+                      f"on the same {ns}-node-per-partition graph of the workload family"}
 
 
 def spmm_roofline(m, blk, r):
@@ -619,7 +734,7 @@ def bench_batch(m, args):
     dt_seq = (time.time() - ts) / args.steps
     log(f"batched {1e3 * dt_batch:.2f} ms/epoch, sequential {1e3 * dt_seq:.2f} ms/epoch")
     out = {
-        "metric": "batched STGD nodes/s (64 uneven instances, one persistent launch per epoch)",
+        "metric": "batched STGD nodes/s (64 uneven instances, every instance's epoch in one launch plan)",
         "value": total_n / dt_batch,
         "unit": "nodes/s",
         "n_gpus": 1,
@@ -635,7 +750,11 @@ def bench_batch(m, args):
                                "alpha0 U[0.5, 3], P 0.02 / 5e-4 scaled by min(1, 5k/n), minibatch 512; "
                                "one step = one epoch of every instance",
                    "instances": count, "total_nodes_per_partition": total_n, "minibatch": b,
-                   "autotuned_grid": grid, "parallelism": "none"},
+                   "autotuned_grid": grid,
+                   "launch_shape": (f"one persistent launch of {grid} workgroups per epoch" if grid > 0 else
+                                    "one grouped launch per phase slot, replayed from a captured HIP graph"),
+                   "degree_scaling": "P_in, P_out times min(1, 5000 / n): mean degree 10-50 at every n",
+                   "parallelism": "none"},
         "sequential_nodes_per_s": total_n / dt_seq,
         "batch_speedup_vs_sequential": dt_seq / dt_batch,
     }
