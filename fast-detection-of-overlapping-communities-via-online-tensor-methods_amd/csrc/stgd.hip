@@ -15,12 +15,13 @@
 // (c3, c1, c0) are the entries of the 4x4 matrices of init_buff (stpm.c:1026-1038).
 //
 // Work decomposition: every GEMM output is cut into 32 x 32 blocks, one per workgroup of four
-// waves. The waves split the reduction dimension and load their MFMA fragments straight from
-// L2/HBM into registers (each wave owns its K range, so nothing is shared through LDS) through a
-// ring of D fragment sets, so that up to D groups of loads are in flight ahead of the MFMAs (in
-// P1's data tiles each wave computes both Y_A Phi and Y_B Phi over its K range, loading each Phi^T
-// fragment once for the two). The partial accumulators meet in padded LDS tiles that all 256
-// threads read back for coalesced epilogue stores. At k = 500 a phase launches 256-768 workgroups.
+// waves. The waves split the reduction dimension (each wave owns its K range) and load their
+// operand blocks with whole-cache-line loads, rearranged into the MFMA layout through a small
+// per-wave LDS buffer (the staging section below), through a ring of D fragment sets, so that up to
+// D groups of loads are in flight ahead of the MFMAs (in P1's data tiles each wave computes both
+// Y_A Phi and Y_B Phi over its K range, loading each Phi^T block once for the two). The partial
+// accumulators meet in padded LDS tiles that all 256 threads read back for coalesced epilogue
+// stores. At k = 500 a phase launches 256-768 workgroups.
 // K order inside a fragment: for v_mfma_f32_32x32x2f32 lane half h holds k = 4h + i of a chunk
 // of eight k for instruction i (the order of a sum over k is free as long as A and B agree);
 // for v_mfma_f32_32x32x16_bf16 lane half h holds the eight contiguous k = 8h .. 8h + 7.
@@ -65,95 +66,160 @@ __host__ __device__ inline Opnd<T> opnd(const T* hi, const T* lo, int64_t ld) {
   return Opnd<T>{hi, lo ? lo : hi, ld};
 }
 
-template <bool AL>
-__device__ inline float4 ld4(const float* p) {
-  if constexpr (AL) return *reinterpret_cast<const float4*>(p);
-  return make_float4(p[0], p[1], p[2], p[3]);
-}
+// Waves per workgroup. Every tile's reduction is split over all of them (kWaves parts of the k or
+// minibatch range), which keeps kWaves waves of loads in flight per tile -- with one tile per compute
+// unit at k ~ 500, the only source of memory-level parallelism. The epilogues run on the first four
+// waves (threads 0..255), which own one element group of four each.
+#ifndef MMSB_STGD_WAVES
+#define MMSB_STGD_WAVES 4
+#endif
+constexpr int kWaves = MMSB_STGD_WAVES;
+constexpr int kThreads = 64 * kWaves;
+static_assert(kWaves == 4 || kWaves == 8, "4 or 8 waves per workgroup");
 
-template <bool AL>
-__device__ inline bf16x8 ld8(const __bf16* p) {
-  if constexpr (AL) return *reinterpret_cast<const bf16x8*>(p);
-  bf16x8 v;
+// ---- operand staging ---------------------------------------------------------------------------
+// One reduction group of a wave's MFMA operand is a block of 32 rows x 128 bytes (32 FP32 or 64 BF16
+// values per row). Read in the MFMA layout -- lane (r, h) taking row r -- every load instruction
+// would touch 32 cache lines and use 32 bytes of each, and with a dozen waves per compute unit
+// streaming such blocks the L1 refetches each line for each of its four 32-byte pieces. Instead
+// every load instruction reads 8 whole 128-byte row segments (lane 8q + s takes 16-byte segment s of
+// row q, plus 8 rows per instruction), and a transpose through the wave's own 4 KiB LDS buffer then
+// leaves register c of lane (r, h) with segment 2c + h of row r: k = 8c + 4h .. +3 (FP32) or
+// 16c + 8h .. +7 (BF16) of the group, the k order the MFMA sequences below assume for A and B alike
+// (the order of a sum over k is free as long as both operands agree). Segment s of row q sits at
+// slot s ^ ((q >> 1) & 7) of the buffer row, which keeps both the writes (8 rows x 8 slots per
+// instruction) and the reads (32 rows x one segment per lane half) on distinct banks in every group
+// of 16 lanes. LDS instructions of one wave execute in order, so the reads see the writes and the next
+// block's writes land after the reads; the waves of a workgroup use separate buffers and never wait
+// on each other here.
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
+template <bool AL, typename T>
+__device__ inline u32x4 ld16(const T* p) {
+  if constexpr (AL) return *reinterpret_cast<const u32x4*>(p);
+  constexpr int E = 16 / sizeof(T);  // rows not 16-byte aligned: element by element
+  T e[E];
 #pragma unroll
-  for (int i = 0; i < 8; ++i) v[i] = p[i];
-  return v;
+  for (int i = 0; i < E; ++i) e[i] = p[i];
+  return __builtin_bit_cast(u32x4, e);
 }
 
-// Reduction group of one fragment set: 32 k for FP32, 64 k for the BF16 modes.
+// The four coalesced loads of one block; row0 points at the block's first row and first k.
+template <bool AL, typename T>
+__device__ inline void stage_load(u32x4 v[4], const T* row0, int64_t ld) {
+  const int lane = threadIdx.x & 63;
+  const T* p = row0 + (int64_t)(lane >> 3) * ld + (lane & 7) * (16 / (int)sizeof(T));
+#pragma unroll
+  for (int c = 0; c < 4; ++c) v[c] = ld16<AL>(p + 8 * c * ld);
+}
+
+__device__ inline u32x4* wave_stage() {
+  __shared__ u32x4 buf[kWaves][256];
+  return buf[threadIdx.x >> 6];
+}
+
+// Rearranges a block loaded by stage_load into the MFMA layout, in place.
+__device__ inline void stage_swap(u32x4 v[4]) {
+  u32x4* buf = wave_stage();
+  const int lane = threadIdx.x & 63;
+  const int q = lane >> 3, s = lane & 7, r = lane & 31, h = lane >> 5;
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    const int row = q + 8 * c;
+    buf[row * 8 + (s ^ ((row >> 1) & 7))] = v[c];
+  }
+  asm volatile("" ::: "memory");  // compiler ordering only: the reads follow the writes
+#pragma unroll
+  for (int c = 0; c < 4; ++c) v[c] = buf[r * 8 + ((2 * c + h) ^ ((r >> 1) & 7))];
+  asm volatile("" ::: "memory");  // ... and the next block's writes follow these reads
+}
+
+__device__ inline floatx4 as_f32(const u32x4& v) { return __builtin_bit_cast(floatx4, v); }
+__device__ inline bf16x8 as_bf16(const u32x4& v) { return __builtin_bit_cast(bf16x8, v); }
+
+// Reduction group of one fragment set: 32 k for FP32, 64 k for the BF16 modes (128 bytes per row).
 template <int MODE>
 constexpr int kgroup() { return MODE == MM_F32 ? 32 : 64; }
 
-// Fragment set of one reduction group for one wave.
+// Fragment set of one reduction group for one wave: load() issues the coalesced loads of its blocks
+// (A rows from pa, B rows from pb, both at the group's first k), swap() puts them in the MFMA layout
+// once they are needed, mma() accumulates.
 template <int MODE>
 struct Frag;
 
 template <>
 struct Frag<MM_F32> {
-  float4 a[4], b[4];
+  u32x4 a[4], b[4];
   template <bool AAL>
-  __device__ inline void load(const float* pa, const float*, const float* pb, const float*, int h) {
-#pragma unroll
-    for (int c = 0; c < 4; ++c) {
-      a[c] = ld4<AAL>(pa + 8 * c + 4 * h);
-      b[c] = ld4<true>(pb + 8 * c + 4 * h);
-    }
+  __device__ inline void load(const float* pa, const float*, int64_t lda, const float* pb, const float*, int64_t ldb) {
+    stage_load<AAL>(a, pa, lda);
+    stage_load<true>(b, pb, ldb);
+  }
+  __device__ inline void swap() {
+    stage_swap(a);
+    stage_swap(b);
   }
   __device__ inline void mma(floatx16& acc) const {
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
-      acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a[c].x, b[c].x, acc, 0, 0, 0);
-      acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a[c].y, b[c].y, acc, 0, 0, 0);
-      acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a[c].z, b[c].z, acc, 0, 0, 0);
-      acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a[c].w, b[c].w, acc, 0, 0, 0);
+      const floatx4 x = as_f32(a[c]), y = as_f32(b[c]);
+      acc = __builtin_amdgcn_mfma_f32_32x32x2f32(x[0], y[0], acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f32_32x32x2f32(x[1], y[1], acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f32_32x32x2f32(x[2], y[2], acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f32_32x32x2f32(x[3], y[3], acc, 0, 0, 0);
     }
   }
 };
 
 template <>
 struct Frag<MM_BF16> {
-  bf16x8 a[4], b[4];
+  u32x4 a[4], b[4];
   template <bool AAL>
-  __device__ inline void load(const __bf16* pa, const __bf16*, const __bf16* pb, const __bf16*, int h) {
-#pragma unroll
-    for (int c = 0; c < 4; ++c) {
-      a[c] = ld8<AAL>(pa + 16 * c + 8 * h);
-      b[c] = ld8<true>(pb + 16 * c + 8 * h);
-    }
+  __device__ inline void load(const __bf16* pa, const __bf16*, int64_t lda, const __bf16* pb, const __bf16*,
+                              int64_t ldb) {
+    stage_load<AAL>(a, pa, lda);
+    stage_load<true>(b, pb, ldb);
+  }
+  __device__ inline void swap() {
+    stage_swap(a);
+    stage_swap(b);
   }
   __device__ inline void mma(floatx16& acc) const {
 #pragma unroll
-    for (int c = 0; c < 4; ++c) acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[c], b[c], acc, 0, 0, 0);
+    for (int c = 0; c < 4; ++c) acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(as_bf16(a[c]), as_bf16(b[c]), acc, 0, 0, 0);
   }
 };
 
 template <>
 struct Frag<MM_BF16X3> {
-  bf16x8 ah[4], al[4], bh[4], bl[4];
+  u32x4 ah[4], al[4], bh[4], bl[4];
   template <bool AAL>
-  __device__ inline void load(const __bf16* pa, const __bf16* pal, const __bf16* pb, const __bf16* pbl, int h) {
-#pragma unroll
-    for (int c = 0; c < 4; ++c) {
-      ah[c] = ld8<AAL>(pa + 16 * c + 8 * h);
-      al[c] = ld8<AAL>(pal + 16 * c + 8 * h);
-      bh[c] = ld8<true>(pb + 16 * c + 8 * h);
-      bl[c] = ld8<true>(pbl + 16 * c + 8 * h);
-    }
+  __device__ inline void load(const __bf16* pa, const __bf16* pal, int64_t lda, const __bf16* pb, const __bf16* pbl,
+                              int64_t ldb) {
+    stage_load<AAL>(ah, pa, lda);
+    stage_load<AAL>(al, pal, lda);
+    stage_load<true>(bh, pb, ldb);
+    stage_load<true>(bl, pbl, ldb);
+  }
+  __device__ inline void swap() {
+    stage_swap(ah);
+    stage_swap(al);
+    stage_swap(bh);
+    stage_swap(bl);
   }
   __device__ inline void mma(floatx16& acc) const {
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
-      acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[c], bh[c], acc, 0, 0, 0);
-      acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[c], bl[c], acc, 0, 0, 0);
-      acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[c], bh[c], acc, 0, 0, 0);
+      const bf16x8 xh = as_bf16(ah[c]), xl = as_bf16(al[c]), yh = as_bf16(bh[c]), yl = as_bf16(bl[c]);
+      acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(xh, yh, acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(xh, yl, acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(xl, yh, acc, 0, 0, 0);
     }
   }
 };
 
 // Ring depth per mode (fragment registers: 32 VGPRs per set for FP32 and BF16, 64 for split BF16).
-// Two sets keep 16 (FP32, BF16) or 32 (split BF16) loads per wave in flight; deeper rings run into
-// the limit of 63 outstanding vector loads per wave and measured slower (P2: 9.0 -> 10.1 us in BF16
-// at depth 8).
+// Two sets keep 8 (FP32, BF16) or 16 (split BF16) loads per wave in flight ahead of the MFMAs.
 #ifndef MMSB_RING_F32
 #define MMSB_RING_F32 2
 #endif
@@ -189,23 +255,22 @@ __device__ inline void wave_mm(floatx16& acc, const Opnd<TA>& A, const Opnd<TB>&
   if (kl <= 0) return;
   constexpr int G = kgroup<MODE>();
   constexpr int D = ring_depth<MODE>();
-  const int lane = threadIdx.x & 63;
-  const int rr = lane & 31, h = lane >> 5;
-  const TA* a = A.hi + (r0 + rr) * A.ld + k0;
-  const TA* al = A.lo + (r0 + rr) * A.ld + k0;
-  const TB* b = B.hi + (c0 + rr) * B.ld + k0;
-  const TB* bl = B.lo + (c0 + rr) * B.ld + k0;
+  const TA* a = A.hi + r0 * A.ld + k0;
+  const TA* al = A.lo + r0 * A.ld + k0;
+  const TB* b = B.hi + c0 * B.ld + k0;
+  const TB* bl = B.lo + c0 * B.ld + k0;
   Frag<MODE> f[D];
 #pragma unroll
   for (int d = 0; d < D - 1; ++d)
-    if (d * G < kl) f[d].template load<AAL>(a + d * G, al + d * G, b + d * G, bl + d * G, h);
+    if (d * G < kl) f[d].template load<AAL>(a + d * G, al + d * G, A.ld, b + d * G, bl + d * G, B.ld);
   for (int64_t g = 0; g < kl; g += D * G) {
 #pragma unroll
     for (int d = 0; d < D; ++d) {
       const int64_t kk = g + d * G;
       if (kk < kl) {
         const int64_t kn = kk + (D - 1) * G;
-        if (kn < kl) f[(d + D - 1) % D].template load<AAL>(a + kn, al + kn, b + kn, bl + kn, h);
+        if (kn < kl) f[(d + D - 1) % D].template load<AAL>(a + kn, al + kn, A.ld, b + kn, bl + kn, B.ld);
+        f[d].swap();
         f[d].mma(acc);
       }
     }
@@ -217,39 +282,31 @@ __device__ inline void wave_mm(floatx16& acc, const Opnd<TA>& A, const Opnd<TB>&
 // once for both.
 template <int MODE>
 struct FragPair {
-  using V = typename std::conditional<MODE == MM_F32, float4, bf16x8>::type;
   using T = op_t<MODE>;
-  static constexpr int W = MODE == MM_F32 ? 4 : 8;  // elements per lane and chunk
-  V a1[4], a2[4], b[4];
-  __device__ inline void load(const T* p1, const T* p2, const T* pb, int h) {
-#pragma unroll
-    for (int c = 0; c < 4; ++c) {
-      if constexpr (MODE == MM_F32) {
-        a1[c] = ld4<true>(p1 + 8 * c + 4 * h);
-        a2[c] = ld4<true>(p2 + 8 * c + 4 * h);
-        b[c] = ld4<true>(pb + 8 * c + 4 * h);
-      } else {
-        a1[c] = ld8<true>(p1 + 16 * c + 8 * h);
-        a2[c] = ld8<true>(p2 + 16 * c + 8 * h);
-        b[c] = ld8<true>(pb + 16 * c + 8 * h);
-      }
-    }
+  u32x4 a1[4], a2[4], b[4];
+  __device__ inline void load(const T* p1, const T* p2, int64_t lda, const T* pb, int64_t ldb) {
+    stage_load<true>(a1, p1, lda);
+    stage_load<true>(a2, p2, lda);
+    stage_load<true>(b, pb, ldb);
+  }
+  __device__ inline void swap() {
+    stage_swap(a1);
+    stage_swap(a2);
+    stage_swap(b);
   }
   __device__ inline void mma(floatx16& acc_a, floatx16& acc_b) const {
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
       if constexpr (MODE == MM_F32) {
-        acc_a = __builtin_amdgcn_mfma_f32_32x32x2f32(a1[c].x, b[c].x, acc_a, 0, 0, 0);
-        acc_b = __builtin_amdgcn_mfma_f32_32x32x2f32(a2[c].x, b[c].x, acc_b, 0, 0, 0);
-        acc_a = __builtin_amdgcn_mfma_f32_32x32x2f32(a1[c].y, b[c].y, acc_a, 0, 0, 0);
-        acc_b = __builtin_amdgcn_mfma_f32_32x32x2f32(a2[c].y, b[c].y, acc_b, 0, 0, 0);
-        acc_a = __builtin_amdgcn_mfma_f32_32x32x2f32(a1[c].z, b[c].z, acc_a, 0, 0, 0);
-        acc_b = __builtin_amdgcn_mfma_f32_32x32x2f32(a2[c].z, b[c].z, acc_b, 0, 0, 0);
-        acc_a = __builtin_amdgcn_mfma_f32_32x32x2f32(a1[c].w, b[c].w, acc_a, 0, 0, 0);
-        acc_b = __builtin_amdgcn_mfma_f32_32x32x2f32(a2[c].w, b[c].w, acc_b, 0, 0, 0);
+        const floatx4 x1 = as_f32(a1[c]), x2 = as_f32(a2[c]), y = as_f32(b[c]);
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          acc_a = __builtin_amdgcn_mfma_f32_32x32x2f32(x1[e], y[e], acc_a, 0, 0, 0);
+          acc_b = __builtin_amdgcn_mfma_f32_32x32x2f32(x2[e], y[e], acc_b, 0, 0, 0);
+        }
       } else {
-        acc_a = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1[c], b[c], acc_a, 0, 0, 0);
-        acc_b = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a2[c], b[c], acc_b, 0, 0, 0);
+        acc_a = __builtin_amdgcn_mfma_f32_32x32x16_bf16(as_bf16(a1[c]), as_bf16(b[c]), acc_a, 0, 0, 0);
+        acc_b = __builtin_amdgcn_mfma_f32_32x32x16_bf16(as_bf16(a2[c]), as_bf16(b[c]), acc_b, 0, 0, 0);
       }
     }
   }
@@ -261,36 +318,24 @@ __device__ inline void wave_mm_pair(floatx16& acc_a, floatx16& acc_b, const op_t
   if (kl <= 0) return;
   constexpr int G = kgroup<MODE>();
   constexpr int D = 2;
-  const int lane = threadIdx.x & 63;
-  const int rr = lane & 31, h = lane >> 5;
-  const op_t<MODE>* a1 = A1 + (r0 + rr) * lda + k0;
-  const op_t<MODE>* a2 = A2 + (r0 + rr) * lda + k0;
-  const op_t<MODE>* b = B + (c0 + rr) * ldb + k0;
+  const op_t<MODE>* a1 = A1 + r0 * lda + k0;
+  const op_t<MODE>* a2 = A2 + r0 * lda + k0;
+  const op_t<MODE>* b = B + c0 * ldb + k0;
   FragPair<MODE> f[D];
-  f[0].load(a1, a2, b, h);
+  f[0].load(a1, a2, lda, b, ldb);
   for (int64_t g = 0; g < kl; g += D * G) {
 #pragma unroll
     for (int d = 0; d < D; ++d) {
       const int64_t kk = g + d * G;
       if (kk < kl) {
         const int64_t kn = kk + (D - 1) * G;
-        if (kn < kl) f[(d + D - 1) % D].load(a1 + kn, a2 + kn, b + kn, h);
+        if (kn < kl) f[(d + D - 1) % D].load(a1 + kn, a2 + kn, lda, b + kn, ldb);
+        f[d].swap();
         f[d].mma(acc_a, acc_b);
       }
     }
   }
 }
-
-// Waves per workgroup. Every tile's reduction is split over all of them (kWaves parts of the k or
-// minibatch range), which keeps kWaves waves of loads in flight per tile -- with one tile per compute
-// unit at k ~ 500, the only source of memory-level parallelism. The epilogues run on the first four
-// waves (threads 0..255), which own one element group of four each.
-#ifndef MMSB_STGD_WAVES
-#define MMSB_STGD_WAVES 4
-#endif
-constexpr int kWaves = MMSB_STGD_WAVES;
-constexpr int kThreads = 64 * kWaves;
-static_assert(kWaves == 4 || kWaves == 8, "4 or 8 waves per workgroup");
 
 __device__ inline bool epi_thread() { return threadIdx.x < 256; }
 
