@@ -108,6 +108,7 @@ struct mmsb_ctx {
   float *phit = nullptr, *gt = nullptr, *phia = nullptr, *h = nullptr, *cgt = nullptr, *cs_part = nullptr, *mab = nullptr;
   unsigned long long* ns_rowsum = nullptr;
   float *mpart = nullptr, *nsq = nullptr;
+  double* fpart = nullptr;
   bf16_bits *phi16[2] = {nullptr, nullptr}, *phit16 = nullptr, *phia16 = nullptr, *phiT16 = nullptr, *phitT16 = nullptr,
             *phiaT16 = nullptr, *gt16 = nullptr, *h16 = nullptr, *nsq16 = nullptr;
   double* frob = nullptr;
@@ -534,6 +535,7 @@ static int alloc_stgd(mmsb_ctx* c) {
   TRY(dalloc(c, c->mab, (size_t)2 * c->kp));
   TRY(dalloc(c, c->ns_rowsum, (size_t)2 * c->kp));
   TRY(dalloc(c, c->mpart, (size_t)(c->kp / 32) * 2 * c->kp));
+  TRY(dalloc(c, c->fpart, (size_t)(c->kp / 32) * (c->kp / 32)));
   TRY(dalloc(c, c->phiT, kk));
   TRY(dalloc(c, c->phitT, kk));
   TRY(dalloc(c, c->phiaT, kk));
@@ -644,6 +646,7 @@ static StgdBuffers stgd_bufs(const mmsb_ctx* c) {
   bufs.cgt = c->cgt;
   bufs.rowsum = c->ns_rowsum;
   bufs.mpart = c->mpart;
+  bufs.fpart = c->fpart;
   return bufs;
 }
 

@@ -69,6 +69,7 @@ struct StgdBuffers {
   void* cgt;            // [kp x b] C_g^T: FP32, or BF16 in BF16 mode
   unsigned long long* rowsum;  // [2 x kp] row sums of |X^T X| in 2^-32 fixed point (Newton-Schulz scale guard)
   float* mpart;         // [kp/32 x 2 x kp] per-row-block partial sums of mu_A^T Phi, mu_B^T Phi
+  double* fpart;        // [(kp/32)^2] per-tile partial sums of the last finished step's log entry
 };
 
 // Optional per-phase timing of direct (uncaptured) steps: ev[0..3] are caller-created events

@@ -104,18 +104,25 @@ __device__ inline u32x4 ld16(const T* p) {
   return __builtin_bit_cast(u32x4, e);
 }
 
-// The four coalesced loads of one block; row0 points at the block's first row and first k.
+// The four coalesced loads of one block; row0 (the block's first row and first k) and ld are
+// wave-uniform. Each address is that uniform base plus 8c rows plus one 32-bit per-lane byte offset,
+// the form the compiler issues with a scalar base register: one offset VGPR per operand instead of a
+// 64-bit address pair per load.
 template <bool AL, typename T>
 __device__ inline void stage_load(u32x4 v[4], const T* row0, int64_t ld) {
-  const int lane = threadIdx.x & 63;
-  const T* p = row0 + (int64_t)(lane >> 3) * ld + (lane & 7) * (16 / (int)sizeof(T));
+  const uint32_t lane = threadIdx.x & 63;
+  const uint32_t off = ((lane >> 3) * (uint32_t)ld + (lane & 7) * (16 / (uint32_t)sizeof(T))) * (uint32_t)sizeof(T);
+  const char* base = reinterpret_cast<const char*>(row0);
 #pragma unroll
-  for (int c = 0; c < 4; ++c) v[c] = ld16<AL>(p + 8 * c * ld);
+  for (int c = 0; c < 4; ++c) v[c] = ld16<AL>(reinterpret_cast<const T*>(base + 8 * c * ld * (int64_t)sizeof(T) + off));
 }
+
+// The calling wave's index within its workgroup, as a wave-uniform (scalar) value.
+__device__ inline int wave_id() { return __builtin_amdgcn_readfirstlane(threadIdx.x >> 6); }
 
 __device__ inline u32x4* wave_stage() {
   __shared__ u32x4 buf[kWaves][256];
-  return buf[threadIdx.x >> 6];
+  return buf[wave_id()];
 }
 
 // Rearranges a block loaded by stage_load into the MFMA layout, in place.
@@ -430,12 +437,13 @@ struct TileShm {
 
 // Writes block (i0, j0) of a new kp x kp iterate held as v[e] = element (i0 + r, j0 + c4 + e), with
 // r = thread / 8 and c4 = 4 (thread % 8): Out row-major, OutT transposed (FP32 and, with SPLIT, BF16
-// pairs), frob += ||Out - prev||_F^2 over the block when frob is given (prev in pre.pv), and, when
+// pairs), *fslot = ||Out - prev||_F^2 over the block when fslot is given (prev in pre.pv; the step's
+// log entry is the sum of its tiles' slots, see frob_sum), and, when
 // mpart is given, mpart[(ti * 2 + s) * kp + j] = sum over the block's rows i of mu_s[i] Out[i][j]
 // (s = A, B; mu in pre.ua, pre.ub). Every thread of the block calls it.
 template <bool SPLIT>
 __device__ inline void finish_tile(TileShm& S, int kp, int ti, int64_t i0, int64_t j0, const float v[4], float* Out,
-                                   uint16_t* Out16, float* OutT, uint16_t* OutT16, const TilePre& pre, double* frob,
+                                   uint16_t* Out16, float* OutT, uint16_t* OutT16, const TilePre& pre, double* fslot,
                                    float* mpart) {
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int r = threadIdx.x >> 3, c4 = 4 * (threadIdx.x & 7);
@@ -446,7 +454,7 @@ __device__ inline void finish_tile(TileShm& S, int kp, int ti, int64_t i0, int64
   put4<SPLIT>(Out, Out16, nel, off, v);
 #pragma unroll
   for (int e = 0; e < 4; ++e) S.t[c4 + e][r] = v[e];
-  if (frob) {
+  if (fslot) {
     double d = 0.0;
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
@@ -482,7 +490,7 @@ __device__ inline void finish_tile(TileShm& S, int kp, int ti, int64_t i0, int64
     const float t[4] = {S.t[c][r4], S.t[c][r4 + 1], S.t[c][r4 + 2], S.t[c][r4 + 3]};
     put4<SPLIT>(OutT, OutT16, nel, (j0 + c) * kp + i0 + r4, t);
   }
-  if (frob && threadIdx.x == 0) atomicAdd(frob, (S.d[0] + S.d[1]) + (S.d[2] + S.d[3]));
+  if (fslot && threadIdx.x == 0) *fslot = (S.d[0] + S.d[1]) + (S.d[2] + S.d[3]);
   if (mpart && threadIdx.x < 64) {
     const int s = threadIdx.x >> 5, c = threadIdx.x & 31;
     mpart[((int64_t)ti * 2 + s) * kp + j0 + c] = (S.m[0][s][c] + S.m[1][s][c]) + (S.m[2][s][c] + S.m[3][s][c]);
@@ -504,10 +512,13 @@ __device__ inline void store4(T* p, const float v[4]) {
 // ---- the phases of one step, as per-tile device functions ----------------------------------------
 // A step is the phase sequence P1, P2, then ns_iters x (GRAM, UPDATE) -- (GRAM, QPOLY, UPDATE) for
 // degree-5 iterations -- or COPY when the step does not renormalise; SETUP precedes the first step of
-// a run. In the reference's QR mode a renormalising step is P1, P2, the host's QR launches and QRFIN. The same tile functions serve the per-phase
-// launches of one instance (stgd_phase_kernel, blockIdx.x = tile) and the persistent launch over many
-// instances (stgd_batch_kernel, tiles from a work queue).
-enum StgdPhase { PH_SETUP = 0, PH_P1 = 1, PH_P2 = 2, PH_GRAM = 3, PH_UPDATE = 4, PH_COPY = 5, PH_QPOLY = 6, PH_QRFIN = 7 };
+// a run and FROB follows its last. In the reference's QR mode a renormalising step is P1, P2, the
+// host's QR launches and QRFIN. The kernel that finishes a step leaves one partial of the step's log
+// entry per tile; the next step's P1 (or FROB) sums them in a fixed order. The same tile functions
+// serve the per-phase launches of one instance (stgd_phase_kernel, blockIdx.x = tile) and the
+// persistent launch over many instances (stgd_batch_kernel, tiles from a work queue).
+enum StgdPhase { PH_SETUP = 0, PH_P1 = 1, PH_P2 = 2, PH_GRAM = 3, PH_UPDATE = 4, PH_COPY = 5, PH_QPOLY = 6, PH_QRFIN = 7,
+                 PH_FROB = 8 };
 
 // Everything the phases of one step of one instance read or write.
 struct StepCtx {
@@ -523,8 +534,9 @@ struct StepCtx {
   int64_t yrow0;       // its first row in the Y buffers (row0, or 0 when they hold only the minibatch)
   int32_t nb;          // rows in the minibatch
   int64_t st;          // step index within the run
-  const double* frob_prev;  // the previous step's log entry within the run (null for the run's first)
-  double* frob;        // this step's ||Phi_{t+1} - Phi_t||_F^2
+  double* frob_prev;   // the previous step's log entry within the run (null for the run's first), formed by
+                       // this step's P1 from the previous step's tile partials
+  double* frob;        // this step's ||Phi_{t+1} - Phi_t||_F^2 (written by FROB after a run's last step)
   double* frob_zero;   // SETUP: the run's frob entries, cleared by tile 0
   int64_t nfrob;
 };
@@ -543,9 +555,38 @@ union StgdSmem {
   TileShm S;
 };
 
+// Tiles of a phase. The symmetric products -- H = (Phi^T Phi)^o2 in P1, G = X^T X in GRAM -- are
+// computed on the upper triangle of 32 x 32 blocks only, each off-diagonal block written twice.
 __host__ __device__ inline int phase_tiles(int phase, int kp, int b) {
-  const int ntk = kp / 32;
-  return phase == PH_P1 ? (b / 32) * ntk + ntk * ntk : ntk * ntk;
+  const int ntk = kp / 32, tri = ntk * (ntk + 1) / 2;
+  switch (phase) {
+    case PH_P1: return (b / 32) * ntk + tri;
+    case PH_GRAM: return tri;
+    case PH_FROB: return 1;
+    default: return ntk * ntk;
+  }
+}
+
+// Block (ti, tj), ti <= tj, of upper-triangle tile t (row-major over the triangle).
+__device__ inline void tri_tile(int t, int ntk, int& ti, int& tj) {
+  int i = 0;
+  while (t >= ntk - i) {
+    t -= ntk - i;
+    ++i;
+  }
+  ti = i;
+  tj = i + t;
+}
+
+// The log entry ||Phi_{t+1} - Phi_t||_F^2 of a finished step from its tiles' partials fpart[0 .. n):
+// lane-strided sums, then a butterfly -- a fixed order, so that every wave that forms the entry gets
+// the same bits. Called by whole waves.
+__device__ inline double frob_sum(const double* fpart, int n) {
+  const int lane = threadIdx.x & 63;
+  double s = 0.0;
+  for (int t = lane; t < n; t += 64) s += fpart[t];
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+  return s;
 }
 
 // Newton-Schulz iteration `it` reads (x, xT) and writes (dst, dstT): phit -> phia -> phit -> ...;
@@ -646,7 +687,8 @@ __device__ inline float cspart_reduce(const PartSum& ps, const float* cs_part, i
   return m;
 }
 
-// P1 tiles [0, ntk * ntk): H tiles; the rest: a 32-row slice of the minibatch x 32 columns of C_g.
+// P1 tiles [0, ntk (ntk + 1) / 2): upper-triangle H tiles; the rest: a 32-row slice of the minibatch x
+// 32 columns of C_g.
 template <int DMODE, int KMODE, typename TY, typename TPH>
 __device__ void p1_tile(const StepCtx& P, int tile, StgdSmem& sm) {
   constexpr bool SPLIT = KMODE == MM_BF16X3;
@@ -654,11 +696,12 @@ __device__ void p1_tile(const StepCtx& P, int tile, StgdSmem& sm) {
   const int kp = a.kp;
   const int ntk = kp / 32;
   const int64_t nel = (int64_t)kp * kp;
-  const int w = threadIdx.x >> 6;
+  const int w = wave_id();
+  const int tri = ntk * (ntk + 1) / 2;
   floatx16 acc = {};
   int64_t k0, kl;
-  if (tile >= ntk * ntk) {  // data tiles follow the H tiles, which then start in the first wave
-    tile -= ntk * ntk;
+  if (tile >= tri) {  // data tiles follow the H tiles, which then start in the first wave
+    tile -= tri;
     const int tb = tile / ntk, tj = tile % ntk;
     const int64_t x0 = (int64_t)tb * 32, j0 = (int64_t)tj * 32;
     PartSum ps;
@@ -702,7 +745,8 @@ __device__ void p1_tile(const StepCtx& P, int tile, StgdSmem& sm) {
       }
     }
   } else {
-    const int ti = tile / ntk, tj = tile % ntk;
+    int ti, tj;
+    tri_tile(tile, ntk, ti, tj);
     const int64_t i0 = (int64_t)ti * 32, j0 = (int64_t)tj * 32;
     const Opnd<op_t<KMODE>> phiT_k = pick<KMODE>(P.B.phiT, P.B.phiT16, nel, kp);
     kslice<KMODE>(kp, kWaves, w, k0, kl);
@@ -719,6 +763,14 @@ __device__ void p1_tile(const StepCtx& P, int tile, StgdSmem& sm) {
       g[e] *= g[e];
     }
     put4<SPLIT>(P.B.h, sp<SPLIT>(P.B.h16), nel, (i0 + r) * kp + j0 + c4, g);
+    if (ti != tj) {  // the mirror block (j, i): thread -> row c of it, columns r4 .. r4 + 3
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        g[e] = red_sum4(red, c4 + e, r);
+        g[e] *= g[e];
+      }
+      put4<SPLIT>(P.B.h, sp<SPLIT>(P.B.h16), nel, (j0 + r) * kp + i0 + c4, g);
+    }
   }
 }
 
@@ -729,7 +781,7 @@ __device__ void p2_tile(const StepCtx& P, int tile, StgdSmem& sm) {
   const StgdArgs& a = P.a;
   const int kp = a.kp, ntk = kp / 32;
   const int64_t nel = (int64_t)kp * kp;
-  const int w = threadIdx.x >> 6;
+  const int w = wave_id();
   const int ti = tile / ntk, tj = tile % ntk;
   const int64_t i0 = (int64_t)ti * 32, j0 = (int64_t)tj * 32;
   Red4& red = sm.p2.red;
@@ -788,7 +840,7 @@ __device__ void p2_tile(const StepCtx& P, int tile, StgdSmem& sm) {
 #pragma unroll
   for (int e = 0; e < 4; ++e) pre.pv[e] = p0[e];
   finish_tile<SPLIT>(sm.p2.S, kp, ti, i0, j0, v, a.phit, sp<SPLIT>(P.B.phit16), P.B.phitT, sp<SPLIT>(P.B.phitT16),
-                     pre, a.frob_pre ? P.frob : nullptr, nullptr);
+                     pre, a.frob_pre ? P.B.fpart + tile : nullptr, nullptr);
 }
 
 // The Newton-Schulz scale guard c = min(1, sqrt(2 / ||G||_inf)) from the row sums of |G|, computed
@@ -820,14 +872,16 @@ __device__ inline void guard_scale(const float rs[8], const unsigned long long* 
   if (lane == 0) *cscale = mx > 2.f ? sqrtf(2.f / mx) : 1.f;
 }
 
-// G = X^T X of Newton-Schulz iteration `it`, and rowsum[i] += sum over the tile's columns of |G_ij|.
+// G = X^T X of Newton-Schulz iteration `it` on upper-triangle tile `tile` (block and mirror), and
+// rowsum[i] += sum over the block's columns of |G_ij| for every row i the block and its mirror cover.
 template <int KMODE>
 __device__ void gram_tile(const StepCtx& P, int it, int tile, StgdSmem& sm) {
   constexpr bool SPLIT = KMODE == MM_BF16X3;
   const int kp = P.a.kp, ntk = kp / 32;
   const int64_t nel = (int64_t)kp * kp;
-  const int w = threadIdx.x >> 6;
-  const int ti = tile / ntk, tj = tile % ntk;
+  const int w = wave_id();
+  int ti, tj;
+  tri_tile(tile, ntk, ti, tj);
   const int64_t i0 = (int64_t)ti * 32, j0 = (int64_t)tj * 32;
   const NsBufs nb = ns_bufs(P.B, P.cur, it, P.ns_iters);
   unsigned long long* rowsum = P.B.rowsum + (it & 1) * kp;
@@ -850,6 +904,17 @@ __device__ void gram_tile(const StepCtx& P, int it, int tile, StgdSmem& sm) {
   put4<SPLIT>(P.B.gt, sp<SPLIT>(P.B.gt16), nel, (i0 + r) * kp + j0 + c4, g);
   for (int off = 1; off < 8; off <<= 1) s += __shfl_xor(s, off);
   if ((threadIdx.x & 7) == 0) atomicAdd(rowsum + i0 + r, fix32(s));
+  if (ti != tj) {  // the mirror block (j, i): thread -> row r of it, columns c4 .. c4 + 3
+    float t = 0.f;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      g[e] = red_sum4(red, c4 + e, r);
+      t += fabsf(g[e]);
+    }
+    put4<SPLIT>(P.B.gt, sp<SPLIT>(P.B.gt16), nel, (j0 + r) * kp + i0 + c4, g);
+    for (int off = 1; off < 8; off <<= 1) t += __shfl_xor(t, off);
+    if ((threadIdx.x & 7) == 0) atomicAdd(rowsum + j0 + r, fix32(t));
+  }
 }
 
 // Degree-5 iteration, second stage: Q = -(5/4) c^2 G + (3/8) c^4 G^2 (symmetric, as G), so that the
@@ -859,7 +924,7 @@ __device__ void qpoly_tile(const StepCtx& P, int it, int tile, StgdSmem& sm) {
   constexpr bool SPLIT = KMODE == MM_BF16X3;
   const int kp = P.a.kp, ntk = kp / 32;
   const int64_t nel = (int64_t)kp * kp;
-  const int w = threadIdx.x >> 6;
+  const int w = wave_id();
   const int ti = tile / ntk, tj = tile % ntk;
   const int64_t i0 = (int64_t)ti * 32, j0 = (int64_t)tj * 32;
   const unsigned long long* rowsum = P.B.rowsum + (it & 1) * kp;
@@ -894,7 +959,7 @@ __device__ void update_tile(const StepCtx& P, int it, int tile, StgdSmem& sm) {
   constexpr bool SPLIT = KMODE == MM_BF16X3;
   const int kp = P.a.kp, ntk = kp / 32;
   const int64_t nel = (int64_t)kp * kp;
-  const int w = threadIdx.x >> 6;
+  const int w = wave_id();
   const int ti = tile / ntk, tj = tile % ntk;
   const int64_t i0 = (int64_t)ti * 32, j0 = (int64_t)tj * 32;
   const NsBufs nb = ns_bufs(P.B, P.cur, it, P.ns_iters);
@@ -929,7 +994,7 @@ __device__ void update_tile(const StepCtx& P, int it, int tile, StgdSmem& sm) {
 #pragma unroll
     for (int e = 0; e < 4; ++e) v[e] = c1 * x[e] + c3 * red_sum4(red, r, c4 + e);
   finish_tile<SPLIT>(sm.p2.S, kp, ti, i0, j0, v, nb.dst, sp<SPLIT>(nb.dst16), nb.dstT, sp<SPLIT>(nb.dstT16), pre,
-                     last ? P.frob : nullptr, last ? P.B.mpart : nullptr);
+                     last ? P.B.fpart + tile : nullptr, last ? P.B.mpart : nullptr);
 }
 
 // COPY: Phi_next = Phi~ with the outputs of a last update; QRFIN: Phi_next = the QR factor the host
@@ -959,7 +1024,23 @@ __device__ void copy_tile(const StepCtx& P, bool setup, int tile, StgdSmem& sm, 
                        pre, nullptr, P.B.mpart);
   else
     finish_tile<SPLIT>(sm.S, kp, ti, i0, j0, v, P.B.phi[P.cur ^ 1], sp<SPLIT>(P.B.phi16[P.cur ^ 1]), P.B.phiT,
-                       sp<SPLIT>(P.B.phiT16), pre, (qrfin || P.a.frob_pre) ? nullptr : P.frob, P.B.mpart);
+                       sp<SPLIT>(P.B.phiT16), pre, (qrfin || P.a.frob_pre) ? nullptr : P.B.fpart + tile, P.B.mpart);
+}
+
+// FROB: the log entry of a run's last step from its tiles' partials (one wave).
+__device__ inline void frob_final(const StepCtx& P) {
+  if (threadIdx.x >= 64) return;
+  const int ntk = P.a.kp / 32;
+  const double f = frob_sum(P.B.fpart, ntk * ntk);
+  if (threadIdx.x == 0) *P.frob = f;
+}
+
+// P1's tile 0 also forms the previous step's log entry (wave 0, after its tile work).
+__device__ inline void p1_log(const StepCtx& P) {
+  if (!P.frob_prev || threadIdx.x >= 64) return;
+  const int ntk = P.a.kp / 32;
+  const double f = frob_sum(P.B.fpart, ntk * ntk);
+  if (threadIdx.x == 0) *P.frob_prev = f;
 }
 
 template <int DMODE, int KMODE, typename TY, typename TPH>
@@ -967,7 +1048,11 @@ __device__ void run_tile(const StepCtx& P, int phase, int it, int tile, StgdSmem
   constexpr bool SPLIT = KMODE == MM_BF16X3;
   switch (phase) {
     case PH_SETUP: copy_tile<SPLIT>(P, true, tile, sm); break;
-    case PH_P1: p1_tile<DMODE, KMODE, TY, TPH>(P, tile, sm); break;
+    case PH_P1:
+      p1_tile<DMODE, KMODE, TY, TPH>(P, tile, sm);
+      if (tile == 0) p1_log(P);
+      break;
+    case PH_FROB: frob_final(P); break;
     case PH_P2: p2_tile<DMODE, KMODE, TY, TPH>(P, tile, sm); break;
     case PH_GRAM: gram_tile<KMODE>(P, it, tile, sm); break;
     case PH_UPDATE: update_tile<KMODE>(P, it, tile, sm); break;
@@ -988,10 +1073,12 @@ __device__ inline bool stop_test(const StgdArgs& a, int64_t t, double f) {
 // fetches of P1 from 24 to 9 MiB per launch, but the step took 2.6 % longer: the waves of a group
 // then request the same lines at the same moment.)
 // With the end test on (P.a.stop), every phase of step g = base + st first checks whether the loop
-// has ended; P1 of each step but a run's first applies the test to the previous step's log entry
-// (complete, since its last kernel has finished) and marks the end -- every workgroup reaches the same
-// decision from the same value, so the whole launch returns together. The skipped kernels write
-// nothing, and the host reads stop_at after the run.
+// has ended; P1 of each step but a run's first forms the previous step's log entry in every wave
+// (its kernels have finished), workgroup 0 stores it, and every workgroup applies the test to it and
+// marks the end -- each reaches the same decision from the same bits, so the whole launch returns
+// together. P1 returns early only when an earlier step ended the loop (stop_at < g): stop_at == g
+// was set by a sibling workgroup, and workgroup 0 must still store the entry. The skipped kernels
+// write nothing, and the host reads stop_at after the run.
 template <int DMODE, int KMODE, typename TY, typename TPH, int PHASE>
 __global__ __launch_bounds__(kThreads) void stgd_phase_kernel(StepCtx P, int32_t it) {
   __shared__ StgdSmem sm;
@@ -999,18 +1086,27 @@ __global__ __launch_bounds__(kThreads) void stgd_phase_kernel(StepCtx P, int32_t
   if constexpr (PHASE != PH_SETUP) {
     if (P.a.stop) {
       const int64_t g = P.a.stop->base + P.st;
-      if (P.a.stop->stop_at <= g) return;  // an earlier step ended the loop (checked first: the log of
-                                           // a skipped step is never written)
+      const int64_t sa = P.a.stop->stop_at;
+      if (sa < g || (PHASE != PH_P1 && sa == g)) return;  // an earlier step ended the loop (checked
+                                                           // first: the log of a skipped step is never written)
       if constexpr (PHASE == PH_P1) {
-        if (P.frob_prev && stop_test(P.a, g - 1, *P.frob_prev)) {
-          if (threadIdx.x == 0) P.a.stop->stop_at = g;
-          return;
+        if (P.frob_prev) {
+          const int ntk = P.a.kp / 32;
+          const double f = frob_sum(P.B.fpart, ntk * ntk);
+          if (blockIdx.x == 0 && threadIdx.x == 0) *P.frob_prev = f;
+          if (stop_test(P.a, g - 1, f)) {
+            if (threadIdx.x == 0) P.a.stop->stop_at = g;
+            return;
+          }
         }
       }
     }
   }
   if constexpr (PHASE == PH_SETUP) copy_tile<SPLIT>(P, true, blockIdx.x, sm);
-  else if constexpr (PHASE == PH_P1) p1_tile<DMODE, KMODE, TY, TPH>(P, blockIdx.x, sm);
+  else if constexpr (PHASE == PH_P1) {
+    p1_tile<DMODE, KMODE, TY, TPH>(P, blockIdx.x, sm);
+    if (!P.a.stop && blockIdx.x == 0) p1_log(P);
+  } else if constexpr (PHASE == PH_FROB) frob_final(P);
   else if constexpr (PHASE == PH_P2) p2_tile<DMODE, KMODE, TY, TPH>(P, blockIdx.x, sm);
   else if constexpr (PHASE == PH_GRAM) gram_tile<KMODE>(P, it, blockIdx.x, sm);
   else if constexpr (PHASE == PH_UPDATE) update_tile<KMODE>(P, it, blockIdx.x, sm);
@@ -1063,7 +1159,7 @@ __device__ inline void make_step(const BatchInst& I, int64_t st, StepCtx& P) {
   P.yrow0 = P.row0;
   P.nb = (int32_t)min((int64_t)I.a.bl, I.a.n_x - P.row0);
   P.st = st;
-  P.frob_prev = nullptr;
+  P.frob_prev = st > 0 ? I.frob_log + st - 1 : nullptr;
   P.a.phi = I.B.phi[P.cur];
   P.a.phit = I.B.phit;
   P.frob = I.frob_log + st;
@@ -1182,6 +1278,7 @@ static int run_steps(hipStream_t s, const StgdArgs& args, const void* YA, const 
       case PH_UPDATE: hipLaunchKernelGGL((stgd_phase_kernel<DMODE, KMODE, TY, TPH, PH_UPDATE>), grid, block, 0, s, P, it); break;
       case PH_QPOLY: hipLaunchKernelGGL((stgd_phase_kernel<DMODE, KMODE, TY, TPH, PH_QPOLY>), grid, block, 0, s, P, it); break;
       case PH_QRFIN: hipLaunchKernelGGL((stgd_phase_kernel<DMODE, KMODE, TY, TPH, PH_QRFIN>), grid, block, 0, s, P, it); break;
+      case PH_FROB: hipLaunchKernelGGL((stgd_phase_kernel<DMODE, KMODE, TY, TPH, PH_FROB>), grid, block, 0, s, P, it); break;
       default: hipLaunchKernelGGL((stgd_phase_kernel<DMODE, KMODE, TY, TPH, PH_COPY>), grid, block, 0, s, P, it); break;
     }
     MMSB_LAUNCH_CHECK();
@@ -1232,6 +1329,9 @@ static int run_steps(hipStream_t s, const StgdArgs& args, const void* YA, const 
     }
     cur ^= 1;
   }
+  P.st = nsteps - 1;  // the last step's log entry
+  P.frob = frob_log + nsteps - 1;
+  MMSB_TRY(launch(PH_FROB, 0));
   return MMSB_OK;
 }
 
@@ -1346,6 +1446,7 @@ int stgd_batch_plan(StgdBatch*& out, int32_t precision, const std::vector<StgdBa
         add(PH_COPY, 0, st);
       }
     }
+    add(PH_FROB, 0, m.nsteps - 1);
   }
   std::vector<BatchSeg> segs;
   std::vector<int32_t> slot_seg;
