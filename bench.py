@@ -85,6 +85,8 @@ def main():
     ap.add_argument("--ns-iters", type=int, default=None,
                     help="Newton-Schulz iterations per step (default: the workload's, else the library's)")
     ap.add_argument("--no-recovery", action="store_true", help="skip the recovery run on the recoverable graph")
+    ap.add_argument("--no-variants", action="store_true",
+                    help="skip the throughput of the other renormalisation and of larger minibatches")
     ap.add_argument("--traffic-probe", action="store_true",
                     help="after whitening, only direct (ungraphed) STGD steps and a streaming copy of known size, "
                          "for rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE passes; prints no JSON line")
@@ -263,7 +265,49 @@ def main():
     if cpu_e2e is not None:
         out["cpu_e2e"] = cpu_e2e
     p.close()
+    if not args.no_variants:
+        out["stgd_variants"] = stgd_variants(m, g, w, n, k, b, ns, args)
     print(json.dumps(out), flush=True)
+
+
+def stgd_rate(m, g, w, n, k, b, ns, precs, epochs):
+    """STGD epochs per second of a fresh pipeline on graph g (whitened anew), per precision."""
+    cfg = m.config(nx=n, na=n, nb=n, nc=n, ncom=k, alpha_0=w["alpha0"], learn_rate=w["eta"], minibatch=b,
+                   epochs=1, polish_epochs=0, oversample=10, power_iters=1, seed=1, **ns)
+    p = m.Pipeline(cfg)
+    p.set_graph(g)
+    p.whiten()
+    p.stgd(0, "fp32")
+    spe = math.ceil(n / b)
+    out = {}
+    for prec in precs:
+        p.stgd(spe, prec)                                  # capture and warm
+        m.check(m.lib.mmsb_sync(p.ctx))
+        ts = time.time()
+        for _ in range(epochs):
+            p.stgd(spe, prec)
+        m.check(m.lib.mmsb_sync(p.ctx))
+        dt = (time.time() - ts) / epochs
+        ph = p.stgd_timed(spe, prec)
+        out[prec] = {"nodes_per_s": n / dt, "ms_per_epoch": 1e3 * dt, "us_per_step": 1e6 * dt / spe,
+                     "phases_us_per_step": {nm: round(1e3 * t / spe, 2) for nm, t in zip(("p1", "p2", "renorm"), ph)}}
+    p.close()
+    return out
+
+
+def stgd_variants(m, g, w, n, k, b, ns, args):
+    """Throughput of the same workload under other settings, each on its own pipeline (not the headline):
+    one degree-5 Newton-Schulz iteration per step instead of the default renormalisation (one kernel
+    fewer per step; the oracle follows either), and larger minibatches (fewer, larger steps per epoch)."""
+    epochs = max(1, args.steps)
+    res = {"renorm_1x_degree5": stgd_rate(m, g, w, n, k, b, {"ns_degree": 5, "ns_iters": 1}, ("bf16", "fp32"), epochs)}
+    log(f"variant degree-5 renormalisation: {res['renorm_1x_degree5']}")
+    for bb in (4096, 16384):
+        res[f"minibatch_{bb}"] = stgd_rate(m, g, w, n, k, bb, ns, ("bf16", "fp32"), epochs)
+        log(f"variant minibatch {bb}: {res[f'minibatch_{bb}']}")
+    for key, val in res.items():
+        val["bf16_speedup_vs_fp32"] = val["bf16"]["nodes_per_s"] / val["fp32"]["nodes_per_s"]
+    return res
 
 
 def traffic_probe(m, p, prec, xa, r, steps=16):

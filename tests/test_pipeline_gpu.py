@@ -134,17 +134,20 @@ def test_stgd_guarded_renorm_matches_oracle(mmsb, degree, iters):
     assert rel_err(p.get("phi"), phi_ref) < 1e-4
     np.testing.assert_allclose(p.get("frob"), log_ref, rtol=2e-3, atol=1e-12)
     # the guard's row sums are order-independent (fixed-point atomics): repeated runs give Phi bit for
-    # bit, in both precisions, though the tiles' atomics land in a different order each time
+    # bit, in both precisions, though the tiles' atomics land in a different order each time; the log
+    # entries (sums of per-tile partials in a fixed order) repeat bit for bit as well
     for prec in ("fp32", "bf16"):
-        outs = []
+        outs, logs = [], []
         for _ in range(3):
             q = mmsb.Pipeline(cfg)
             q.set_yvectors(*ys, *mus)
             q.set_phi(phi0)
             q.stgd(12, prec)
             outs.append(q.get("phi"))
+            logs.append(q.get("frob"))
             q.close()
         assert all(np.array_equal(outs[0], o) for o in outs[1:]), prec
+        assert all(np.array_equal(logs[0], f) for f in logs[1:]), prec
     p.close()
 
 
