@@ -85,6 +85,8 @@ def main():
     ap.add_argument("--ns-iters", type=int, default=None,
                     help="Newton-Schulz iterations per step (default: the workload's, else the library's)")
     ap.add_argument("--no-recovery", action="store_true", help="skip the recovery run on the recoverable graph")
+    ap.add_argument("--csr-host", action="store_true",
+                    help="keep the CSR blocks in host memory and stream them for every product (csr_host)")
     ap.add_argument("--no-variants", action="store_true",
                     help="skip the throughput of the other renormalisation and of larger minibatches")
     ap.add_argument("--traffic-probe", action="store_true",
@@ -132,7 +134,7 @@ def main():
     ns = {**{key: w[key] for key in ("ns_degree", "ns_iters") if key in w}, **ns}
     cfg = m.config(nx=n, na=n, nb=n, nc=n, ncom=k, alpha_0=w["alpha0"], learn_rate=w["eta"], minibatch=b,
                    epochs=2 if args.precision == "bf16" else 1, polish_epochs=1 if args.precision == "bf16" else 0,
-                   oversample=10, power_iters=1, seed=1, precision=args.precision, **ns)
+                   oversample=10, power_iters=1, seed=1, precision=args.precision, csr_host=int(args.csr_host), **ns)
     ns = {"ns_degree": cfg.ns_degree, "ns_iters": cfg.ns_iters}
     # ---- end-to-end: CSR on host -> whitening -> STGD schedule -> eigen -> Pi on host ----------
     t0 = time.time()
@@ -241,7 +243,8 @@ def main():
                                f"{ns['ns_iters']} Newton-Schulz iteration(s) of degree {ns['ns_degree']}, "
                                f"one step = one STGD epoch over X",
                    "n_per_partition": n, "k": k, "minibatch": b, "learn_rate": w["eta"], "nnz": nnz, **ns,
-                   "m2_scaling": "paper", "end_to_end_schedule": f"{cfg.epochs - cfg.polish_epochs} {args.precision} "
+                   "m2_scaling": "paper", "csr_blocks": "host memory, streamed" if args.csr_host else "HBM",
+                   "end_to_end_schedule": f"{cfg.epochs - cfg.polish_epochs} {args.precision} "
                    f"epoch(s) + {cfg.polish_epochs} FP32 polishing epoch(s)", "parallelism": "none"},
         "end_to_end_s": t_e2e,
         "stages_s": {"upload_and_transpose": t_up, "whiten": t_wh, "stgd_schedule": t_st, "eigen_post": t_post,

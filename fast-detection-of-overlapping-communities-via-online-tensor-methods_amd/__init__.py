@@ -79,7 +79,8 @@ class StpmConfigC(ctypes.Structure):
                 ("m2_scaling", ctypes.c_int32), ("member_tau", ctypes.c_double), ("seed", ctypes.c_uint64),
                 ("ba_first_is_row", ctypes.c_int32), ("feof_repeat", ctypes.c_int32), ("device", ctypes.c_int32),
                 ("verbose", ctypes.c_int32), ("use_graphs", ctypes.c_int32),
-                ("y_mode", ctypes.c_int32), ("renorm_mode", ctypes.c_int32)]
+                ("y_mode", ctypes.c_int32), ("renorm_mode", ctypes.c_int32),
+                ("csr_host", ctypes.c_int32)]
 
 
 PREC = {"fp32": 0, "bf16": 1}
@@ -116,6 +117,7 @@ SIGNATURES: dict = {
     "mmsb_write_values": (ctypes.c_int, [ctypes.c_char_p, _vp, ctypes.c_int64]),
     "mmsb_write_pi_triplets": (ctypes.c_int, [ctypes.c_char_p, _vp, ctypes.c_int64, ctypes.c_int32]),
     "stpm_config_default": (None, [ctypes.POINTER(StpmConfigC)]),
+    "stpm_config_size": (ctypes.c_size_t, []),
     "mmsb_create": (ctypes.c_int, [ctypes.POINTER(StpmConfigC), ctypes.POINTER(_vp)]),
     "mmsb_destroy": (None, [_vp]),
     "mmsb_set_block": (ctypes.c_int, [_vp, ctypes.c_int32, ctypes.POINTER(CsrC)]),
@@ -169,6 +171,9 @@ def _load():
         fn = getattr(lib, name)
         fn.restype = res
         fn.argtypes = args
+    if lib.stpm_config_size() != ctypes.sizeof(StpmConfigC):
+        raise ImportError(f"stpm_config layout mismatch: the library has {lib.stpm_config_size()} bytes, the binding "
+                          f"{ctypes.sizeof(StpmConfigC)}; rebuild the library")
     return lib
 
 

@@ -19,7 +19,7 @@ static void usage() {
   std::printf("keys: NX NA NB NC NCOM ALPHA_0 LEARN_RATE THRESH PINV_TOL SGD_ITER_MIN SGD_ITER_MAX BINARY\n"
               "      SYNTHETIC FOLDER_NAME FILE_PREFIX MINIBATCH EPOCHS THETA RENORM_EVERY NS_ITERS NS_DEGREE\n"
               "      PRECISION POLISH_EPOCHS OVERSAMPLE POWER_ITERS M2_SCALING MEMBER_TAU SEED BA_FIRST_IS_ROW\n"
-              "      FEOF_REPEAT VERBOSE USE_GRAPHS Y_MODE RENORM (ns | qr)\n");
+              "      FEOF_REPEAT VERBOSE USE_GRAPHS Y_MODE RENORM (ns | qr) CSR_HOST\n");
 }
 
 static int set_field(stpm_config& c, const std::string& key, const std::string& val) {
@@ -61,6 +61,7 @@ static int set_field(stpm_config& c, const std::string& key, const std::string& 
   else if (key == "VERBOSE") I(c.verbose);
   else if (key == "USE_GRAPHS") I(c.use_graphs);
   else if (key == "Y_MODE") I(c.y_mode);
+  else if (key == "CSR_HOST") I(c.csr_host);
   else if (key == "RENORM") c.renorm_mode = (val == "qr" || val == "QR" || val == "1") ? MMSB_RENORM_QR : MMSB_RENORM_NS;
   else return -1;
   return 0;

@@ -23,6 +23,7 @@
 #include <algorithm>
 #include <atomic>
 #include <chrono>
+#include <new>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -54,6 +55,8 @@ namespace mmsb {
 
 static constexpr double kCholShift = 1e-14;
 
+// A CSR block: resident in device memory, or (csr_host) kept in host memory and streamed to the
+// device in row chunks by every product.
 struct DevCsr {
   int32_t n_rows = 0, n_cols = 0;
   int64_t nnz = 0;
@@ -61,6 +64,9 @@ struct DevCsr {
   int32_t* colidx = nullptr;
   float* vals = nullptr;
   int32_t* order = nullptr;  // rows by descending length: the SpMM's schedule
+  bool host = false;         // the arrays below hold the block; the device pointers stay null
+  std::vector<int32_t> h_rowptr, h_colidx;
+  std::vector<float> h_vals;
 };
 
 static int64_t r64(int64_t a, int64_t b) { return (a + b - 1) / b * b; }
@@ -195,6 +201,17 @@ static int upload_csr(mmsb_ctx* c, DevCsr& d, int32_t nr, int32_t nc, int64_t nn
   d.n_rows = nr;
   d.n_cols = nc;
   d.nnz = nnz;
+  if (c->cfg.csr_host) {  // kept in host memory (a copy owned by the context)
+    try {
+      d.h_rowptr.assign(rp, rp + (size_t)nr + 1);
+      d.h_colidx.assign(ci, ci + (size_t)nnz);
+      if (v) d.h_vals.assign(v, v + (size_t)nnz);
+    } catch (const std::bad_alloc&) {
+      return set_error(MMSB_ERR_NOMEM, "set_block: out of host memory for a %lld-nonzero block", (long long)nnz);
+    }
+    d.host = true;
+    return MMSB_OK;
+  }
   TRY(dalloc(c, d.rowptr, (size_t)nr + 1));
   TRY(dalloc(c, d.colidx, (size_t)std::max<int64_t>(nnz, 1)));
   TRY(h2d(c->s, d.rowptr, rp, sizeof(int32_t) * ((size_t)nr + 1)));
@@ -207,8 +224,36 @@ static int upload_csr(mmsb_ctx* c, DevCsr& d, int32_t nr, int32_t nc, int64_t nn
   return csr_row_order(c->s, nr, d.rowptr, d.order);
 }
 
+// G^T of a host-resident block: a counting sort over the columns that visits the rows in order, so
+// each row of G^T lists its entries by ascending row of G -- the order of the device's stable sort.
+static int transpose_csr_host(const DevCsr& g, DevCsr& t) {
+  t.n_rows = g.n_cols;
+  t.n_cols = g.n_rows;
+  t.nnz = g.nnz;
+  t.host = true;
+  const bool w = !g.h_vals.empty();
+  try {
+    t.h_rowptr.assign((size_t)t.n_rows + 1, 0);
+    t.h_colidx.resize((size_t)g.nnz);
+    if (w) t.h_vals.resize((size_t)g.nnz);
+  } catch (const std::bad_alloc&) {
+    return set_error(MMSB_ERR_NOMEM, "set_block: out of host memory for a transpose");
+  }
+  for (int64_t e = 0; e < g.nnz; ++e) ++t.h_rowptr[(size_t)g.h_colidx[(size_t)e] + 1];
+  for (int32_t j = 0; j < t.n_rows; ++j) t.h_rowptr[(size_t)j + 1] += t.h_rowptr[(size_t)j];
+  std::vector<int32_t> next(t.h_rowptr.begin(), t.h_rowptr.end() - 1);
+  for (int32_t i = 0; i < g.n_rows; ++i)
+    for (int32_t e = g.h_rowptr[(size_t)i]; e < g.h_rowptr[(size_t)i + 1]; ++e) {
+      const int32_t pos = next[(size_t)g.h_colidx[(size_t)e]]++;
+      t.h_colidx[(size_t)pos] = i;
+      if (w) t.h_vals[(size_t)pos] = g.h_vals[(size_t)e];
+    }
+  return MMSB_OK;
+}
+
 // G^T built on the device from the uploaded G.
 static int transpose_csr(mmsb_ctx* c, const DevCsr& g, DevCsr& t) {
+  if (g.host) return transpose_csr_host(g, t);
   t.n_rows = g.n_cols;
   t.n_cols = g.n_rows;
   t.nnz = g.nnz;
@@ -220,8 +265,43 @@ static int transpose_csr(mmsb_ctx* c, const DevCsr& g, DevCsr& t) {
   return csr_row_order(c->s, t.n_rows, t.rowptr, t.order);
 }
 
+// Nonzeros per streamed chunk of a host-resident block (16 MiB of column indices).
+static constexpr int64_t kHostChunkNnz = (int64_t)1 << 22;
+
 static int spmm(mmsb_ctx* c, const DevCsr& g, float alpha, const float* B, int32_t ldb, int32_t r, float* Y, int32_t ldy) {
+  if (g.host)
+    return spmm_f32_host_csr(c->s, g.n_rows, g.n_cols, g.nnz, g.h_rowptr.data(), g.h_colidx.data(),
+                             g.h_vals.empty() ? nullptr : g.h_vals.data(), alpha, B, ldb, r, Y, ldy, kHostChunkNnz);
   return spmm_f32(c->s, g.n_rows, g.n_cols, g.nnz, g.rowptr, g.colidx, g.vals, alpha, B, ldb, r, Y, ldy, g.order);
+}
+
+// Rows [row0, row0 + nrows) of a block times B into Y (Y's first row is row0's).
+static int spmm_rows(mmsb_ctx* c, hipStream_t s, const DevCsr& g, int64_t row0, int32_t nrows, const float* B,
+                     int32_t ldb, int32_t r, float* Y, int32_t ldy) {
+  if (!g.host) return spmm_f32(s, nrows, g.n_cols, g.nnz, g.rowptr + row0, g.colidx, g.vals, 1.f, B, ldb, r, Y, ldy);
+  const int32_t z0 = g.h_rowptr[(size_t)row0];
+  std::vector<int32_t> rp((size_t)nrows + 1);
+  for (int32_t j = 0; j <= nrows; ++j) rp[(size_t)j] = g.h_rowptr[(size_t)row0 + j] - z0;
+  return spmm_f32_host_csr(s, nrows, g.n_cols, rp[(size_t)nrows], rp.data(), g.h_colidx.data() + z0,
+                           g.h_vals.empty() ? nullptr : g.h_vals.data() + z0, 1.f, B, ldb, r, Y, ldy, kHostChunkNnz);
+}
+
+// (1 / n_X) x the row sums of a block's transpose (the column means), into the device vector out: on the
+// device for a resident block, on the host -- in the same order and FP64 -- for a host-resident one.
+static int col_means(mmsb_ctx* c, const DevCsr& t, double scale, double* out) {
+  if (!t.host) return csr_rowsum(c->s, t.n_rows, t.rowptr, t.vals, scale, out);
+  std::vector<double> m((size_t)t.n_rows);
+  for (int32_t i = 0; i < t.n_rows; ++i) {
+    double acc;
+    if (!t.h_vals.empty()) {
+      acc = 0.0;
+      for (int32_t e = t.h_rowptr[(size_t)i]; e < t.h_rowptr[(size_t)i + 1]; ++e) acc += (double)t.h_vals[(size_t)e];
+    } else {
+      acc = (double)(t.h_rowptr[(size_t)i + 1] - t.h_rowptr[(size_t)i]);
+    }
+    m[(size_t)i] = scale * acc;
+  }
+  return h2d(c->s, out, m.data(), sizeof(double) * m.size());
 }
 
 // ---- CholeskyQR2: Q = X R^{-1} (Q may alias X); optionally R = R2 R1 -------------------------
@@ -365,12 +445,19 @@ static int choose_y_window(mmsb_ctx* c, bool& window) {
   const int32_t mode = c->cfg.y_mode;
   if (mode != MMSB_Y_AUTO) {
     window = mode == MMSB_Y_RECOMPUTE;
+    if (window && c->cfg.csr_host)
+      return set_error(MMSB_ERR_ARG, "whiten: recomputed minibatches read the CSR blocks on the device; with csr_host "
+                                     "the neighbour vectors must be stored (y_mode auto or stored)");
     return MMSB_OK;
   }
   size_t free_b = 0, total_b = 0;
   HCHK(hipMemGetInfo(&free_b, &total_b));
   const double need = resident_bytes(c);
   window = need > 0.9 * (double)free_b;
+  if (window && c->cfg.csr_host)
+    return set_error(MMSB_ERR_NOMEM, "whiten: the stored neighbour vectors need %.1f GB of %.1f GB free, and with "
+                                     "csr_host the minibatches cannot be recomputed from CSR", need * 1e-9,
+                     (double)free_b * 1e-9);
   c->log("whiten: resident form needs %.1f GB of %.1f GB free: %s\n", need * 1e-9, (double)free_b * 1e-9,
          window ? "minibatches recomputed from CSR" : "neighbour vectors kept");
   return MMSB_OK;
@@ -425,9 +512,9 @@ static int whiten_impl(mmsb_ctx* c) {
     TRY(alloc_y(c));
   }
   // means (pre_proc.c:154-162)
-  TRY(csr_rowsum(s, XAt.n_rows, XAt.rowptr, XAt.vals, 1.0 / (double)nx, c->mu[0]));
-  TRY(csr_rowsum(s, XBt.n_rows, XBt.rowptr, XBt.vals, 1.0 / (double)nx, c->mu[1]));
-  TRY(csr_rowsum(s, XCt.n_rows, XCt.rowptr, XCt.vals, 1.0 / (double)nx, c->mu[2]));
+  TRY(col_means(c, XAt, 1.0 / (double)nx, c->mu[0]));
+  TRY(col_means(c, XBt, 1.0 / (double)nx, c->mu[1]));
+  TRY(col_means(c, XCt, 1.0 / (double)nx, c->mu[2]));
   c->log("whiten: randomized SVD of P_BC (n_B=%lld n_C=%lld r=%d q=%d)\n", (long long)nb, (long long)ncn, r, cf.power_iters);
   HCHK(hipStreamSynchronize(s));
   double t_mark = now_s();
@@ -1258,6 +1345,8 @@ static int set_sizes(mmsb_ctx* c) {
 }  // namespace mmsb
 
 // ===================================== C-ABI =========================================================
+extern "C" size_t stpm_config_size(void) { return sizeof(stpm_config); }
+
 extern "C" void stpm_config_default(stpm_config* c) {
   std::memset(c, 0, sizeof(*c));
   c->nx = 29079;  // stpm.h:72-76
@@ -1381,7 +1470,7 @@ extern "C" int mmsb_set_block(mmsb_ctx* c, int32_t which, const mmsb_csr* b) {
       MMSB_CHECK_ARG(e < 0, "set_block: column index %d out of range", e < 0 ? 0 : b->colidx[e]);
   }
   HCHK(hipSetDevice(c->cfg.device));
-  if (c->blk[which].rowptr) return set_error(MMSB_ERR_STATE, "set_block: block %d already set", which);
+  if (c->have[which]) return set_error(MMSB_ERR_STATE, "set_block: block %d already set", which);
   TRY(upload_csr(c, c->blk[which], b->n_rows, b->n_cols, b->nnz, b->rowptr, b->colidx, b->vals));
   TRY(transpose_csr(c, c->blk[which], c->blkT[which]));
   c->have[which] = true;
@@ -1588,9 +1677,7 @@ extern "C" int mmsb_grad(mmsb_ctx* c, int64_t row0, int32_t nrows, const double*
     const DevCsr* g[3] = {&c->blk[0], &c->blk[1], &c->blk[2]};
     const float* rhs[3] = {c->W, c->MB, c->MC};
     float* out[3] = {ya, yb, yc};
-    for (int i = 0; i < 3 && st == MMSB_OK; ++i)
-      st = spmm_f32(s, nrows, g[i]->n_cols, g[i]->nnz, g[i]->rowptr + row0, g[i]->colidx, g[i]->vals, 1.f, rhs[i], kp, k,
-                    out[i], kp);
+    for (int i = 0; i < 3 && st == MMSB_OK; ++i) st = spmm_rows(c, s, *g[i], row0, nrows, rhs[i], kp, k, out[i], kp);
   } else {
     e = hipMemcpy2DAsync(ya, sizeof(float) * kp, c->YA + row0 * kp, sizeof(float) * kp, sizeof(float) * kp, nrows,
                          hipMemcpyDeviceToDevice, s);

@@ -436,8 +436,9 @@ struct TileShm {
 };
 
 // Writes block (i0, j0) of a new kp x kp iterate held as v[e] = element (i0 + r, j0 + c4 + e), with
-// r = thread / 8 and c4 = 4 (thread % 8): Out row-major, OutT transposed (FP32 and, with SPLIT, BF16
-// pairs), *fslot = ||Out - prev||_F^2 over the block when fslot is given (prev in pre.pv; the step's
+// r = thread / 8 and c4 = 4 (thread % 8): Out row-major (FP32 and, with SPLIT, BF16 pairs), OutT
+// transposed (FP32 without SPLIT, BF16 pairs with it: the transposes are only ever MFMA operands, and
+// a later FP32 run rebuilds its FP32 transpose in SETUP), *fslot = ||Out - prev||_F^2 over the block when fslot is given (prev in pre.pv; the step's
 // log entry is the sum of its tiles' slots, see frob_sum), and, when
 // mpart is given, mpart[(ti * 2 + s) * kp + j] = sum over the block's rows i of mu_s[i] Out[i][j]
 // (s = A, B; mu in pre.ua, pre.ub). Every thread of the block calls it.
@@ -488,7 +489,7 @@ __device__ inline void finish_tile(TileShm& S, int kp, int ti, int64_t i0, int64
   if (act) {
     const int c = threadIdx.x >> 3, r4 = 4 * (threadIdx.x & 7);
     const float t[4] = {S.t[c][r4], S.t[c][r4 + 1], S.t[c][r4 + 2], S.t[c][r4 + 3]};
-    put4<SPLIT>(OutT, OutT16, nel, (j0 + c) * kp + i0 + r4, t);
+    put4<SPLIT>(SPLIT ? nullptr : OutT, OutT16, nel, (j0 + c) * kp + i0 + r4, t);
   }
   if (fslot && threadIdx.x == 0) *fslot = (S.d[0] + S.d[1]) + (S.d[2] + S.d[3]);
   if (mpart && threadIdx.x < 64) {

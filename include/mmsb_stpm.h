@@ -244,6 +244,11 @@ typedef struct stpm_config {
   int32_t use_graphs;         /* replay whole STGD epochs as captured HIP graphs                 */
   int32_t y_mode;             /* MMSB_Y_*: neighbour vectors stored or recomputed per minibatch  */
   int32_t renorm_mode;        /* MMSB_RENORM_*: Newton-Schulz (default) or the reference's QR    */
+  int32_t csr_host;           /* 1: the CSR blocks (and their transposes) stay in host memory and are
+                               * streamed to the device in row chunks for every product -- for graphs
+                               * whose blocks do not fit in HBM; the neighbour vectors must then be
+                               * stored (y_mode auto or stored). 0 (default): blocks resident in HBM.
+                               * Results are identical either way.                                */
 } stpm_config;
 
 /* Fills the reference defaults of stpm.h:47-106 (DBLP sizes, NCOM 100, ALPHA_0 1, LEARN_RATE 1e-3,
@@ -251,6 +256,10 @@ typedef struct stpm_config {
  * step with 2 degree-3 Newton-Schulz iterations, FP32, oversampling 10, 1 power iteration, paper M2
  * scaling, tau 0, seed 0, CPU file orientation. */
 void stpm_config_default(stpm_config* cfg);
+/* sizeof(stpm_config) as compiled into the library, so that a binding that lays the struct out
+ * itself (ctypes, cgo, JNI) can check its layout. No counterpart in the reference (stpm.h is a set of
+ * macros there). */
+size_t stpm_config_size(void);
 
 typedef struct mmsb_ctx mmsb_ctx;
 

@@ -93,3 +93,13 @@ def test_release_and_strip_limit_without_a_gpu(mmsb):
 
 def test_renorm_mode_field_round_trips(mmsb):
     assert mmsb.config().renorm_mode == 0 and mmsb.config(renorm_mode="qr").renorm_mode == 1
+
+
+def test_csr_host_field_defaults_to_resident_blocks(mmsb):
+    """stpm_config.csr_host is the last field of the struct (the binding's layout must match the
+    header's size) and defaults to 0 (blocks resident in HBM)."""
+    import ctypes
+    cfg = mmsb.config()
+    assert cfg.csr_host == 0 and mmsb.config(csr_host=1).csr_host == 1
+    assert type(cfg)._fields_[-1][0] == "csr_host"
+    assert ctypes.sizeof(cfg) == mmsb.lib.stpm_config_size()
