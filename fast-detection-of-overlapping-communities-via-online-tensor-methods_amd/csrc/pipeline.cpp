@@ -224,30 +224,58 @@ static int upload_csr(mmsb_ctx* c, DevCsr& d, int32_t nr, int32_t nc, int64_t nn
   return csr_row_order(c->s, nr, d.rowptr, d.order);
 }
 
-// G^T of a host-resident block: a counting sort over the columns that visits the rows in order, so
-// each row of G^T lists its entries by ascending row of G -- the order of the device's stable sort.
+// G^T of a host-resident block: a counting sort over the columns in which the rows are visited in
+// order, so each row of G^T lists its entries by ascending row of G -- the order of the device's
+// stable sort. Threads take contiguous row ranges; thread th's entries of column j are placed after
+// those of threads th' < th, which keeps that order.
 static int transpose_csr_host(const DevCsr& g, DevCsr& t) {
   t.n_rows = g.n_cols;
   t.n_cols = g.n_rows;
   t.nnz = g.nnz;
   t.host = true;
   const bool w = !g.h_vals.empty();
+  const int64_t nr = g.n_rows, nc = g.n_cols;
+  const int nt = g.nnz < ((int64_t)1 << 20) ? 1 : (int)std::min<int64_t>(std::min(16, std::max(1, default_threads())), nr);
+  std::vector<std::vector<int32_t>> next((size_t)nt);
   try {
-    t.h_rowptr.assign((size_t)t.n_rows + 1, 0);
+    t.h_rowptr.assign((size_t)nc + 1, 0);
     t.h_colidx.resize((size_t)g.nnz);
     if (w) t.h_vals.resize((size_t)g.nnz);
+    for (auto& v : next) v.assign((size_t)nc, 0);
   } catch (const std::bad_alloc&) {
     return set_error(MMSB_ERR_NOMEM, "set_block: out of host memory for a transpose");
   }
-  for (int64_t e = 0; e < g.nnz; ++e) ++t.h_rowptr[(size_t)g.h_colidx[(size_t)e] + 1];
-  for (int32_t j = 0; j < t.n_rows; ++j) t.h_rowptr[(size_t)j + 1] += t.h_rowptr[(size_t)j];
-  std::vector<int32_t> next(t.h_rowptr.begin(), t.h_rowptr.end() - 1);
-  for (int32_t i = 0; i < g.n_rows; ++i)
-    for (int32_t e = g.h_rowptr[(size_t)i]; e < g.h_rowptr[(size_t)i + 1]; ++e) {
-      const int32_t pos = next[(size_t)g.h_colidx[(size_t)e]]++;
-      t.h_colidx[(size_t)pos] = i;
-      if (w) t.h_vals[(size_t)pos] = g.h_vals[(size_t)e];
+  parallel_chunks(nr, nt, [&](int th, int64_t lo, int64_t hi) {  // per-thread column counts
+    int32_t* cnt = next[(size_t)th].data();
+    for (int64_t e = g.h_rowptr[(size_t)lo]; e < g.h_rowptr[(size_t)hi]; ++e) ++cnt[g.h_colidx[(size_t)e]];
+  });
+  parallel_chunks(nc, nt, [&](int, int64_t lo, int64_t hi) {  // column totals
+    for (int64_t j = lo; j < hi; ++j) {
+      int32_t s = 0;
+      for (int th = 0; th < nt; ++th) s += next[(size_t)th][(size_t)j];
+      t.h_rowptr[(size_t)j + 1] = s;
     }
+  });
+  for (int64_t j = 0; j < nc; ++j) t.h_rowptr[(size_t)j + 1] += t.h_rowptr[(size_t)j];
+  parallel_chunks(nc, nt, [&](int, int64_t lo, int64_t hi) {  // counts -> each thread's first slot
+    for (int64_t j = lo; j < hi; ++j) {
+      int32_t pos = t.h_rowptr[(size_t)j];
+      for (int th = 0; th < nt; ++th) {
+        const int32_t c = next[(size_t)th][(size_t)j];
+        next[(size_t)th][(size_t)j] = pos;
+        pos += c;
+      }
+    }
+  });
+  parallel_chunks(nr, nt, [&](int th, int64_t lo, int64_t hi) {  // scatter, rows in order
+    int32_t* nx = next[(size_t)th].data();
+    for (int64_t i = lo; i < hi; ++i)
+      for (int32_t e = g.h_rowptr[(size_t)i]; e < g.h_rowptr[(size_t)i + 1]; ++e) {
+        const int32_t pos = nx[g.h_colidx[(size_t)e]]++;
+        t.h_colidx[(size_t)pos] = (int32_t)i;
+        if (w) t.h_vals[(size_t)pos] = g.h_vals[(size_t)e];
+      }
+  });
   return MMSB_OK;
 }
 

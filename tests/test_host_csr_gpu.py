@@ -36,13 +36,23 @@ def run(mmsb, g, host, sizes, k, weighted, steps, prec):
     return p, out
 
 
-@pytest.mark.parametrize("k,weighted,prec", [(8, False, "fp32"), (12, True, "bf16"), (270, False, "fp32")])
+CASES = {  # k: (partition sizes, P in, P out)
+    8: ((1500, 1300, 1400, 1200), 0.4, 0.01),
+    12: ((1500, 1300, 1400, 1200), 0.4, 0.01),
+    20: ((20000, 19000, 18000, 17000), 0.05, 0.002),   # ~1.7M nonzeros per block: the threaded transpose
+    270: ((2000, 1900, 1800, 1700), 0.1, 0.004),
+}
+
+
+@pytest.mark.parametrize("k,weighted,prec", [(8, False, "fp32"), (12, True, "bf16"), (20, True, "bf16"),
+                                             (270, False, "fp32")])
 def test_host_resident_blocks_give_identical_results(mmsb, k, weighted, prec):
-    """Partitions of four sizes; binary and weighted blocks; k = 270 makes r = 280 > 256, so the
-    products run in 256-column strips from both forms."""
-    sizes = (1500, 1300, 1400, 1200) if k < 100 else (2000, 1900, 1800, 1700)
+    """Partitions of four sizes; binary and weighted blocks; blocks of more than 2^20 nonzeros, whose
+    host transposes are built by several threads; k = 270 makes r = 280 > 256, so the products run
+    in 256-column strips from both forms."""
+    sizes, p_in, p_out = CASES[k]
     nx, na, nb, nc = sizes
-    g = mmsb.generate(nx, na, nb, nc, k, 1.0, 0.4 if k < 100 else 0.1, 0.01 if k < 100 else 0.004, seed=11)
+    g = mmsb.generate(nx, na, nb, nc, k, 1.0, p_in, p_out, seed=11)
     if weighted:
         rng = np.random.default_rng(11)
         for nm in ("XA", "XB", "XC", "BA", "CA"):
