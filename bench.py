@@ -42,10 +42,12 @@ WORKLOADS = {
     "c2": dict(n=100000, k=100, alpha0=1.0, dc=False, p_in=0.01, p_out=2e-4, b=1024, eta=1e-3),
     "c3": dict(n=1000000, k=500, alpha0=2.0, dc=True, mean_degree=80.0, b=1024, eta=1e-6),
 }
-# A graph of the same size class on which the method recovers the planted communities (60 expected
-# neighbours per partition inside a node's community, 20 outside): scored end to end with the
-# schedule that recovers them.
-RECOVERY = dict(n=250000, k=50, alpha0=1.0, d_in=60.0, d_out=20.0, b=1024, eta=3e-2, epochs=8, polish=1)
+# A graph of the workload's size (1M nodes per partition, k = 500) on which the method recovers the
+# planted communities: 60 expected neighbours per partition inside a node's community, 20 outside
+# (P = 0.03 / 2e-5), alpha0 = 1. With a constant step, STGD settles on a fixed point within the given
+# epochs, where FP32, BF16 and BF16 + an FP32 polishing epoch are compared with the FP64 oracle's own
+# fixed point: (minibatch, eta, epochs) per case.
+CONVERGED = dict(n=1000000, k=500, alpha0=1.0, p_in=0.03, p_out=2e-5, cases=((1024, 3e-3, 2), (16384, 1e-2, 8)))
 
 
 def log(msg):
@@ -216,9 +218,11 @@ def main():
     if not args.no_oracle and args.cpu_e2e_sample > 0 and args.cpu_e2e_sample < n:
         cpu_e2e = cpu_e2e_sample(m, w, args.cpu_e2e_sample, t_e2e, n)
         log(f"cpu e2e {cpu_e2e}")
+    converged = None
     if recovery is not None and args.workload == "c3" and not args.no_recovery:
-        recovery["recoverable_1M_nodes"] = recovery_run(m)
-        log(f"recovery at scale: {recovery['recoverable_1M_nodes']}")
+        p.close()
+        p = None
+        converged = converged_comparison(m)
 
     # ---- SpMM roofline on G_XA (n x n, nnz) times a tall n x r FP32 matrix ----------------------
     spmm = spmm_roofline(m, g["XA"], r=k + 10)
@@ -267,7 +271,10 @@ def main():
         out["cpu_baseline"] = cpu
     if cpu_e2e is not None:
         out["cpu_e2e"] = cpu_e2e
-    p.close()
+    if converged is not None:
+        out["converged"] = converged
+    if p is not None:
+        p.close()
     if not args.no_variants:
         out["stgd_variants"] = stgd_variants(m, g, w, n, k, b, ns, args)
     print(json.dumps(out), flush=True)
@@ -372,7 +379,23 @@ def measured_traffic(workload, prec):
     return None
 
 
-def accuracy_vs_oracle(m, p, phi0, n, k, b, w, spe, epochs):
+def column_match(phi, ref):
+    """The one-to-one matching of phi's columns to ref's (largest |phi^T ref| assignment) and their
+    signs: phi[:, perm] * sign is phi aligned with ref. STGD's fixed points are the tensor's
+    eigenvectors, and which column lands on which eigenvector depends on the trajectory; Pi's columns
+    and the eigenvalues follow Phi's."""
+    import numpy as np
+    from scipy.optimize import linear_sum_assignment
+    c = phi.T @ ref
+    ri, ci = linear_sum_assignment(-np.abs(c))
+    perm = np.empty(ref.shape[1], int)
+    perm[ci] = ri
+    sign = np.sign(c[perm, np.arange(ref.shape[1])])
+    sign[sign == 0] = 1.0
+    return perm, sign
+
+
+def accuracy_vs_oracle(m, p, phi0, n, k, b, w, spe, epochs, truth=None):
     """From the same Phi0 and whitened vectors: `epochs` epochs of the FP64 oracle (numpy), of FP32
     STGD, of BF16 STGD, and of BF16 STGD whose last epoch is an FP32 polish; relative Frobenius
     errors of Phi, of the raw Pi_X (linear in Phi) and of the normalised memberships Pi_X-hat
@@ -407,8 +430,23 @@ def accuracy_vs_oracle(m, p, phi0, n, k, b, w, spe, epochs):
                         "Newton-Schulz iterations per step",
            "phi_rel_err": {}, "pi_x_rel_err": {}, "pin_x_rel_err": {}, "lambda_max_abs_err": {},
            "membership_agreement": {}, "oracle_phi_moved": rel(phi64, phi0)}
+    if truth is not None:
+        acc["recovery"] = {"oracle_fp64": matched_recovery(pin64, truth)}
+        acc["column_matched"] = {"phi_rel_err": {}, "pi_x_rel_err": {}, "pin_x_rel_err": {}, "lambda_max_abs_err": {},
+                                 "membership_agreement": {}}
     for name, phi in runs.items():
         pi, pin, lam = post_of(phi)
+        if truth is not None:
+            acc["recovery"][name] = matched_recovery(pin, truth)
+            perm, sign = column_match(phi, phi64)
+            pim = pi[:, perm] * sign
+            pinm = core.norm_prob_rows(pim)
+            cm = acc["column_matched"]
+            cm["phi_rel_err"][name] = rel(phi[:, perm] * sign, phi64)
+            cm["pi_x_rel_err"][name] = rel(pim, pi64)
+            cm["pin_x_rel_err"][name] = rel(pinm, pin64)
+            cm["lambda_max_abs_err"][name] = float(np.abs(lam[perm] - lam64).max())
+            cm["membership_agreement"][name] = membership_agreement(pinm, pi64)
         acc["phi_rel_err"][name] = rel(phi, phi64)
         acc["pi_x_rel_err"][name] = rel(pi, pi64)
         acc["pin_x_rel_err"][name] = rel(pin, pin64)
@@ -463,37 +501,54 @@ def matched_recovery(pin, truth):
             "recovery_ratio_p01": float((p < 0.01).any(0).mean()), "nodes_scored": int(est.shape[0])}
 
 
-def recovery_run(m):
-    """End to end on a graph of the same size class (1M nodes) on which the method recovers the planted
-    communities, with the schedule that recovers them, scored by matched_recovery."""
+def converged_comparison(m):
+    """The planted graph of CONVERGED: per (minibatch, eta, epochs) case, from the same Phi0 and whitened
+    vectors, the oracle's FP64 epochs against FP32, BF16, and BF16 with the last epoch in FP32
+    (accuracy_vs_oracle, with column-matched Phi errors and the recovery of the planted communities of
+    each), plus the device time of each schedule: FP32 throughout against BF16 + one FP32 polishing
+    epoch."""
     import numpy as np
-    R = RECOVERY
-    n, k = R["n"], R["k"]
-    g = m.generate(n, n, n, n, k, R["alpha0"], R["d_in"] / (n / k), R["d_out"] / n, seed=0, threads=16,
-                   keep_truth=True)
+    C = CONVERGED
+    n, k = C["n"], C["k"]
+    t0 = time.time()
+    g = m.generate(n, n, n, n, k, C["alpha0"], C["p_in"], C["p_out"], seed=0, threads=16, keep_truth=True)
     x_nodes = np.nonzero(g["part"] == 0)[0]
-    truth = np.zeros((n, k))
+    truth = np.zeros((n, k), np.float32)
     truth[g["local"][x_nodes]] = g["pi"][x_nodes]
     for key in ("pi", "theta", "part", "local"):
         del g[key]
-    cfg = m.config(nx=n, na=n, nb=n, nc=n, ncom=k, alpha_0=R["alpha0"], learn_rate=R["eta"], minibatch=R["b"],
-                   epochs=R["epochs"] + R["polish"], polish_epochs=R["polish"], oversample=10, power_iters=1, seed=1,
-                   precision="bf16")
-    t0 = time.time()
-    p = m.Pipeline(cfg)
-    p.set_graph(g)
-    p.whiten()
-    p.stgd(-1, "bf16")
-    p.eigen()
-    p.post()
-    pin = p.get_f32("pin_x")
-    t = time.time() - t0
-    p.close()
-    out = matched_recovery(pin, truth)
-    out.update({"end_to_end_s": t, "edges": int(g["n_edges"]),
-                "graph": f"{n} nodes/partition, k={k}, alpha0={R['alpha0']}, {R['d_in']:g} / {R['d_out']:g} expected "
-                         f"neighbours per partition inside / outside the community",
-                "schedule": f"{R['epochs']} BF16 + {R['polish']} FP32 epochs, minibatch {R['b']}, eta {R['eta']:g}"})
+    log(f"converged: planted graph, {g['n_edges']} edges, {time.time() - t0:.1f}s")
+    out = {"graph": f"{n} nodes/partition, k={k}, alpha0={C['alpha0']}, P {C['p_in']:g} / {C['p_out']:g} (60 / 20 "
+                    f"expected neighbours per partition inside / outside the community), {g['n_edges']} edges",
+           "cases": []}
+    for b, eta, epochs in C["cases"]:
+        w = {"alpha0": C["alpha0"], "eta": eta}
+        cfg = m.config(nx=n, na=n, nb=n, nc=n, ncom=k, alpha_0=C["alpha0"], learn_rate=eta, minibatch=b, epochs=1,
+                       polish_epochs=0, oversample=10, power_iters=1, seed=1)
+        p = m.Pipeline(cfg)
+        p.set_graph(g)
+        p.whiten()
+        p.stgd(0, "fp32")
+        phi0 = p.get("phi")
+        spe = math.ceil(n / b)
+        ms = {}
+        for prec in ("fp32", "bf16"):
+            p.stgd(spe, prec)
+            m.check(m.lib.mmsb_sync(p.ctx))
+            ts = time.time()
+            for _ in range(3):
+                p.stgd(spe, prec)
+            m.check(m.lib.mmsb_sync(p.ctx))
+            ms[prec] = 1e3 * (time.time() - ts) / 3
+        acc, _ = accuracy_vs_oracle(m, p, phi0, n, k, b, w, spe, epochs, truth=truth)
+        p.close()
+        t_fp32 = epochs * ms["fp32"]
+        t_mixed = (epochs - 1) * ms["bf16"] + ms["fp32"]
+        case = {"minibatch": b, "eta": eta, "epochs": epochs, "ms_per_epoch": ms,
+                "schedule_ms": {"fp32": t_fp32, "bf16_polish": t_mixed},
+                "bf16_polish_speedup_vs_fp32": t_fp32 / t_mixed, **acc}
+        log(f"converged b={b}: column-matched {acc['column_matched']}, speedup {t_fp32 / t_mixed:.2f}")
+        out["cases"].append(case)
     return out
 
 
