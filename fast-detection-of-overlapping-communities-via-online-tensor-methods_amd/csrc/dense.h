@@ -63,10 +63,11 @@ int copy2d_f32(hipStream_t s, int32_t rows, int32_t cols, const float* in, int32
 int spmm_f32(hipStream_t s, int32_t m, int32_t n, int64_t nnz, const int32_t* rowptr, const int32_t* colidx, const float* vals,
              float alpha, const float* B, int32_t ldb, int32_t r, float* Y, int32_t ldy, const int32_t* order = nullptr);
 // The same product with the CSR block in host memory, streamed in row chunks of at most chunk_nnz
-// nonzeros (spmm_stream.cpp); returns after the product has finished.
+// nonzeros (spmm_stream.cpp); returns after the product has finished. check = false skips the scan of
+// the row pointers and column indices, for blocks the caller has already validated.
 int spmm_f32_host_csr(hipStream_t s, int32_t m, int32_t n, int64_t nnz, const int32_t* rowptr, const int32_t* colidx,
                       const float* vals, float alpha, const float* B, int32_t ldb, int32_t r, float* Y, int32_t ldy,
-                      int64_t chunk_nnz);
+                      int64_t chunk_nnz, bool check = true);
 
 // Whitened rows of one STGD minibatch: for each of the three jobs (y_A, y_B, y_C), rows [0, nb) of
 // the CSR block whose rowptr is given (offset to the minibatch's first row) times R [n_cols x ldr]

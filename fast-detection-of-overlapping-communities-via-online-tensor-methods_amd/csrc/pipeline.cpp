@@ -293,13 +293,14 @@ static int transpose_csr(mmsb_ctx* c, const DevCsr& g, DevCsr& t) {
   return csr_row_order(c->s, t.n_rows, t.rowptr, t.order);
 }
 
-// Nonzeros per streamed chunk of a host-resident block (16 MiB of column indices).
+// Nonzeros per streamed chunk of a host-resident block (16 MiB of column indices). The blocks were
+// checked by mmsb_set_block, so the streamed products skip the scan.
 static constexpr int64_t kHostChunkNnz = (int64_t)1 << 22;
 
 static int spmm(mmsb_ctx* c, const DevCsr& g, float alpha, const float* B, int32_t ldb, int32_t r, float* Y, int32_t ldy) {
   if (g.host)
     return spmm_f32_host_csr(c->s, g.n_rows, g.n_cols, g.nnz, g.h_rowptr.data(), g.h_colidx.data(),
-                             g.h_vals.empty() ? nullptr : g.h_vals.data(), alpha, B, ldb, r, Y, ldy, kHostChunkNnz);
+                             g.h_vals.empty() ? nullptr : g.h_vals.data(), alpha, B, ldb, r, Y, ldy, kHostChunkNnz, false);
   return spmm_f32(c->s, g.n_rows, g.n_cols, g.nnz, g.rowptr, g.colidx, g.vals, alpha, B, ldb, r, Y, ldy, g.order);
 }
 
@@ -311,7 +312,7 @@ static int spmm_rows(mmsb_ctx* c, hipStream_t s, const DevCsr& g, int64_t row0, 
   std::vector<int32_t> rp((size_t)nrows + 1);
   for (int32_t j = 0; j <= nrows; ++j) rp[(size_t)j] = g.h_rowptr[(size_t)row0 + j] - z0;
   return spmm_f32_host_csr(s, nrows, g.n_cols, rp[(size_t)nrows], rp.data(), g.h_colidx.data() + z0,
-                           g.h_vals.empty() ? nullptr : g.h_vals.data() + z0, 1.f, B, ldb, r, Y, ldy, kHostChunkNnz);
+                           g.h_vals.empty() ? nullptr : g.h_vals.data() + z0, 1.f, B, ldb, r, Y, ldy, kHostChunkNnz, false);
 }
 
 // (1 / n_X) x the row sums of a block's transpose (the column means), into the device vector out: on the

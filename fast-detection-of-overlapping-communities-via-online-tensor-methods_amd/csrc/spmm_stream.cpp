@@ -143,14 +143,14 @@ void par_copy(void* dst, const void* src, size_t n) {
 
 int spmm_f32_host_csr(hipStream_t s, int32_t m, int32_t n, int64_t nnz, const int32_t* rowptr, const int32_t* colidx,
                       const float* vals, float alpha, const float* B, int32_t ldb, int32_t r, float* Y, int32_t ldy,
-                      int64_t chunk_nnz) {
+                      int64_t chunk_nnz, bool check) {
   MMSB_CHECK_ARG(m >= 0 && n >= 0 && nnz >= 0 && nnz < (int64_t)INT32_MAX, "spmm_host_csr: bad sizes m=%d n=%d nnz=%lld", m,
                  n, (long long)nnz);
   MMSB_CHECK_ARG(chunk_nnz >= 1, "spmm_host_csr: chunk_nnz must be positive");
   if (m == 0) return MMSB_OK;
   MMSB_CHECK_ARG(rowptr && (nnz == 0 || colidx), "spmm_host_csr: null CSR");
   MMSB_CHECK_ARG(rowptr[0] == 0 && rowptr[m] == nnz, "spmm_host_csr: rowptr does not span [0, nnz]");
-  {  // the block is on the host: check it before any of it reaches the kernel
+  if (check) {  // the block is on the host: check it before any of it reaches the kernel
     const int nt = default_threads();
     std::vector<int64_t> bad_row(nt, -1), bad_nz(nt, -1);
     parallel_chunks(m, nt, [&](int t, int64_t lo, int64_t hi) {
