@@ -58,7 +58,7 @@ def test_adjoint_matches_finite_differences_and_oracle(mmsb, dev, k, nb, scaling
     g = phi.grad.detach().cpu().numpy()
     ref_g = core.tensor_grad(h["phi"], h["ya"], h["yb"], h["yc"], h["mua"], h["mub"], h["muc"], tuple(co), theta)
     ref_l = core.tensor_loss(h["phi"], h["ya"], h["yb"], h["yc"], h["mua"], h["mub"], h["muc"], tuple(co), theta)
-    np.testing.assert_allclose(float(loss), ref_l, rtol=1e-11)
+    np.testing.assert_allclose(float(loss.detach()), ref_l, rtol=1e-11)
     np.testing.assert_allclose(g, ref_g, rtol=1e-10, atol=1e-10 * np.abs(ref_g).max())
     # central finite differences of the GPU loss itself (FP64, h = 1e-6)
     step = 1e-6
