@@ -691,7 +691,7 @@ __device__ inline float cspart_reduce(const PartSum& ps, const float* cs_part, i
 // P1 tiles [0, ntk (ntk + 1) / 2): upper-triangle H tiles; the rest: a 32-row slice of the minibatch x
 // 32 columns of C_g.
 template <int DMODE, int KMODE, typename TY, typename TPH>
-__device__ void p1_tile(const StepCtx& P, int tile, StgdSmem& sm) {
+__device__ __forceinline__ void p1_tile(const StepCtx& P, int tile, StgdSmem& sm) {
   constexpr bool SPLIT = KMODE == MM_BF16X3;
   const StgdArgs& a = P.a;
   const int kp = a.kp;
@@ -776,7 +776,7 @@ __device__ void p1_tile(const StepCtx& P, int tile, StgdSmem& sm) {
 }
 
 template <int DMODE, int KMODE, typename TY, typename TPH>
-__device__ void p2_tile(const StepCtx& P, int tile, StgdSmem& sm) {
+__device__ __forceinline__ void p2_tile(const StepCtx& P, int tile, StgdSmem& sm) {
   constexpr bool SPLIT = KMODE == MM_BF16X3;
   constexpr int VEC = 16 / sizeof(TY);  // Y_C^T elements per 16-byte load
   const StgdArgs& a = P.a;
@@ -876,7 +876,7 @@ __device__ inline void guard_scale(const float rs[8], const unsigned long long* 
 // G = X^T X of Newton-Schulz iteration `it` on upper-triangle tile `tile` (block and mirror), and
 // rowsum[i] += sum over the block's columns of |G_ij| for every row i the block and its mirror cover.
 template <int KMODE>
-__device__ void gram_tile(const StepCtx& P, int it, int tile, StgdSmem& sm) {
+__device__ __forceinline__ void gram_tile(const StepCtx& P, int it, int tile, StgdSmem& sm) {
   constexpr bool SPLIT = KMODE == MM_BF16X3;
   const int kp = P.a.kp, ntk = kp / 32;
   const int64_t nel = (int64_t)kp * kp;
@@ -921,7 +921,7 @@ __device__ void gram_tile(const StepCtx& P, int it, int tile, StgdSmem& sm) {
 // Degree-5 iteration, second stage: Q = -(5/4) c^2 G + (3/8) c^4 G^2 (symmetric, as G), so that the
 // update is Xout = c X (15/8 I + Q).
 template <int KMODE>
-__device__ void qpoly_tile(const StepCtx& P, int it, int tile, StgdSmem& sm) {
+__device__ __forceinline__ void qpoly_tile(const StepCtx& P, int it, int tile, StgdSmem& sm) {
   constexpr bool SPLIT = KMODE == MM_BF16X3;
   const int kp = P.a.kp, ntk = kp / 32;
   const int64_t nel = (int64_t)kp * kp;
@@ -956,7 +956,7 @@ __device__ void qpoly_tile(const StepCtx& P, int it, int tile, StgdSmem& sm) {
 // Degree 3: Xout = c (1.5 X - 0.5 c^2 X G); degree 5: Xout = c (1.875 X + X Q). The outputs are those
 // of finish_tile (the last iteration also logs ||Phi_next - Phi||^2 and the partial sums of m_A, m_B).
 template <int KMODE>
-__device__ void update_tile(const StepCtx& P, int it, int tile, StgdSmem& sm) {
+__device__ __forceinline__ void update_tile(const StepCtx& P, int it, int tile, StgdSmem& sm) {
   constexpr bool SPLIT = KMODE == MM_BF16X3;
   const int kp = P.a.kp, ntk = kp / 32;
   const int64_t nel = (int64_t)kp * kp;
@@ -1003,7 +1003,7 @@ __device__ void update_tile(const StepCtx& P, int it, int tile, StgdSmem& sm) {
 // before its QR). SETUP: the derived copies of the current Phi (split pairs, Phi^T, the m partial
 // sums), and tile 0 clears the run's frob entries.
 template <bool SPLIT>
-__device__ void copy_tile(const StepCtx& P, bool setup, int tile, StgdSmem& sm, bool qrfin = false) {
+__device__ __forceinline__ void copy_tile(const StepCtx& P, bool setup, int tile, StgdSmem& sm, bool qrfin = false) {
   const int kp = P.a.kp, ntk = kp / 32;
   const int ti = tile / ntk, tj = tile % ntk;
   const int64_t i0 = (int64_t)ti * 32, j0 = (int64_t)tj * 32;
@@ -1029,7 +1029,7 @@ __device__ void copy_tile(const StepCtx& P, bool setup, int tile, StgdSmem& sm, 
 }
 
 // FROB: the log entry of a run's last step from its tiles' partials (one wave).
-__device__ inline void frob_final(const StepCtx& P) {
+__device__ __forceinline__ void frob_final(const StepCtx& P) {
   if (threadIdx.x >= 64) return;
   const int ntk = P.a.kp / 32;
   const double f = frob_sum(P.B.fpart, ntk * ntk);
@@ -1037,7 +1037,7 @@ __device__ inline void frob_final(const StepCtx& P) {
 }
 
 // P1's tile 0 also forms the previous step's log entry (wave 0, after its tile work).
-__device__ inline void p1_log(const StepCtx& P) {
+__device__ __forceinline__ void p1_log(const StepCtx& P) {
   if (!P.frob_prev || threadIdx.x >= 64) return;
   const int ntk = P.a.kp / 32;
   const double f = frob_sum(P.B.fpart, ntk * ntk);
@@ -1045,7 +1045,7 @@ __device__ inline void p1_log(const StepCtx& P) {
 }
 
 template <int DMODE, int KMODE, typename TY, typename TPH>
-__device__ void run_tile(const StepCtx& P, int phase, int it, int tile, StgdSmem& sm) {
+__device__ __forceinline__ void run_tile(const StepCtx& P, int phase, int it, int tile, StgdSmem& sm) {
   constexpr bool SPLIT = KMODE == MM_BF16X3;
   switch (phase) {
     case PH_SETUP: copy_tile<SPLIT>(P, true, tile, sm); break;
@@ -1129,6 +1129,7 @@ struct BatchSeg {
   int32_t ntiles;
   int64_t step;     // step index within the launch
   int64_t first;    // index of the segment's first tile within its slot
+  int64_t ctx;      // its step's context in the plan's context array
 };
 
 struct BatchInst {
@@ -1146,7 +1147,7 @@ struct BatchInst {
   int64_t nfrob;
 };
 
-__device__ inline void make_step(const BatchInst& I, int64_t st, StepCtx& P) {
+__host__ __device__ inline void make_step(const BatchInst& I, int64_t st, StepCtx& P) {
   P.a = I.a;
   P.B = I.B;
   P.YA = I.YA;
@@ -1168,31 +1169,36 @@ __device__ inline void make_step(const BatchInst& I, int64_t st, StepCtx& P) {
   P.nfrob = I.nfrob;
 }
 
+// The segment of a slot that holds slot tile t (the last with first <= t). Every lane runs the same
+// search on the same addresses, so it stays in scalar registers.
+__device__ inline int32_t find_seg(const BatchSeg* segs, int32_t s0, int32_t s1, int64_t t) {
+  int32_t lo = s0, hi = s1 - 1;
+  while (lo < hi) {
+    const int32_t mid = (lo + hi + 1) >> 1;
+    if (segs[mid].first <= t) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+// The step contexts are built on the host with the plan, one per (member, step), and read here through
+// a uniform reference into global memory, as the per-phase kernels read theirs from the kernel
+// arguments: in scalar registers. (A context held in LDS is per-lane data to the compiler, which costs
+// the scalar operand bases and doubles the register footprint; one built in registers goes to the
+// stack, as it is indexed at run time. The tile functions are forced inline for the same reason: an
+// out-of-line call passes the context as a per-lane pointer.)
 template <int DMODE, int KMODE, typename TY, typename TPH>
-__global__ __launch_bounds__(kThreads) void stgd_batch_kernel(const BatchInst* insts, const BatchSeg* segs,
+__global__ __launch_bounds__(kThreads) void stgd_batch_kernel(const StepCtx* __restrict__ ctxs, const BatchSeg* __restrict__ segs,
                                                          const int32_t* slot_seg, const int64_t* slot_tiles,
                                                          int32_t nslots) {
   cg::grid_group grid = cg::this_grid();
   __shared__ StgdSmem sm;
-  __shared__ StepCtx P;
-  __shared__ int32_t sidx;
   for (int32_t sl = 0; sl < nslots; ++sl) {
     const int32_t s0 = slot_seg[sl], s1 = slot_seg[sl + 1];
     const int64_t ntiles = slot_tiles[sl];
     for (int64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
-      if (threadIdx.x == 0) {
-        int lo = s0, hi = s1 - 1;  // last segment with first <= t
-        while (lo < hi) {
-          const int mid = (lo + hi + 1) >> 1;
-          if (segs[mid].first <= t) lo = mid;
-          else hi = mid - 1;
-        }
-        sidx = lo;
-        make_step(insts[segs[lo].inst], segs[lo].step, P);
-      }
-      __syncthreads();
-      const BatchSeg& sg = segs[sidx];
-      run_tile<DMODE, KMODE, TY, TPH>(P, sg.phase, sg.it, (int)(t - sg.first), sm);
+      const BatchSeg sg = segs[find_seg(segs, s0, s1, t)];
+      run_tile<DMODE, KMODE, TY, TPH>(ctxs[sg.ctx], sg.phase, sg.it, (int)(t - sg.first), sm);
       __syncthreads();
     }
     grid.sync();
@@ -1202,24 +1208,35 @@ __global__ __launch_bounds__(kThreads) void stgd_batch_kernel(const BatchInst* i
 // One phase slot of the plan as its own launch (blockIdx.x = tile of the slot): the grouped
 // alternative to the persistent kernel, where the kernel boundary orders consecutive slots.
 template <int DMODE, int KMODE, typename TY, typename TPH>
-__global__ __launch_bounds__(kThreads) void stgd_slot_kernel(const BatchInst* insts, const BatchSeg* segs, int32_t s0, int32_t s1) {
+__global__ __launch_bounds__(kThreads) void stgd_slot_kernel(const StepCtx* __restrict__ ctxs, const BatchSeg* __restrict__ segs, int32_t s0, int32_t s1) {
   __shared__ StgdSmem sm;
-  __shared__ StepCtx P;
-  __shared__ int32_t sidx;
   const int64_t t = blockIdx.x;
-  if (threadIdx.x == 0) {
-    int lo = s0, hi = s1 - 1;  // last segment with first <= t
-    while (lo < hi) {
-      const int mid = (lo + hi + 1) >> 1;
-      if (segs[mid].first <= t) lo = mid;
-      else hi = mid - 1;
-    }
-    sidx = lo;
-    make_step(insts[segs[lo].inst], segs[lo].step, P);
-  }
-  __syncthreads();
-  const BatchSeg& sg = segs[sidx];
-  run_tile<DMODE, KMODE, TY, TPH>(P, sg.phase, sg.it, (int)(t - sg.first), sm);
+  const BatchSeg sg = segs[find_seg(segs, s0, s1, t)];
+  run_tile<DMODE, KMODE, TY, TPH>(ctxs[sg.ctx], sg.phase, sg.it, (int)(t - sg.first), sm);
+}
+
+// A slot whose segments are all of one phase, launched with that phase's code only: the generic slot
+// kernel holds every phase and gets the register budget of the largest (one wave per SIMD), this one
+// that of its phase, as the per-phase kernels of single instances.
+template <int DMODE, int KMODE, typename TY, typename TPH, int PHASE>
+__global__ __launch_bounds__(kThreads) void stgd_slot_phase_kernel(const StepCtx* __restrict__ ctxs, const BatchSeg* __restrict__ segs, int32_t s0,
+                                                                  int32_t s1) {
+  __shared__ StgdSmem sm;
+  constexpr bool SPLIT = KMODE == MM_BF16X3;
+  const int64_t t = blockIdx.x;
+  const BatchSeg sg = segs[find_seg(segs, s0, s1, t)];
+  const StepCtx& P = ctxs[sg.ctx];
+  const int tile = (int)(t - sg.first);
+  if constexpr (PHASE == PH_SETUP) copy_tile<SPLIT>(P, true, tile, sm);
+  else if constexpr (PHASE == PH_P1) {
+    p1_tile<DMODE, KMODE, TY, TPH>(P, tile, sm);
+    if (tile == 0) p1_log(P);
+  } else if constexpr (PHASE == PH_P2) p2_tile<DMODE, KMODE, TY, TPH>(P, tile, sm);
+  else if constexpr (PHASE == PH_GRAM) gram_tile<KMODE>(P, sg.it, tile, sm);
+  else if constexpr (PHASE == PH_UPDATE) update_tile<KMODE>(P, sg.it, tile, sm);
+  else if constexpr (PHASE == PH_QPOLY) qpoly_tile<KMODE>(P, sg.it, tile, sm);
+  else if constexpr (PHASE == PH_FROB) frob_final(P);
+  else copy_tile<SPLIT>(P, false, tile, sm);
 }
 
 __global__ void set_i64_kernel(int64_t* p, int64_t v) {
@@ -1380,12 +1397,13 @@ struct StgdBatch {
   int32_t nslots = 0;
   int32_t grid = 0;
   int32_t grid_max = 0;
-  BatchInst* d_inst = nullptr;     // [ninst]
+  StepCtx* d_ctx = nullptr;        // [contexts]: one per (member, step)
   BatchSeg* d_segs = nullptr;      // [segments]
   int32_t* d_slot_seg = nullptr;   // [nslots + 1]: first segment of each slot
   int64_t* d_slot_tiles = nullptr; // [nslots]
   std::vector<int32_t> h_slot_seg;  // host copies for the per-slot launches
   std::vector<int64_t> h_slot_tiles;
+  std::vector<int32_t> h_slot_phase;  // the phase of every segment of the slot, or -1 when they differ
 };
 
 template <int DMODE, int KMODE, typename TY, typename TPH>
@@ -1404,6 +1422,8 @@ int stgd_batch_plan(StgdBatch*& out, int32_t precision, const std::vector<StgdBa
   MMSB_CHECK_ARG(precision == MMSB_PREC_FP32 || precision == MMSB_PREC_BF16, "stgd batch: unknown precision %d", precision);
   std::vector<BatchInst> insts(members.size());
   std::vector<std::vector<BatchSeg>> per(members.size());
+  std::vector<StepCtx> ctxs;                  // one per (member, step)
+  std::vector<int64_t> ctx0(members.size());  // each member's first
   for (size_t i = 0; i < members.size(); ++i) {
     const StgdBatchMember& m = members[i];
     MMSB_TRY(check_args(m.args, ns_iters, ns_degree));
@@ -1424,6 +1444,12 @@ int stgd_batch_plan(StgdBatch*& out, int32_t precision, const std::vector<StgdBa
     I.spe = ceil_div(m.args.n_x, m.args.bl);
     I.frob_log = m.frob_log;
     I.nfrob = m.nsteps;
+    ctx0[i] = (int64_t)ctxs.size();
+    for (int64_t st = 0; st < m.nsteps; ++st) {
+      StepCtx P{};
+      make_step(I, st, P);
+      ctxs.push_back(P);
+    }
     auto add = [&](int phase, int it, int64_t st) {
       BatchSeg sg{};
       sg.inst = (int32_t)i;
@@ -1431,6 +1457,7 @@ int stgd_batch_plan(StgdBatch*& out, int32_t precision, const std::vector<StgdBa
       sg.it = it;
       sg.ntiles = phase_tiles(phase, m.args.kp, m.args.b);
       sg.step = st;
+      sg.ctx = ctx0[i] + st;
       per[i].push_back(sg);
     };
     add(PH_SETUP, 0, 0);
@@ -1447,24 +1474,40 @@ int stgd_batch_plan(StgdBatch*& out, int32_t precision, const std::vector<StgdBa
         add(PH_COPY, 0, st);
       }
     }
-    add(PH_FROB, 0, m.nsteps - 1);
   }
   std::vector<BatchSeg> segs;
   std::vector<int32_t> slot_seg;
   std::vector<int64_t> slot_tiles;
+  std::vector<int32_t> slot_phase;
+  // every member's log entry of its last step in one final slot: nothing writes a member's tile
+  // partials after its last step, and the other slots keep one phase each
+  for (size_t i = 0; i < members.size(); ++i) {
+    BatchSeg sg{};
+    sg.inst = (int32_t)i;
+    sg.phase = PH_FROB;
+    sg.ntiles = 1;
+    sg.step = members[i].nsteps - 1;
+    sg.ctx = ctx0[i] + sg.step;
+    per[i].push_back(sg);
+  }
   size_t maxlen = 0;
-  for (auto& v : per) maxlen = std::max(maxlen, v.size());
-  for (size_t sl = 0; sl < maxlen; ++sl) {
+  for (auto& v : per) maxlen = std::max(maxlen, v.size() - 1);
+  for (size_t sl = 0; sl <= maxlen; ++sl) {
     slot_seg.push_back((int32_t)segs.size());
     int64_t t = 0;
+    int32_t phase = -2;
     for (size_t i = 0; i < per.size(); ++i) {
-      if (sl >= per[i].size()) continue;
-      BatchSeg sg = per[i][sl];
+      const size_t last = per[i].size() - 1;  // the FROB segment
+      const size_t at = sl == maxlen ? last : sl;
+      if (sl < maxlen && sl >= last) continue;
+      BatchSeg sg = per[i][at];
       sg.first = t;
       t += sg.ntiles;
       segs.push_back(sg);
+      phase = phase == -2 ? sg.phase : (phase == sg.phase ? phase : -1);
     }
     slot_tiles.push_back(t);
+    slot_phase.push_back(phase < 0 ? -1 : phase);
   }
   slot_seg.push_back((int32_t)segs.size());
   MMSB_CHECK_ARG(segs.size() < (size_t)INT32_MAX, "stgd batch: too many segments");
@@ -1474,16 +1517,17 @@ int stgd_batch_plan(StgdBatch*& out, int32_t precision, const std::vector<StgdBa
   b->nslots = (int32_t)slot_tiles.size();
   b->h_slot_seg = slot_seg;
   b->h_slot_tiles = slot_tiles;
+  b->h_slot_phase = slot_phase;
   auto fail = [&](hipError_t e) {
     stgd_batch_free(b);
     return set_error(MMSB_ERR_HIP, "stgd batch: %s", hipGetErrorString(e));
   };
   hipError_t e;
-  if ((e = hipMalloc(&b->d_inst, sizeof(BatchInst) * insts.size())) != hipSuccess) return fail(e);
+  if ((e = hipMalloc(&b->d_ctx, sizeof(StepCtx) * ctxs.size())) != hipSuccess) return fail(e);
   if ((e = hipMalloc(&b->d_segs, sizeof(BatchSeg) * segs.size())) != hipSuccess) return fail(e);
   if ((e = hipMalloc(&b->d_slot_seg, sizeof(int32_t) * slot_seg.size())) != hipSuccess) return fail(e);
   if ((e = hipMalloc(&b->d_slot_tiles, sizeof(int64_t) * slot_tiles.size())) != hipSuccess) return fail(e);
-  if ((e = hipMemcpy(b->d_inst, insts.data(), sizeof(BatchInst) * insts.size(), hipMemcpyHostToDevice)) != hipSuccess)
+  if ((e = hipMemcpy(b->d_ctx, ctxs.data(), sizeof(StepCtx) * ctxs.size(), hipMemcpyHostToDevice)) != hipSuccess)
     return fail(e);
   if ((e = hipMemcpy(b->d_segs, segs.data(), sizeof(BatchSeg) * segs.size(), hipMemcpyHostToDevice)) != hipSuccess)
     return fail(e);
@@ -1515,22 +1559,39 @@ int32_t stgd_batch_grid_max(const StgdBatch* b) { return b ? b->grid_max : 0; }
 
 int stgd_batch_launch(hipStream_t s, StgdBatch* b) {
   MMSB_CHECK_ARG(b, "stgd batch: null plan");
-  void* args[] = {&b->d_inst, &b->d_segs, &b->d_slot_seg, &b->d_slot_tiles, &b->nslots};
+  void* args[] = {&b->d_ctx, &b->d_segs, &b->d_slot_seg, &b->d_slot_tiles, &b->nslots};
   MMSB_HIP(hipLaunchCooperativeKernel(batch_kernel(b->precision), dim3(b->grid), dim3(kThreads), args, 0, s));
   return MMSB_OK;
+}
+
+template <int DMODE, int KMODE, typename TY, typename TPH>
+static void launch_slot(hipStream_t s, const StgdBatch* b, int32_t sl) {
+  const dim3 grid((unsigned)b->h_slot_tiles[sl]), block(kThreads);
+  const int32_t s0 = b->h_slot_seg[sl], s1 = b->h_slot_seg[sl + 1];
+  switch (b->h_slot_phase[sl]) {
+#define MMSB_SLOT(PH)                                                                                              \
+  case PH:                                                                                                         \
+    hipLaunchKernelGGL((stgd_slot_phase_kernel<DMODE, KMODE, TY, TPH, PH>), grid, block, 0, s, b->d_ctx, b->d_segs, s0, s1); \
+    break;
+    MMSB_SLOT(PH_SETUP)
+    MMSB_SLOT(PH_P1)
+    MMSB_SLOT(PH_P2)
+    MMSB_SLOT(PH_GRAM)
+    MMSB_SLOT(PH_UPDATE)
+    MMSB_SLOT(PH_COPY)
+    MMSB_SLOT(PH_QPOLY)
+    MMSB_SLOT(PH_FROB)
+#undef MMSB_SLOT
+    default:  // segments of different phases
+      hipLaunchKernelGGL((stgd_slot_kernel<DMODE, KMODE, TY, TPH>), grid, block, 0, s, b->d_ctx, b->d_segs, s0, s1);
+  }
 }
 
 int stgd_batch_launch_slots(hipStream_t s, StgdBatch* b) {
   MMSB_CHECK_ARG(b, "stgd batch: null plan");
   for (int32_t sl = 0; sl < b->nslots; ++sl) {
-    const dim3 grid((unsigned)b->h_slot_tiles[sl]);
-    const int32_t s0 = b->h_slot_seg[sl], s1 = b->h_slot_seg[sl + 1];
-    if (b->precision == MMSB_PREC_BF16)
-      hipLaunchKernelGGL((stgd_slot_kernel<MM_BF16, MM_BF16X3, __bf16, __bf16>), grid, dim3(kThreads), 0, s, b->d_inst, b->d_segs,
-                         s0, s1);
-    else
-      hipLaunchKernelGGL((stgd_slot_kernel<MM_F32, MM_F32, float, float>), grid, dim3(kThreads), 0, s, b->d_inst, b->d_segs, s0,
-                         s1);
+    if (b->precision == MMSB_PREC_BF16) launch_slot<MM_BF16, MM_BF16X3, __bf16, __bf16>(s, b, sl);
+    else launch_slot<MM_F32, MM_F32, float, float>(s, b, sl);
     MMSB_LAUNCH_CHECK();
   }
   return MMSB_OK;
@@ -1538,7 +1599,7 @@ int stgd_batch_launch_slots(hipStream_t s, StgdBatch* b) {
 
 void stgd_batch_free(StgdBatch* b) {
   if (!b) return;
-  (void)hipFree(b->d_inst);
+  (void)hipFree(b->d_ctx);
   (void)hipFree(b->d_segs);
   (void)hipFree(b->d_slot_seg);
   (void)hipFree(b->d_slot_tiles);
