@@ -135,7 +135,9 @@ class WhitenConfig:
 
 
 # ---- whitening (pre_proc.c:112-372 / CPU second_whiten Util.cpp:38-127) ----------------------
-def whiten(G_XA: Csr, G_XB: Csr, G_XC: Csr, cfg: WhitenConfig) -> dict:
+def whiten(G_XA: Csr, G_XB: Csr, G_XC: Csr, cfg: WhitenConfig, progress=None) -> dict:
+    """progress(stage), when given, is called after each stage (long runs report where they are)."""
+    say = progress or (lambda _msg: None)
     nx = G_XA.n_rows
     k, r = cfg.k, cfg.k + cfg.oversample
     inv = 1.0 / nx
@@ -149,10 +151,13 @@ def whiten(G_XA: Csr, G_XB: Csr, G_XC: Csr, cfg: WhitenConfig) -> dict:
     # randomized SVD of P_BC (replaces pinv_nys_asym, gemm_pinv_nys.c:325-421)
     Om = gaussian_matrix(cfg.seed, 0, G_XC.n_cols, r, cfg.fp32_omega)
     Q, _ = cholqr2(P_BC(Om))
+    say("whiten: P_BC sketch")
     for _ in range(cfg.power_iters):
         Qz, _ = cholqr2(P_CB(Q))
         Q, _ = cholqr2(P_BC(Qz))
+        say("whiten: P_BC power iteration")
     Q2, R2 = cholqr2(P_CB(Q))                          # P_CB Q = Q2 R2  =>  P_BC ~ Q R2^T Q2^T
+    say("whiten: P_BC factors")
     Ut, sig, Vth = np.linalg.svd(R2.T)
     U = Q @ Ut[:, :k]
     V = Q2 @ Vth.T[:, :k]
@@ -163,7 +168,9 @@ def whiten(G_XA: Csr, G_XB: Csr, G_XC: Csr, cfg: WhitenConfig) -> dict:
 
     # Z_B = P_AC P_BC^+ = L_B U^T, Z_C = P_AB P_CB^+ = L_C V^T   (pre_proc.c:215-218,269)
     L_B = T_XA.mul(G_XC.mul(V), inv) * sinv
+    say("whiten: L_B")
     L_C = T_XA.mul(G_XB.mul(U), inv) * sinv
+    say("whiten: L_C")
 
     # M2 as an operator on n_A-vectors (pre_proc.c:292-302; Util.cpp:81-113), symmetrised
     if cfg.m2_scaling == "paper":
@@ -182,6 +189,7 @@ def whiten(G_XA: Csr, G_XB: Csr, G_XC: Csr, cfg: WhitenConfig) -> dict:
     Qm, _ = cholqr2(M2(Om2))
     for _ in range(cfg.power_iters):
         Qm, _ = cholqr2(M2(Qm))
+    say("whiten: M2 range")
     S = Qm.T @ M2(Qm)
     S = 0.5 * (S + S.T)
     lam, E = np.linalg.eigh(S)
