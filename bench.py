@@ -586,8 +586,8 @@ def converged_comparison(m):
 def cpu_e2e_sample(m, w, ns, gpu_e2e_s, n, g=None):
     """The oracle's whole pipeline (FP64 numpy / scipy) and the library's, on the same sample graph of the
     workload family (ns nodes per partition, same k, alpha0, degree distribution and mean degree): a
-    measured CPU / GPU ratio at the sample size, each stage timed. The ratio at the full size is not
-    extrapolated from it."""
+    measured CPU / GPU ratio at the sample size, each stage timed. ns equal to the workload's size
+    runs both on the workload's own graph (minutes of CPU time at 1M nodes per partition)."""
     import numpy as np
     from oracle import core
     p_in, p_out = c3_probs(ns, w["k"], w["mean_degree"]) if "mean_degree" in w else (w["p_in"], w["p_out"])
