@@ -34,15 +34,19 @@ enum { IB = 16, KC = 128 };  /* a task: IB row tiles x one column panel; q in bl
 static void gemm(int64_t m, int64_t n, int64_t p, const double *A, int64_t sai, int64_t sap,
                  const double *B, int64_t ldb, double *C, int64_t ldc)
 {
-    const int64_t mt = (m + MR - 1) / MR, nt = (n + NR - 1) / NR, mb = (mt + IB - 1) / IB;
+    const int64_t mt = (m + MR - 1) / MR, nt = (n + NR - 1) / NR;
+    int64_t ib = IB;                 /* fewer row tiles per task when there are few tasks per thread */
+    while (ib > 1 && ((mt + ib - 1) / ib) * nt < 4 * (int64_t)omp_get_max_threads())
+        ib >>= 1;
+    const int64_t mb = (mt + ib - 1) / ib;
 #pragma omp parallel for collapse(2) schedule(static)
     for (int64_t bi = 0; bi < mb; bi++) {
         for (int64_t tj = 0; tj < nt; tj++) {
             const int64_t j0 = tj * NR, nj = n - j0 < NR ? n - j0 : NR;
-            const int64_t ti_end = (bi + 1) * IB < mt ? (bi + 1) * IB : mt;
+            const int64_t ti_end = (bi + 1) * ib < mt ? (bi + 1) * ib : mt;
             for (int64_t q0 = 0; q0 < p || q0 == 0; q0 += KC) {
                 const int64_t q1 = q0 + KC < p ? q0 + KC : p;
-                for (int64_t ti = bi * IB; ti < ti_end; ti++) {
+                for (int64_t ti = bi * ib; ti < ti_end; ti++) {
                     const int64_t i0 = ti * MR, mi = m - i0 < MR ? m - i0 : MR;
                     if (mi == MR && nj == NR) {
                         v4d acc[MR][2];
@@ -69,14 +73,37 @@ static void gemm(int64_t m, int64_t n, int64_t p, const double *A, int64_t sai, 
                             *(v4du *)(C + (i0 + r) * ldc + j0) = acc[r][0];
                             *(v4du *)(C + (i0 + r) * ldc + j0 + 4) = acc[r][1];
                         }
-                    } else {
-                        for (int64_t r = 0; r < mi; r++)
-                            for (int64_t c = 0; c < nj; c++) {
-                                double s = q0 == 0 ? 0.0 : C[(i0 + r) * ldc + j0 + c];
-                                for (int64_t q = q0; q < q1; q++)
-                                    s += A[(i0 + r) * sai + q * sap] * B[q * ldb + j0 + c];
-                                C[(i0 + r) * ldc + j0 + c] = s;
+                    } else {          /* an edge tile: the same FMA chains on zero-padded operands */
+                        double cb[MR][NR];
+                        for (int r = 0; r < MR; r++)
+                            for (int c = 0; c < NR; c++)
+                                cb[r][c] = (q0 > 0 && r < mi && c < nj) ? C[(i0 + r) * ldc + j0 + c] : 0.0;
+                        v4d acc[MR][2];
+                        for (int r = 0; r < MR; r++) {
+                            acc[r][0] = *(const v4du *)&cb[r][0];
+                            acc[r][1] = *(const v4du *)&cb[r][4];
+                        }
+                        const double *a0 = A + i0 * sai;
+                        for (int64_t q = q0; q < q1; q++) {
+                            const double *br = B + q * ldb + j0;
+                            double bt[NR] = {0.0};
+                            for (int64_t c = 0; c < nj; c++)
+                                bt[c] = br[c];
+                            const v4d b0 = *(const v4du *)bt, b1 = *(const v4du *)(bt + 4);
+                            const double *aq = a0 + q * sap;
+                            for (int r = 0; r < MR; r++) {
+                                const double av = r < mi ? aq[r * sai] : 0.0;
+                                acc[r][0] += av * b0;
+                                acc[r][1] += av * b1;
                             }
+                        }
+                        for (int r = 0; r < MR; r++) {
+                            *(v4du *)&cb[r][0] = acc[r][0];
+                            *(v4du *)&cb[r][4] = acc[r][1];
+                        }
+                        for (int64_t r = 0; r < mi; r++)
+                            for (int64_t c = 0; c < nj; c++)
+                                C[(i0 + r) * ldc + j0 + c] = cb[r][c];
                     }
                 }
                 if (q1 >= p)
@@ -84,6 +111,18 @@ static void gemm(int64_t m, int64_t n, int64_t p, const double *A, int64_t sai, 
             }
         }
     }
+}
+
+/* C (n x n') = A^T B for row-major A (p x n) and B (p x n'): A^T is first laid out row-major in
+ * at (n x p), so the product runs with unit-stride rows of A^T; the FMA chains are those of gemm. */
+static void gemm_tn(int64_t n, int64_t n2, int64_t p, const double *A, int64_t lda, const double *B, int64_t ldb,
+                    double *C, int64_t ldc, double *at)
+{
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < n; i++)
+        for (int64_t q = 0; q < p; q++)
+            at[i * p + q] = A[q * lda + i];
+    gemm(n, n2, p, at, p, 1, B, ldb, C, ldc);
 }
 
 /* y = x^T M for a length-k vector x and a k x k row-major M. */
@@ -100,11 +139,11 @@ static void vecmat(int32_t k, const double *x, const double *M, double *y)
 
 /* X <- polar-factor step(s) of X (k x k) by guarded Newton-Schulz iterations. */
 static void renorm_ns(int32_t k, double *X, int32_t iters, int32_t degree, double *G, double *G2, double *M,
-                      double *tmp)
+                      double *tmp, double *at)
 {
     const int64_t kk = (int64_t)k * k;
     for (int32_t it = 0; it < iters; it++) {
-        gemm(k, k, k, X, 1, k, X, k, G, k);                  /* G = X^T X */
+        gemm_tn(k, k, k, X, k, X, k, G, k, at);              /* G = X^T X */
         double ginf = 0.0;
         for (int32_t i = 0; i < k; i++) {                     /* ||G||_inf: largest absolute row sum */
             double s = 0.0;
@@ -153,12 +192,13 @@ int stgd_cpu_f64(const double *ya, const double *yb, const double *yc, int64_t n
     const int64_t kk = (int64_t)k * k;
     const int64_t mb = batch < n ? batch : n;
     const int64_t nbatches = (n + batch - 1) / batch;
-    double *buf = malloc(sizeof(double) * (size_t)(3 * mb * k + 7 * kk + 4 * (int64_t)k));
+    const int64_t mk = (mb > k ? mb : k) * k;
+    double *buf = malloc(sizeof(double) * (size_t)(3 * mb * k + 7 * kk + 4 * (int64_t)k + mk));
     if (!buf)
         return -2;
     double *A = buf, *B = A + mb * k, *CG = B + mb * k;
     double *G = CG + mb * k, *H = G + kk, *T = H + kk, *NEW = T + kk, *G2 = NEW + kk, *M = G2 + kk, *TMP = M + kk;
-    double *ma = TMP + kk, *mbv = ma + k, *mab = mbv + k, *cmu = mab + k;
+    double *ma = TMP + kk, *mbv = ma + k, *mab = mbv + k, *cmu = mab + k, *AT = cmu + k;
     double ph[4] = {0, 0, 0, 0};
 
     for (int64_t s = 0; s < steps; s++) {
@@ -176,12 +216,16 @@ int stgd_cpu_f64(const double *ya, const double *yb, const double *yc, int64_t n
         vecmat(k, mu_a, phi, ma);                            /* mu_A^T Phi */
         vecmat(k, mu_b, phi, mbv);                           /* mu_B^T Phi */
 #pragma omp parallel for schedule(static)
-        for (int32_t j = 0; j < k; j++) {                    /* column means of a o b, row order */
-            double s2 = 0.0;
+        for (int32_t j0 = 0; j0 < k; j0 += NR) {             /* column means of a o b, row order */
+            const int32_t j1 = j0 + NR < k ? j0 + NR : k;
+            double s2[NR] = {0.0};
             for (int64_t r = 0; r < m; r++)
-                s2 += A[r * k + j] * B[r * k + j];
-            mab[j] = s2 * inv_m;
-            cmu[j] = c0 * ma[j] * mbv[j] + c1 * mab[j];
+                for (int32_t j = j0; j < j1; j++)
+                    s2[j - j0] += A[r * k + j] * B[r * k + j];
+            for (int32_t j = j0; j < j1; j++) {
+                mab[j] = s2[j - j0] * inv_m;
+                cmu[j] = c0 * ma[j] * mbv[j] + c1 * mab[j];
+            }
         }
 #pragma omp parallel for schedule(static)
         for (int64_t r = 0; r < m; r++)                      /* c_g = c3 a o b + c1 (a o mu_B Phi + mu_A Phi o b) */
@@ -189,14 +233,14 @@ int stgd_cpu_f64(const double *ya, const double *yb, const double *yc, int64_t n
                 const double a = A[r * k + j], b = B[r * k + j];
                 CG[r * k + j] = c3 * a * b + c1 * (a * mbv[j] + ma[j] * b);
             }
-        gemm(k, k, m, yc + lo * k, 1, k, CG, k, T, k);       /* Y_C[lo:hi]^T c_g */
+        gemm_tn(k, k, m, yc + lo * k, k, CG, k, T, k, AT);   /* Y_C[lo:hi]^T c_g */
         for (int32_t i = 0; i < k; i++)
             for (int32_t j = 0; j < k; j++)
                 T[(int64_t)i * k + j] = T[(int64_t)i * k + j] * inv_m + mu_c[i] * cmu[j];
         double t2 = omp_get_wtime();
         ph[1] += t2 - t1;
 
-        gemm(k, k, k, phi, 1, k, phi, k, G, k);              /* G = Phi^T Phi */
+        gemm_tn(k, k, k, phi, k, phi, k, G, k, AT);          /* G = Phi^T Phi */
         for (int64_t e = 0; e < kk; e++)
             H[e] = G[e] * G[e];
         gemm(k, k, k, phi, k, 1, H, k, TMP, k);              /* Phi (Phi^T Phi)^o2 */
@@ -205,7 +249,7 @@ int stgd_cpu_f64(const double *ya, const double *yb, const double *yc, int64_t n
         double t3 = omp_get_wtime();
         ph[2] += t3 - t2;
 
-        renorm_ns(k, NEW, ns_iters, ns_degree, G, G2, M, TMP);
+        renorm_ns(k, NEW, ns_iters, ns_degree, G, G2, M, TMP, AT);
         double d2 = 0.0;
         for (int64_t e = 0; e < kk; e++) {
             const double d = NEW[e] - phi[e];
