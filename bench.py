@@ -395,7 +395,7 @@ def column_match(phi, ref):
     return perm, sign
 
 
-def accuracy_vs_oracle(m, p, phi0, n, k, b, w, spe, epochs, truth=None):
+def accuracy_vs_oracle(m, p, phi0, n, k, b, w, spe, epochs, truth=None, time_c=True):
     """From the same Phi0 and whitened vectors: `epochs` epochs of the FP64 oracle (numpy), of FP32
     STGD, of BF16 STGD, and of BF16 STGD whose last epoch is an FP32 polish; relative Frobenius
     errors of Phi, of the raw Pi_X (linear in Phi) and of the normalised memberships Pi_X-hat
@@ -410,7 +410,7 @@ def accuracy_vs_oracle(m, p, phi0, n, k, b, w, spe, epochs, truth=None):
                          ns_iters=2, ns_degree=3)
     t_oracle = time.time() - t0
     log(f"oracle (numpy) {epochs} epoch(s): {t_oracle:.1f}s")
-    c_port = c_restatement(ya, yb, yc, mus, phi0, phi64, w, b, epochs)
+    c_port = c_restatement(ya, yb, yc, mus, phi0, phi64, w, b, epochs) if time_c else None
     del ya, yb, yc
     runs = {}
     for name, plan in (("fp32", ["fp32"] * epochs), ("bf16", ["bf16"] * epochs),
@@ -571,7 +571,7 @@ def converged_comparison(m):
                 p.stgd(spe, prec)
             m.check(m.lib.mmsb_sync(p.ctx))
             ms[prec] = 1e3 * (time.time() - ts) / 3
-        acc, _ = accuracy_vs_oracle(m, p, phi0, n, k, b, w, spe, epochs, truth=truth)
+        acc, _ = accuracy_vs_oracle(m, p, phi0, n, k, b, w, spe, epochs, truth=truth, time_c=False)
         p.close()
         t_fp32 = epochs * ms["fp32"]
         t_mixed = (epochs - 1) * ms["bf16"] + ms["fp32"]
