@@ -71,3 +71,13 @@ def test_c_restatement_fixed_steps_and_rejects_bad_arguments(cstgd):
         cstgd.stgd(*Y, *mu, phi0, alpha0=1.0, eta=0.1, batch=0)
     with pytest.raises(ValueError):
         cstgd.stgd(Y[0][:, :5], Y[1], Y[2], *mu, phi0, alpha0=1.0, eta=0.1, batch=16)
+
+
+def test_pipeline_timing_script_c1(cstgd):
+    """The per-phase CPU timing of the restatements (oracle/time_pipeline.py) at C1: every phase is
+    timed and the two STGD restatements agree on Phi after C1's 200 epochs."""
+    from oracle import time_pipeline
+    out = time_pipeline.run("c1", 1)
+    assert {"whiten", "yvectors", "stgd_numpy", "stgd_c_openmp", "eigen_post"} <= set(out["phases_s"])
+    assert out["stgd_c_vs_numpy_max_abs"] < 1e-10
+    assert out["end_to_end_s"] > 0 and out["nnz_xa"] > 0
