@@ -161,6 +161,10 @@ def main():
     t_dl = time.time() - t2 - t_post
     log(f"end-to-end {t_e2e:.2f}s (upload {t_up:.2f}, whiten {t_wh:.2f}, stgd {t_st:.2f}, post {t_post:.2f}, "
         f"download {t_dl:.2f})")
+    # the same end to end again in this process: the first run above also pays the HIP runtime's
+    # start-up and the library's one-time set-up (pinned staging, code objects), a later graph does not
+    warm = e2e_again(m, cfg, g, args.precision)
+    log(f"end-to-end, second graph in the process {warm['end_to_end_s']:.2f}s {warm['stages_s']}")
     recovery = None
     if truth_x is not None:
         recovery = {"workload": matched_recovery(pin[0], truth_x)}
@@ -251,6 +255,8 @@ def main():
                    "end_to_end_schedule": f"{cfg.epochs - cfg.polish_epochs} {args.precision} "
                    f"epoch(s) + {cfg.polish_epochs} FP32 polishing epoch(s)", "parallelism": "none"},
         "end_to_end_s": t_e2e,
+        "end_to_end_second_run_s": warm["end_to_end_s"],
+        "stages_second_run_s": warm["stages_s"],
         "stages_s": {"upload_and_transpose": t_up, "whiten": t_wh, "stgd_schedule": t_st, "eigen_post": t_post,
                      "download_pi": t_dl, "generate": t_gen},
         "diagnostics": diag,
@@ -278,6 +284,30 @@ def main():
     if not args.no_variants:
         out["stgd_variants"] = stgd_variants(m, g, w, n, k, b, ns, args)
     print(json.dumps(out), flush=True)
+
+
+def e2e_again(m, cfg, g, precision):
+    """The end-to-end schedule on graph g through a fresh pipeline, in a process that has run it once."""
+    t0 = time.time()
+    p = m.Pipeline(cfg)
+    p.set_graph(g)
+    t_up = time.time() - t0
+    p.whiten()
+    t_wh = time.time() - t0 - t_up
+    p.stgd(0, "fp32")
+    t1 = time.time()
+    p.stgd(-1, precision)
+    t_st = time.time() - t1
+    t2 = time.time()
+    p.eigen()
+    p.post()
+    t_post = time.time() - t2
+    _ = [p.get_f32("pin_" + s_) for s_ in "xabc"]
+    t_e2e = time.time() - t0
+    p.close()
+    return {"end_to_end_s": t_e2e, "stages_s": {"upload_and_transpose": round(t_up, 4), "whiten": round(t_wh, 4),
+                                                 "stgd_schedule": round(t_st, 4), "eigen_post": round(t_post, 4),
+                                                 "download_pi": round(t_e2e - (t2 - t0) - t_post, 4)}}
 
 
 def stgd_rate(m, g, w, n, k, b, ns, precs, epochs):
