@@ -700,7 +700,21 @@ def spmm_roofline(m, blk, r):
     nnz = colidx.size
     bytes_alg = 4 * (nr + 1) + 4 * nnz + 4 * r * nnz + 4 * r * nr   # binary CSR: no value array
     gbs = bytes_alg / (ms * 1e-3) / 1e9
+    # a streaming reference measured on this part: an elementwise negate, 2 GiB read + 2 GiB written
+    src = torch.ones(1 << 29, device=dev, dtype=torch.float32)
+    dst = torch.empty_like(src)
+    for _ in range(2):
+        torch.neg(src, out=dst)
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    ev0.record(stream)
+    for _ in range(10):
+        torch.neg(src, out=dst)
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    stream_gbs = 2 * src.numel() * 4 / (ev0.elapsed_time(ev1) / 10 * 1e-3) / 1e9
+    del src, dst
     return {"bound": "hbm", "achieved": gbs, "peak": PEAK["hbm"] / 1e9, "unit": "GB/s", "frac": gbs * 1e9 / PEAK["hbm"],
+            "stream_negate_gbs": stream_gbs, "frac_of_stream_negate": gbs / stream_gbs,
             "ms": ms, "rows": nr, "nnz": int(nnz), "r": r, "bytes_per_launch": bytes_alg,
             "row_order": "longest rows first (mmsb_csr_row_order), as in the pipeline",
             "ms_natural_row_order": ms_natural,
