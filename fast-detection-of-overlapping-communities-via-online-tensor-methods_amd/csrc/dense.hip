@@ -123,6 +123,38 @@ __global__ void tn_reduce_kernel(int64_t pq, int32_t slices, const double* __res
   C[t] = (beta == 0.0) ? alpha * acc : alpha * acc + beta * C[t];
 }
 
+// ---- the vector shapes over many rows: C[1 x m] = v^T M (p = 1) or C[m x 1] = M^T v (q = 1) ----------
+// One thread per output column walks a slice of the rows (M's rows read coalesced, v[i] the same for
+// the whole wave) with four independent FMA chains; tn_reduce_kernel then sums the slices in a fixed
+// order. (The tile kernel below would stage 64-column slabs of which one column is used.)
+constexpr int64_t kVecMinRows = 65536;
+
+static int tn_vec_slices(int64_t n, int32_t m) {
+  const int64_t blocks = ceil_div(m, 256);
+  int64_t s = ceil_div(2048, blocks);                // aim for ~2048 workgroups
+  s = std::min<int64_t>(s, std::max<int64_t>(1, n / 256));
+  return (int)std::max<int64_t>(1, std::min<int64_t>(s, 1024));
+}
+
+template <typename TM, typename TV>
+__global__ __launch_bounds__(256) void tn_vec_kernel(int64_t n, int32_t m, const TM* __restrict__ M, int32_t ldm,
+                                                     const TV* __restrict__ v, int32_t incv, int32_t slices,
+                                                     double* __restrict__ work) {
+  const int32_t j = blockIdx.x * 256 + threadIdx.x;
+  if (j >= m) return;
+  const int64_t r0 = n * blockIdx.y / slices, r1 = n * (blockIdx.y + 1) / slices;
+  double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+  int64_t i = r0;
+  for (; i + 4 <= r1; i += 4) {
+    a0 = fma((double)v[i * incv], (double)M[i * ldm + j], a0);
+    a1 = fma((double)v[(i + 1) * incv], (double)M[(i + 1) * ldm + j], a1);
+    a2 = fma((double)v[(i + 2) * incv], (double)M[(i + 2) * ldm + j], a2);
+    a3 = fma((double)v[(i + 3) * incv], (double)M[(i + 3) * ldm + j], a3);
+  }
+  for (; i < r1; ++i) a0 = fma((double)v[i * incv], (double)M[i * ldm + j], a0);
+  work[(int64_t)blockIdx.y * m + j] = (a0 + a1) + (a2 + a3);
+}
+
 static int tn_slices(int64_t n, int32_t p, int32_t q) {
   const int64_t tiles = ceil_div(p, 64) * ceil_div(q, 64);
   int64_t s = ceil_div(2048, tiles);                 // aim for ~2048 workgroups
@@ -161,6 +193,9 @@ static int tn_mfma_slices(int64_t n, int32_t p, int32_t q, bool sym) {
 
 size_t tn_workspace_bytes(int64_t n, int32_t p, int32_t q) {
   size_t b = sizeof(double) * (size_t)tn_slices(n, p, q) * (size_t)p * (size_t)q;
+  if ((p == 1 || q == 1) && n >= kVecMinRows)  // bounded by the slice cap, so that a narrower call fits too
+    b = std::max(b, sizeof(double) * (size_t)std::min<int64_t>(1024, std::max<int64_t>(1, n / 256)) * (size_t)p *
+                        (size_t)q);
   if (use_mfma(n, p, q))
     for (bool sym : {false, true})
       b = std::max(b, sizeof(double) * (size_t)tn_mfma_slices(n, p, q, sym) * (size_t)p * (size_t)q);
@@ -278,6 +313,20 @@ int gemm_tn(hipStream_t s, int64_t n, int32_t p, int32_t q, double alpha, const 
             int32_t ldb, double beta, double* C, double* work) {
   MMSB_CHECK_ARG(n >= 0 && p >= 1 && q >= 1 && lda >= p && ldb >= q, "gemm_tn: bad shape");
   const int64_t pq = (int64_t)p * q;
+  if ((p == 1 || q == 1) && n >= kVecMinRows) {
+    const int32_t m = p == 1 ? q : p;
+    const int slices = tn_vec_slices(n, m);
+    const dim3 grid((unsigned)ceil_div(m, 256), (unsigned)slices);
+    if (p == 1)  // C = A^T B with A a column: v = A, M = B
+      hipLaunchKernelGGL((tn_vec_kernel<TB, TA>), grid, dim3(256), 0, s, n, m, B, ldb, A, lda, slices, work);
+    else         // q == 1: v = B, M = A
+      hipLaunchKernelGGL((tn_vec_kernel<TA, TB>), grid, dim3(256), 0, s, n, m, A, lda, B, ldb, slices, work);
+    MMSB_LAUNCH_CHECK();
+    hipLaunchKernelGGL(tn_reduce_kernel, dim3((unsigned)ceil_div(m, 256)), dim3(256), 0, s, (int64_t)m, slices, work,
+                       alpha, beta, C);
+    MMSB_LAUNCH_CHECK();
+    return MMSB_OK;
+  }
   if (use_mfma(n, p, q)) {
     const bool sym = std::is_same<TA, TB>::value && (const void*)A == (const void*)B && lda == ldb && p == q;
     const int slices = tn_mfma_slices(n, p, q, sym);
